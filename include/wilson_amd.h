@@ -1,0 +1,159 @@
+/*
+ * wilson_amd.h -- C ABI of the MI355X (gfx950) Wilson-fermion operator and
+ * normal-equation CG library (libwilson_amd.so).
+ *
+ * The entry points replace the operator/solver calls of the reference theory
+ * object (reference paths relative to the jaxQFT tree):
+ *
+ *   wl_apply(WL_OP_D)      <- SU3WilsonNf2.apply_D      jaxqft/models/su3_wilson_nf2.py:1036-1041
+ *                             U1WilsonNf2.apply_D       jaxqft/models/u1_wilson_nf2.py:526-540
+ *   wl_apply(WL_OP_DDAG)   <- apply_Ddag                su3_wilson_nf2.py:1043-1045, u1_wilson_nf2.py:542-553
+ *   wl_apply(WL_OP_DDAGD)  <- apply_normal              su3_wilson_nf2.py:1101-1105, u1_wilson_nf2.py:634-635
+ *   wl_apply(WL_OP_G5D)    <- gamma5 . apply_D          su3_wilson_nf2.py:1688-1690, :2044-2050
+ *   wl_apply(WL_OP_K/KDAG) <- apply_offdiag[_dagger]    su3_wilson_nf2.py:1047-1055
+ *   wl_cg_normal           <- solve_normal[_with_guess] su3_wilson_nf2.py:1410-1432 (+ _solve_linear :1243-1269)
+ *                             U1WilsonNf2.solve_normal  u1_wilson_nf2.py:873-883
+ *   wl_solve_stats.true_resid <- normal_residual_stats  su3_wilson_nf2.py:1456-1466
+ *   wl_gauge_upload        <- _links_with_fermion_bc    su3_wilson_nf2.py:751-759 (phases are applied in-kernel)
+ *
+ * Conventions
+ *   - Spinor arrays use the reference layout (cols, L0, L1, L2, L3, 4, Nc),
+ *     complex128 or complex64, C-order, last lattice axis fastest.  Column c
+ *     uses gauge configuration c / nrhs.
+ *   - Links use the reference layouts BMXYIJ (B, 4, L0..L3, Nc, Nc) or BXYMIJ
+ *     (B, L0..L3, 4, Nc, Nc); U(1) drops the (Nc, Nc) axes.  Links are given
+ *     WITHOUT the fermion boundary phase; bc[mu] is applied on the x_mu = L_mu-1
+ *     slice exactly as the reference does.
+ *   - D psi = (m + 4r) psi - 1/2 sum_mu [(r - g_mu) U_mu(x) psi(x+mu)
+ *                                       + (r + g_mu) U_mu(x-mu)^dag psi(x-mu)].
+ *   - All array pointers are device pointers unless stated; buffers are
+ *     caller-owned, gauge and field handles are library-owned.
+ *   - Calls are ordered on the given HIP stream (void* = hipStream_t, NULL =
+ *     default stream).  One context serializes its own calls; separate
+ *     contexts may be used concurrently.
+ *   - Every function returns 0 on success or a WL_ERR_* code; wl_last_error()
+ *     gives the message of the calling thread's last failure.
+ */
+#ifndef WILSON_AMD_H
+#define WILSON_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WL_OK 0
+#define WL_ERR_BAD_ARG 1
+#define WL_ERR_HIP 2
+#define WL_ERR_OOM 3
+#define WL_ERR_UNSUPPORTED 4
+
+/* storage / arithmetic precision of internal fields */
+#define WL_F64 0
+#define WL_F32 1
+#define WL_F16 2 /* FP16 storage with a per-site FP32 scale, FP32 arithmetic */
+
+/* caller array element types */
+#define WL_C128 0
+#define WL_C64 1
+
+/* link layouts (su3_ym.py:195-202) */
+#define WL_LAYOUT_BMXYIJ 0
+#define WL_LAYOUT_BXYMIJ 1
+
+/* operators */
+#define WL_OP_D 0
+#define WL_OP_DDAG 1
+#define WL_OP_G5D 2
+#define WL_OP_K 3
+#define WL_OP_KDAG 4
+#define WL_OP_DDAGD 5
+
+/* CG stopping rule */
+#define WL_STOP_TRUE_RESIDUAL 0 /* ||b - A x|| <= tol ||b||, recomputed in FP64 */
+#define WL_STOP_ITERATED 1      /* reference rule: iterated residual only (App. jax cg) */
+
+/* per-column solve status */
+#define WL_STATUS_RUNNING (-1)
+#define WL_STATUS_CONVERGED 0
+#define WL_STATUS_MAXITER 1
+#define WL_STATUS_NAN 2
+#define WL_STATUS_BREAKDOWN 3
+
+typedef struct wl_ctx wl_ctx;
+typedef struct wl_gauge wl_gauge;
+typedef struct wl_field wl_field;
+
+typedef struct {
+  int nd;    /* must be 4 */
+  int L[4];  /* lattice extents, last axis fastest */
+  int nc;    /* 3 (SU(3)) or 1 (U(1)) */
+  int batch; /* number of gauge configurations B */
+} wl_geom;
+
+typedef struct {
+  double mass;     /* bare mass m (su3_wilson_nf2.py:595) */
+  double wilson_r; /* Wilson r (must be 1: the projector kernels need rank-2 (1 +- g)) */
+  double bc_re[4]; /* fermion boundary phases p_mu (su3_wilson_nf2.py:711-724) */
+  double bc_im[4];
+} wl_params;
+
+typedef struct {
+  double tol;       /* relative residual target (cg_tol) */
+  int maxiter;      /* cg_maxiter */
+  int stop_rule;    /* WL_STOP_* */
+  int inner_prec;   /* WL_F64 (plain CG) / WL_F32 / WL_F16 (reliable-update mixed precision) */
+  int inner_recon;  /* 18 or 12 (12-real link reconstruction) for the inner operator */
+  double delta;     /* reliable-update trigger on the residual drop (mixed precision), e.g. 0.1 */
+  int poll_every;   /* iterations queued between host polls of the convergence flags */
+  int use_graph;    /* capture the poll_every-iteration body as a HIP graph */
+  const double* mass; /* host array of per-column masses, or NULL for the context mass */
+} wl_solve_params;
+
+typedef struct {
+  int iters;            /* CG iterations performed */
+  int status;           /* WL_STATUS_* */
+  int reliable_updates; /* FP64 true-residual recomputations */
+  int pad;
+  double true_resid;    /* ||b - D^dag D x|| / ||b|| in FP64 at the last update */
+  double iter_resid;    /* iterated residual / ||b|| */
+} wl_solve_stats;
+
+int wl_version(void);
+const char* wl_last_error(void);
+
+int wl_ctx_create(const wl_geom* geom, const wl_params* params, wl_ctx** out);
+void wl_ctx_destroy(wl_ctx* ctx);
+int wl_ctx_set_mass(wl_ctx* ctx, double mass);
+
+/* where: 0 = device pointer, 1 = host pointer */
+int wl_gauge_upload(wl_ctx* ctx, const void* U, int layout, int dtype, int where, wl_gauge** out);
+void wl_gauge_release(wl_gauge* g);
+/* prebuild the (prec, recon) encoding of a gauge field (otherwise built on first use) */
+int wl_gauge_prepare(wl_gauge* g, int prec, int recon, void* stream);
+
+/* Operator on reference-layout arrays of ncol = batch * nrhs spinors. */
+int wl_apply(wl_ctx* ctx, wl_gauge* g, int op, const void* in, void* out, int nrhs, int dtype, void* stream);
+
+/* Resident internal-layout fields (structure of arrays, see DESIGN). */
+int wl_field_create(wl_ctx* ctx, int prec, int ncol, wl_field** out);
+void wl_field_destroy(wl_field* f);
+int wl_field_import(wl_field* f, const void* src, int dtype, void* stream);
+int wl_field_export(const wl_field* f, void* dst, int dtype, void* stream);
+size_t wl_field_bytes(const wl_field* f);
+/* out = op(in) with links encoded as (in->prec, recon); in and out share precision */
+int wl_dslash(wl_ctx* ctx, wl_gauge* g, int op, int recon, const wl_field* in, wl_field* out, int nrhs, void* stream);
+
+/* Solve D^dag D x = b per column.  b, x: reference-layout arrays (ncol =
+ * batch * nrhs columns); x0 (nullable) is the initial guess; stats: ncol
+ * entries (host memory).  Returns WL_OK even if some columns did not converge
+ * -- check stats[i].status. */
+int wl_cg_normal(wl_ctx* ctx, wl_gauge* g, const void* b, void* x, const void* x0, int nrhs, int dtype,
+                 const wl_solve_params* sp, wl_solve_stats* stats, void* stream);
+/* Same on resident FP64 fields; x is used as the initial guess when use_x0. */
+int wl_cg_normal_field(wl_ctx* ctx, wl_gauge* g, const wl_field* b, wl_field* x, int use_x0, int nrhs,
+                       const wl_solve_params* sp, wl_solve_stats* stats, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WILSON_AMD_H */

@@ -1,0 +1,262 @@
+"""ctypes binding of libwilson_amd.so (the C ABI declared in include/wilson_amd.h).
+
+The library must be built in-tree (``__graft_entry__.build()`` or
+``python -m jaxqft_amd.build_native``).  There is no CPU fallback: every
+operator call runs the HIP kernels, and a missing library or GPU raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import re
+from ctypes import POINTER, byref, c_double, c_int, c_size_t, c_void_p
+from pathlib import Path
+from typing import List, Optional
+
+import torch  # noqa: F401  (loads torch's HIP runtime first, so the library binds to the same one)
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libwilson_amd.so"
+HEADER_PATH = Path(__file__).resolve().parent.parent / "include" / "wilson_amd.h"
+
+WL_OK, WL_ERR_BAD_ARG, WL_ERR_HIP, WL_ERR_OOM, WL_ERR_UNSUPPORTED = 0, 1, 2, 3, 4
+WL_F64, WL_F32, WL_F16 = 0, 1, 2
+WL_C128, WL_C64 = 0, 1
+WL_LAYOUT_BMXYIJ, WL_LAYOUT_BXYMIJ = 0, 1
+WL_OP_D, WL_OP_DDAG, WL_OP_G5D, WL_OP_K, WL_OP_KDAG, WL_OP_DDAGD = 0, 1, 2, 3, 4, 5
+WL_STOP_TRUE_RESIDUAL, WL_STOP_ITERATED = 0, 1
+STATUS_NAMES = {-1: "running", 0: "converged", 1: "maxiter", 2: "nan", 3: "breakdown"}
+PREC = {"f64": WL_F64, "f32": WL_F32, "f16": WL_F16}
+
+
+class WlGeom(ctypes.Structure):
+    _fields_ = [("nd", c_int), ("L", c_int * 4), ("nc", c_int), ("batch", c_int)]
+
+
+class WlParams(ctypes.Structure):
+    _fields_ = [("mass", c_double), ("wilson_r", c_double), ("bc_re", c_double * 4), ("bc_im", c_double * 4)]
+
+
+class WlSolveParams(ctypes.Structure):
+    _fields_ = [
+        ("tol", c_double),
+        ("maxiter", c_int),
+        ("stop_rule", c_int),
+        ("inner_prec", c_int),
+        ("inner_recon", c_int),
+        ("delta", c_double),
+        ("poll_every", c_int),
+        ("use_graph", c_int),
+        ("mass", POINTER(c_double)),
+    ]
+
+
+class WlSolveStats(ctypes.Structure):
+    _fields_ = [
+        ("iters", c_int),
+        ("status", c_int),
+        ("reliable_updates", c_int),
+        ("pad", c_int),
+        ("true_resid", c_double),
+        ("iter_resid", c_double),
+    ]
+
+
+class NativeError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libwilson_amd error {code}: {msg}")
+        self.code = code
+
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+def declared_symbols() -> List[str]:
+    """Function names declared in include/wilson_amd.h."""
+    txt = HEADER_PATH.read_text()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(wl_[a-z0-9_]+)\s*\(", txt)))
+
+
+def load() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'` "
+            "(the Wilson operator has no CPU fallback)"
+        )
+    lib = ctypes.CDLL(str(LIB_PATH))
+    vp = c_void_p
+    sig = {
+        "wl_version": (c_int, []),
+        "wl_last_error": (ctypes.c_char_p, []),
+        "wl_ctx_create": (c_int, [POINTER(WlGeom), POINTER(WlParams), POINTER(vp)]),
+        "wl_ctx_destroy": (None, [vp]),
+        "wl_ctx_set_mass": (c_int, [vp, c_double]),
+        "wl_gauge_upload": (c_int, [vp, vp, c_int, c_int, c_int, POINTER(vp)]),
+        "wl_gauge_release": (None, [vp]),
+        "wl_gauge_prepare": (c_int, [vp, c_int, c_int, vp]),
+        "wl_apply": (c_int, [vp, vp, c_int, vp, vp, c_int, c_int, vp]),
+        "wl_field_create": (c_int, [vp, c_int, c_int, POINTER(vp)]),
+        "wl_field_destroy": (None, [vp]),
+        "wl_field_import": (c_int, [vp, vp, c_int, vp]),
+        "wl_field_export": (c_int, [vp, vp, c_int, vp]),
+        "wl_field_bytes": (c_size_t, [vp]),
+        "wl_dslash": (c_int, [vp, vp, c_int, c_int, vp, vp, c_int, vp]),
+        "wl_cg_normal": (c_int, [vp, vp, vp, vp, vp, c_int, c_int, POINTER(WlSolveParams), POINTER(WlSolveStats), vp]),
+        "wl_cg_normal_field": (c_int, [vp, vp, vp, vp, c_int, c_int, POINTER(WlSolveParams), POINTER(WlSolveStats), vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int) -> None:
+    if rc != WL_OK:
+        lib = load()
+        msg = lib.wl_last_error()
+        raise NativeError(rc, msg.decode() if msg else "")
+
+
+def require_gpu() -> None:
+    if not torch.cuda.is_available():
+        raise RuntimeError("the native Wilson operator needs a HIP device (no CPU fallback)")
+
+
+def current_stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+class Context:
+    """Owns a wl_ctx (geometry, mass, boundary phases) on the current device."""
+
+    def __init__(self, lattice_shape, nc: int, batch: int, mass: float, wilson_r: float, bc):
+        require_gpu()
+        lib = load()
+        L = tuple(int(v) for v in lattice_shape)
+        if len(L) != 4:
+            raise ValueError(f"the native operator supports Nd=4 lattices only (got {L})")
+        g = WlGeom(4, (c_int * 4)(*L), int(nc), int(batch))
+        bc = [complex(v) for v in bc]
+        p = WlParams(float(mass), float(wilson_r), (c_double * 4)(*[z.real for z in bc]),
+                     (c_double * 4)(*[z.imag for z in bc]))
+        h = c_void_p()
+        check(lib.wl_ctx_create(byref(g), byref(p), byref(h)))
+        self.h = h
+        self.lattice_shape = L
+        self.nc = int(nc)
+        self.batch = int(batch)
+        self.mass = float(mass)
+        self.volume = L[0] * L[1] * L[2] * L[3]
+        self._lib = lib
+
+    def set_mass(self, mass: float) -> None:
+        check(self._lib.wl_ctx_set_mass(self.h, float(mass)))
+        self.mass = float(mass)
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            try:
+                self._lib.wl_ctx_destroy(h)
+            except Exception:
+                pass
+            self.h = None
+
+
+class Gauge:
+    """Device-resident gauge configurations (library-owned handle)."""
+
+    def __init__(self, ctx: Context, U: torch.Tensor, layout: int):
+        if not U.is_cuda:
+            raise ValueError("gauge tensor must be on the GPU")
+        U = U.contiguous()
+        dtype = WL_C128 if U.dtype == torch.complex128 else WL_C64
+        h = c_void_p()
+        check(ctx._lib.wl_gauge_upload(ctx.h, c_void_p(U.data_ptr()), int(layout), dtype, 0, byref(h)))
+        self.h = h
+        self.ctx = ctx
+
+    def prepare(self, prec: int, recon: int) -> None:
+        check(self.ctx._lib.wl_gauge_prepare(self.h, int(prec), int(recon), c_void_p(current_stream())))
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            try:
+                self.ctx._lib.wl_gauge_release(h)
+            except Exception:
+                pass
+            self.h = None
+
+
+class Field:
+    """Internal-layout (structure of arrays) spinor columns on the device."""
+
+    def __init__(self, ctx: Context, prec: int, ncol: int):
+        h = c_void_p()
+        check(ctx._lib.wl_field_create(ctx.h, int(prec), int(ncol), byref(h)))
+        self.h = h
+        self.ctx = ctx
+        self.prec = int(prec)
+        self.ncol = int(ncol)
+
+    @property
+    def nbytes(self) -> int:
+        return int(self.ctx._lib.wl_field_bytes(self.h))
+
+    def import_(self, src: torch.Tensor) -> None:
+        src = src.contiguous()
+        dt = WL_C128 if src.dtype == torch.complex128 else WL_C64
+        check(self.ctx._lib.wl_field_import(self.h, c_void_p(src.data_ptr()), dt, c_void_p(current_stream())))
+
+    def export(self, dst: torch.Tensor) -> None:
+        dt = WL_C128 if dst.dtype == torch.complex128 else WL_C64
+        check(self.ctx._lib.wl_field_export(self.h, c_void_p(dst.data_ptr()), dt, c_void_p(current_stream())))
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            try:
+                self.ctx._lib.wl_field_destroy(h)
+            except Exception:
+                pass
+            self.h = None
+
+
+def solve_params(tol: float, maxiter: int, stop_rule: int = WL_STOP_TRUE_RESIDUAL, inner_prec: int = WL_F64,
+                 inner_recon: int = 18, delta: float = 0.1, poll_every: int = 8, use_graph: bool = False,
+                 masses=None):
+    sp = WlSolveParams()
+    sp.tol = float(tol)
+    sp.maxiter = int(maxiter)
+    sp.stop_rule = int(stop_rule)
+    sp.inner_prec = int(inner_prec)
+    sp.inner_recon = int(inner_recon)
+    sp.delta = float(delta)
+    sp.poll_every = int(poll_every)
+    sp.use_graph = 1 if use_graph else 0
+    keep = None
+    if masses is not None:
+        keep = (c_double * len(masses))(*[float(m) for m in masses])
+        sp.mass = ctypes.cast(keep, POINTER(c_double))
+    else:
+        sp.mass = POINTER(c_double)()
+    return sp, keep
+
+
+def stats_list(stats_arr, n: int):
+    return [
+        {
+            "iters": int(stats_arr[i].iters),
+            "status": STATUS_NAMES.get(int(stats_arr[i].status), str(stats_arr[i].status)),
+            "reliable_updates": int(stats_arr[i].reliable_updates),
+            "true_resid": float(stats_arr[i].true_resid),
+            "iter_resid": float(stats_arr[i].iter_resid),
+        }
+        for i in range(n)
+    ]

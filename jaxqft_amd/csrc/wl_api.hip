@@ -1,0 +1,747 @@
+// Host side of libwilson_amd: contexts, resident gauge fields, internal
+// fields, the operator entry point and the normal-equation CG engine.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/wilson_amd.h"
+#include "wl_device.h"
+#include "wl_kernels.h"
+
+namespace wl {
+hipError_t launch_dslash(int prec, int nc, int recon, int op, int epi, const DslashArgs& a, hipStream_t stream);
+hipError_t launch_norm2(int p, int nc, BlasArgs a, hipStream_t stream);
+hipError_t launch_cg_update(int py, int p, int nc, BlasArgs a, hipStream_t stream);
+hipError_t launch_xpay(int p, int nc, BlasArgs a, hipStream_t stream);
+hipError_t launch_accumulate(int px, int py, int nc, BlasArgs a, hipStream_t stream);
+hipError_t launch_convert(int po, int pi, int nc, BlasArgs a, hipStream_t stream);
+hipError_t launch_finalize_update(ColState* cols, int ncol, hipStream_t stream);
+hipError_t pack_spinor(int prec, int nc, const void* src, int dtype, void* dst, long long V, int ncol,
+                       size_t col_bytes, hipStream_t st);
+hipError_t unpack_spinor(int prec, int nc, const void* src, void* dst, int dtype, long long V, int ncol,
+                         size_t col_bytes, hipStream_t st);
+hipError_t pack_gauge_f64(int nc, const void* src, int dtype, int layout, void* dst, long long V, int ncfg,
+                          size_t cfg_bytes, size_t dir_bytes, hipStream_t st);
+hipError_t convert_gauge(int prec, int nc, int recon, const void* src, void* dst, long long V, int ncfg,
+                         size_t src_cfg_bytes, size_t src_dir_bytes, size_t cfg_bytes, size_t dir_bytes,
+                         hipStream_t st);
+}  // namespace wl
+
+using namespace wl;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                            \
+  do {                                                                                          \
+    hipError_t _e = (expr);                                                                     \
+    if (_e != hipSuccess) return fail(WL_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(_e), __FILE__, __LINE__); \
+  } while (0)
+
+#define CHK(expr)            \
+  do {                       \
+    int _r = (expr);         \
+    if (_r != WL_OK) return _r; \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t n = 0;
+  int ensure(size_t bytes) {
+    if (bytes <= n) return WL_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+      return fail(e == hipErrorOutOfMemory ? WL_ERR_OOM : WL_ERR_HIP, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    }
+    n = bytes;
+    return WL_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+struct wl_ctx {
+  wl_geom geom;
+  wl_params params;
+  Geom g;
+  long long V;
+  int device;
+  std::mutex mu;
+  // workspace
+  DevBuf partials, counters, cols;
+  DevBuf ws[10];
+  ColState* host_cols = nullptr;  // pinned mirror
+  size_t host_cols_n = 0;
+};
+
+struct wl_gauge {
+  wl_ctx* ctx;
+  int ncfg;
+  DevBuf master;  // F64, full links
+  std::map<int, DevBuf> enc;  // key prec*100 + recon
+};
+
+struct wl_field {
+  wl_ctx* ctx;
+  int prec;
+  int ncol;
+  size_t col_bytes;
+  DevBuf buf;
+};
+
+static inline size_t cfg_bytes(const wl_ctx* c, int prec, int recon) {
+  return 4 * link_dir_bytes(prec, c->geom.nc, recon, (size_t)c->V);
+}
+
+static int check_ctx(const wl_ctx* c) {
+  if (!c) return fail(WL_ERR_BAD_ARG, "null context");
+  return WL_OK;
+}
+
+extern "C" {
+
+int wl_version(void) { return 1; }
+
+const char* wl_last_error(void) { return g_err.c_str(); }
+
+int wl_ctx_create(const wl_geom* geom, const wl_params* params, wl_ctx** out) {
+  if (!geom || !params || !out) return fail(WL_ERR_BAD_ARG, "null argument");
+  if (geom->nd != 4) return fail(WL_ERR_UNSUPPORTED, "only Nd=4 lattices are supported (got %d)", geom->nd);
+  if (geom->nc != 3 && geom->nc != 1) return fail(WL_ERR_UNSUPPORTED, "Nc must be 3 or 1 (got %d)", geom->nc);
+  if (geom->batch < 1) return fail(WL_ERR_BAD_ARG, "batch must be >= 1");
+  long long V = 1;
+  for (int i = 0; i < 4; ++i) {
+    if (geom->L[i] < 1) return fail(WL_ERR_BAD_ARG, "lattice extent L[%d]=%d must be >= 1", i, geom->L[i]);
+    V *= geom->L[i];
+  }
+  if (V > (1LL << 31)) return fail(WL_ERR_UNSUPPORTED, "volume %lld too large", V);
+  if (std::fabs(params->wilson_r - 1.0) > 1e-12)
+    return fail(WL_ERR_UNSUPPORTED, "wilson_r=%g: the half-spinor kernels need r=1", params->wilson_r);
+  wl_ctx* c = new wl_ctx();
+  c->geom = *geom;
+  c->params = *params;
+  c->V = V;
+  for (int i = 0; i < 4; ++i) c->g.L[i] = geom->L[i];
+  c->g.stride[3] = 1;
+  c->g.stride[2] = geom->L[3];
+  c->g.stride[1] = geom->L[3] * geom->L[2];
+  c->g.stride[0] = geom->L[3] * geom->L[2] * geom->L[1];
+  c->g.V = V;
+  if (hipGetDevice(&c->device) != hipSuccess) {
+    delete c;
+    return fail(WL_ERR_HIP, "no HIP device");
+  }
+  *out = c;
+  return WL_OK;
+}
+
+void wl_ctx_destroy(wl_ctx* c) {
+  if (!c) return;
+  c->partials.release();
+  c->counters.release();
+  c->cols.release();
+  for (auto& b : c->ws) b.release();
+  if (c->host_cols) (void)hipHostFree(c->host_cols);
+  delete c;
+}
+
+int wl_ctx_set_mass(wl_ctx* c, double mass) {
+  CHK(check_ctx(c));
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->params.mass = mass;
+  return WL_OK;
+}
+
+int wl_gauge_upload(wl_ctx* c, const void* U, int layout, int dtype, int where, wl_gauge** out) {
+  CHK(check_ctx(c));
+  if (!U || !out) return fail(WL_ERR_BAD_ARG, "null argument");
+  if (layout != WL_LAYOUT_BMXYIJ && layout != WL_LAYOUT_BXYMIJ) return fail(WL_ERR_BAD_ARG, "bad layout %d", layout);
+  if (dtype != WL_C128 && dtype != WL_C64) return fail(WL_ERR_BAD_ARG, "bad dtype %d", dtype);
+  std::lock_guard<std::mutex> lk(c->mu);
+  const int nc = c->geom.nc;
+  const int ncfg = c->geom.batch;
+  size_t src_bytes = (size_t)ncfg * 4 * c->V * nc * nc * (dtype == WL_C128 ? 16 : 8);
+  wl_gauge* g = new wl_gauge();
+  g->ctx = c;
+  g->ncfg = ncfg;
+  size_t cb = cfg_bytes(c, F64, 18);
+  int rc = g->master.ensure(cb * ncfg);
+  if (rc) {
+    delete g;
+    return rc;
+  }
+  const void* src = U;
+  DevBuf staging;
+  if (where == 1) {
+    rc = staging.ensure(src_bytes);
+    if (rc) {
+      g->master.release();
+      delete g;
+      return rc;
+    }
+    hipError_t e = hipMemcpy(staging.p, U, src_bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      staging.release();
+      g->master.release();
+      delete g;
+      return fail(WL_ERR_HIP, "gauge upload copy: %s", hipGetErrorString(e));
+    }
+    src = staging.p;
+  }
+  hipError_t e = pack_gauge_f64(nc, src, dtype, layout, g->master.p, c->V, ncfg, cb, cb / 4, 0);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  staging.release();
+  if (e != hipSuccess) {
+    g->master.release();
+    delete g;
+    return fail(WL_ERR_HIP, "gauge pack: %s", hipGetErrorString(e));
+  }
+  *out = g;
+  return WL_OK;
+}
+
+void wl_gauge_release(wl_gauge* g) {
+  if (!g) return;
+  g->master.release();
+  for (auto& kv : g->enc) kv.second.release();
+  delete g;
+}
+
+}  // extern "C"
+
+// returns device pointer of the (prec, recon) encoding, building it if needed
+static int gauge_encoding(wl_gauge* g, int prec, int recon, hipStream_t st, const void** out) {
+  wl_ctx* c = g->ctx;
+  if (c->geom.nc == 1) recon = 18;
+  if (recon != 18 && recon != 12) return fail(WL_ERR_BAD_ARG, "recon must be 18 or 12 (got %d)", recon);
+  if (prec == F64 && recon == 18) {
+    *out = g->master.p;
+    return WL_OK;
+  }
+  int key = prec * 100 + recon;
+  auto it = g->enc.find(key);
+  if (it != g->enc.end()) {
+    *out = it->second.p;
+    return WL_OK;
+  }
+  DevBuf b;
+  size_t cb = cfg_bytes(c, prec, recon);
+  CHK(b.ensure(cb * g->ncfg));
+  size_t mcb = cfg_bytes(c, F64, 18);
+  hipError_t e = convert_gauge(prec, c->geom.nc, recon, g->master.p, b.p, c->V, g->ncfg, mcb, mcb / 4, cb, cb / 4, st);
+  if (e != hipSuccess) {
+    b.release();
+    return fail(WL_ERR_HIP, "gauge convert: %s", hipGetErrorString(e));
+  }
+  g->enc[key] = b;
+  *out = b.p;
+  return WL_OK;
+}
+
+static void fill_dslash_common(const wl_ctx* c, DslashArgs& a) {
+  memset(&a, 0, sizeof(a));
+  a.g = c->g;
+  for (int i = 0; i < 4; ++i) {
+    a.bc_re[i] = c->params.bc_re[i];
+    a.bc_im[i] = c->params.bc_im[i];
+  }
+  a.m0 = c->params.mass + 4.0 * c->params.wilson_r;
+  a.r = c->params.wilson_r;
+  a.ntiles = (int)((c->V + NT - 1) / NT);
+}
+
+// op on internal fields of precision prec (no gating, no reduction)
+static int dslash_internal(wl_ctx* c, wl_gauge* g, int op, int prec, int recon, const void* in, void* out, int ncol,
+                           int nrhs, size_t col_bytes, hipStream_t st) {
+  const void* U;
+  CHK(gauge_encoding(g, prec, recon, st, &U));
+  DslashArgs a;
+  fill_dslash_common(c, a);
+  int rc = (c->geom.nc == 1) ? 18 : recon;
+  a.U = U;
+  a.cfg_bytes = cfg_bytes(c, prec, rc);
+  a.dir_bytes = a.cfg_bytes / 4;
+  a.col_bytes = col_bytes;
+  a.ncol = ncol;
+  a.cols_per_cfg = nrhs;
+  a.in = in;
+  a.out = out;
+  hipError_t e = launch_dslash(prec, c->geom.nc, rc, op, 0, a, st);
+  if (e != hipSuccess) return fail(WL_ERR_HIP, "dslash launch: %s", hipGetErrorString(e));
+  return WL_OK;
+}
+
+static int check_op(int op) {
+  if (op < WL_OP_D || op > WL_OP_DDAGD) return fail(WL_ERR_BAD_ARG, "bad op %d", op);
+  return WL_OK;
+}
+
+static int op_internal(wl_ctx* c, wl_gauge* g, int op, int prec, int recon, const void* in, void* out, void* tmp,
+                       int ncol, int nrhs, size_t col_bytes, hipStream_t st) {
+  if (op == WL_OP_DDAGD) {
+    CHK(dslash_internal(c, g, WL_OP_D, prec, recon, in, tmp, ncol, nrhs, col_bytes, st));
+    return dslash_internal(c, g, WL_OP_DDAG, prec, recon, tmp, out, ncol, nrhs, col_bytes, st);
+  }
+  return dslash_internal(c, g, op, prec, recon, in, out, ncol, nrhs, col_bytes, st);
+}
+
+extern "C" {
+
+int wl_apply(wl_ctx* c, wl_gauge* g, int op, const void* in, void* out, int nrhs, int dtype, void* stream) {
+  CHK(check_ctx(c));
+  if (!g || g->ctx != c) return fail(WL_ERR_BAD_ARG, "gauge handle does not belong to this context");
+  if (!in || !out) return fail(WL_ERR_BAD_ARG, "null field pointer");
+  if (nrhs < 1) return fail(WL_ERR_BAD_ARG, "nrhs must be >= 1");
+  if (dtype != WL_C128 && dtype != WL_C64) return fail(WL_ERR_BAD_ARG, "bad dtype %d", dtype);
+  CHK(check_op(op));
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = (hipStream_t)stream;
+  const int nc = c->geom.nc;
+  const int ncol = c->geom.batch * nrhs;
+  const size_t cb = spinor_col_bytes(F64, nc, (size_t)c->V);
+  CHK(c->ws[0].ensure(cb * ncol));
+  CHK(c->ws[1].ensure(cb * ncol));
+  CHK(c->ws[2].ensure(cb * ncol));
+  hipError_t e = pack_spinor(F64, nc, in, dtype, c->ws[0].p, c->V, ncol, cb, st);
+  if (e != hipSuccess) return fail(WL_ERR_HIP, "pack: %s", hipGetErrorString(e));
+  CHK(op_internal(c, g, op, F64, 18, c->ws[0].p, c->ws[1].p, c->ws[2].p, ncol, nrhs, cb, st));
+  e = unpack_spinor(F64, nc, c->ws[1].p, out, dtype, c->V, ncol, cb, st);
+  if (e != hipSuccess) return fail(WL_ERR_HIP, "unpack: %s", hipGetErrorString(e));
+  return WL_OK;
+}
+
+int wl_gauge_prepare(wl_gauge* g, int prec, int recon, void* stream) {
+  if (!g) return fail(WL_ERR_BAD_ARG, "null gauge");
+  if (prec < F64 || prec > F16) return fail(WL_ERR_BAD_ARG, "bad precision %d", prec);
+  std::lock_guard<std::mutex> lk(g->ctx->mu);
+  const void* U;
+  return gauge_encoding(g, prec, recon, (hipStream_t)stream, &U);
+}
+
+int wl_field_create(wl_ctx* c, int prec, int ncol, wl_field** out) {
+  CHK(check_ctx(c));
+  if (!out || ncol < 1) return fail(WL_ERR_BAD_ARG, "bad field arguments");
+  if (prec < F64 || prec > F16) return fail(WL_ERR_BAD_ARG, "bad precision %d", prec);
+  wl_field* f = new wl_field();
+  f->ctx = c;
+  f->prec = prec;
+  f->ncol = ncol;
+  f->col_bytes = spinor_col_bytes(prec, c->geom.nc, (size_t)c->V);
+  int rc = f->buf.ensure(f->col_bytes * ncol);
+  if (rc) {
+    delete f;
+    return rc;
+  }
+  hipError_t e = hipMemset(f->buf.p, 0, f->col_bytes * ncol);
+  if (e != hipSuccess) {
+    f->buf.release();
+    delete f;
+    return fail(WL_ERR_HIP, "field clear: %s", hipGetErrorString(e));
+  }
+  *out = f;
+  return WL_OK;
+}
+
+void wl_field_destroy(wl_field* f) {
+  if (!f) return;
+  f->buf.release();
+  delete f;
+}
+
+size_t wl_field_bytes(const wl_field* f) { return f ? f->col_bytes * f->ncol : 0; }
+
+int wl_field_import(wl_field* f, const void* src, int dtype, void* stream) {
+  if (!f || !src) return fail(WL_ERR_BAD_ARG, "null argument");
+  if (dtype != WL_C128 && dtype != WL_C64) return fail(WL_ERR_BAD_ARG, "bad dtype %d", dtype);
+  wl_ctx* c = f->ctx;
+  hipError_t e = pack_spinor(f->prec, c->geom.nc, src, dtype, f->buf.p, c->V, f->ncol, f->col_bytes, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(WL_ERR_HIP, "import: %s", hipGetErrorString(e));
+  return WL_OK;
+}
+
+int wl_field_export(const wl_field* f, void* dst, int dtype, void* stream) {
+  if (!f || !dst) return fail(WL_ERR_BAD_ARG, "null argument");
+  if (dtype != WL_C128 && dtype != WL_C64) return fail(WL_ERR_BAD_ARG, "bad dtype %d", dtype);
+  wl_ctx* c = f->ctx;
+  hipError_t e = unpack_spinor(f->prec, c->geom.nc, f->buf.p, dst, dtype, c->V, f->ncol, f->col_bytes, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(WL_ERR_HIP, "export: %s", hipGetErrorString(e));
+  return WL_OK;
+}
+
+int wl_dslash(wl_ctx* c, wl_gauge* g, int op, int recon, const wl_field* in, wl_field* out, int nrhs, void* stream) {
+  CHK(check_ctx(c));
+  if (!g || g->ctx != c || !in || !out || in->ctx != c || out->ctx != c)
+    return fail(WL_ERR_BAD_ARG, "handles do not belong to this context");
+  if (in->prec != out->prec || in->ncol != out->ncol) return fail(WL_ERR_BAD_ARG, "in/out fields differ in precision or columns");
+  if (in == out) return fail(WL_ERR_BAD_ARG, "in and out must differ");
+  if (nrhs < 1 || in->ncol != c->geom.batch * nrhs) return fail(WL_ERR_BAD_ARG, "field has %d columns, expected batch*nrhs", in->ncol);
+  CHK(check_op(op));
+  std::lock_guard<std::mutex> lk(c->mu);
+  void* tmp = nullptr;
+  if (op == WL_OP_DDAGD) {
+    CHK(c->ws[9].ensure(in->col_bytes * in->ncol));
+    tmp = c->ws[9].p;
+  }
+  return op_internal(c, g, op, in->prec, recon, in->buf.p, out->buf.p, tmp, in->ncol, nrhs, in->col_bytes,
+                     (hipStream_t)stream);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// CG on D^dag D with reliable updates
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Solver {
+  wl_ctx* c;
+  wl_gauge* g;
+  hipStream_t st;
+  int nc, ncol, nrhs, ntiles;
+  int P, R;       // inner precision, recon
+  int PY;         // precision of the inner solution accumulator
+  bool mixed;
+  size_t cb64, cbP, cbY;
+  const void* U64;
+  const void* UP;
+  void *b, *x;    // F64 caller-side internal fields
+  void *rhi, *thi;
+  void *r, *p, *Ap, *t, *y;
+  Reduction red;
+
+  void dargs(DslashArgs& a, int prec, const void* U, int recon) {
+    fill_dslash_common(c, a);
+    a.U = U;
+    a.cfg_bytes = cfg_bytes(c, prec, nc == 1 ? 18 : recon);
+    a.dir_bytes = a.cfg_bytes / 4;
+    a.ncol = ncol;
+    a.cols_per_cfg = nrhs;
+    a.m0_from_state = 1;
+    a.red = red;
+  }
+  BlasArgs bargs(int gate) {
+    BlasArgs a;
+    memset(&a, 0, sizeof(a));
+    a.V = c->V;
+    a.ncol = ncol;
+    a.ntiles = ntiles;
+    a.gate = gate;
+    a.red = red;
+    return a;
+  }
+
+  // gated reliable update: x += y; r_hi = b - A x; r = r_hi; p = r + beta p
+  int reliable_update() {
+    hipError_t e;
+    if (mixed) {
+      BlasArgs a = bargs(2);
+      a.a = x; a.col_bytes_a = cb64;
+      a.b = y; a.col_bytes_b = cbY;
+      e = launch_accumulate(F64, PY, nc, a, st);
+      if (e != hipSuccess) return fail(WL_ERR_HIP, "accumulate: %s", hipGetErrorString(e));
+    }
+    DslashArgs d;
+    dargs(d, F64, U64, 18);
+    d.only_update = 1;
+    d.in = x; d.out = thi; d.col_bytes = cb64;
+    e = launch_dslash(F64, nc, 18, WL_OP_D, 0, d, st);
+    if (e != hipSuccess) return fail(WL_ERR_HIP, "dslash: %s", hipGetErrorString(e));
+    d.in = thi; d.out = rhi; d.w = b;
+    d.red.action = RA_TRUERES;
+    e = launch_dslash(F64, nc, 18, WL_OP_DDAG, 2, d, st);
+    if (e != hipSuccess) return fail(WL_ERR_HIP, "dslash: %s", hipGetErrorString(e));
+    if (mixed) {
+      BlasArgs a = bargs(2);
+      a.a = r; a.col_bytes_a = cbP;
+      a.b = rhi; a.col_bytes_b = cb64;
+      e = launch_convert(P, F64, nc, a, st);
+      if (e != hipSuccess) return fail(WL_ERR_HIP, "convert: %s", hipGetErrorString(e));
+    }
+    {
+      BlasArgs a = bargs(2);
+      a.a = p; a.col_bytes_a = cbP;
+      a.b = r; a.col_bytes_b = cbP;
+      e = launch_xpay(P, nc, a, st);
+      if (e != hipSuccess) return fail(WL_ERR_HIP, "xpay: %s", hipGetErrorString(e));
+    }
+    e = launch_finalize_update(red.cols, ncol, st);
+    if (e != hipSuccess) return fail(WL_ERR_HIP, "finalize: %s", hipGetErrorString(e));
+    return WL_OK;
+  }
+
+  int iteration() {
+    hipError_t e;
+    DslashArgs d;
+    dargs(d, P, UP, R);
+    d.skip_inactive = 1;
+    d.in = p; d.out = t; d.col_bytes = cbP;
+    e = launch_dslash(P, nc, nc == 1 ? 18 : R, WL_OP_D, 0, d, st);
+    if (e != hipSuccess) return fail(WL_ERR_HIP, "dslash: %s", hipGetErrorString(e));
+    d.in = t; d.out = Ap; d.w = p;
+    d.red.action = RA_PAP;
+    e = launch_dslash(P, nc, nc == 1 ? 18 : R, WL_OP_DDAG, 1, d, st);
+    if (e != hipSuccess) return fail(WL_ERR_HIP, "dslash: %s", hipGetErrorString(e));
+    {
+      BlasArgs a = bargs(1);
+      a.red.action = RA_ITER;
+      a.a = y; a.col_bytes_a = cbY;
+      a.b = p; a.col_bytes_b = cbP;
+      a.c = r; a.col_bytes_c = cbP;
+      a.d = Ap; a.col_bytes_d = cbP;
+      e = launch_cg_update(PY, P, nc, a, st);
+      if (e != hipSuccess) return fail(WL_ERR_HIP, "cg update: %s", hipGetErrorString(e));
+    }
+    {
+      BlasArgs a = bargs(1);
+      a.a = p; a.col_bytes_a = cbP;
+      a.b = r; a.col_bytes_b = cbP;
+      e = launch_xpay(P, nc, a, st);
+      if (e != hipSuccess) return fail(WL_ERR_HIP, "xpay: %s", hipGetErrorString(e));
+    }
+    return WL_OK;
+  }
+};
+
+}  // namespace
+
+static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int nrhs, const wl_solve_params* sp,
+                     wl_solve_stats* stats, hipStream_t st) {
+  const int nc = c->geom.nc;
+  const int ncol = c->geom.batch * nrhs;
+  if (sp->inner_prec < F64 || sp->inner_prec > F16) return fail(WL_ERR_BAD_ARG, "bad inner precision %d", sp->inner_prec);
+  if (sp->maxiter < 0) return fail(WL_ERR_BAD_ARG, "maxiter must be >= 0");
+  if (!(sp->tol >= 0.0)) return fail(WL_ERR_BAD_ARG, "tol must be >= 0");
+  if (sp->use_graph && st == nullptr) return fail(WL_ERR_BAD_ARG, "graph capture needs a non-default stream");
+  Solver s;
+  s.c = c;
+  s.g = g;
+  s.st = st;
+  s.nc = nc;
+  s.ncol = ncol;
+  s.nrhs = nrhs;
+  s.ntiles = (int)((c->V + NT - 1) / NT);
+  s.P = sp->inner_prec;
+  s.R = (nc == 1) ? 18 : (sp->inner_recon == 12 ? 12 : 18);
+  s.mixed = s.P != F64;
+  s.PY = s.mixed ? F32 : F64;
+  s.cb64 = spinor_col_bytes(F64, nc, (size_t)c->V);
+  s.cbP = spinor_col_bytes(s.P, nc, (size_t)c->V);
+  s.cbY = spinor_col_bytes(s.PY, nc, (size_t)c->V);
+  CHK(gauge_encoding(g, F64, 18, st, &s.U64));
+  CHK(gauge_encoding(g, s.P, s.R, st, &s.UP));
+  s.b = b;
+  s.x = x;
+  // workspace
+  CHK(c->ws[0].ensure(s.cb64 * ncol));  // r_hi
+  CHK(c->ws[1].ensure(s.cb64 * ncol));  // t_hi
+  CHK(c->ws[2].ensure(s.cbP * ncol));   // p
+  CHK(c->ws[3].ensure(s.cbP * ncol));   // Ap
+  CHK(c->ws[4].ensure(s.cbP * ncol));   // t
+  s.rhi = c->ws[0].p;
+  s.thi = c->ws[1].p;
+  s.p = c->ws[2].p;
+  s.Ap = c->ws[3].p;
+  s.t = c->ws[4].p;
+  if (s.mixed) {
+    CHK(c->ws[5].ensure(s.cbP * ncol));  // r
+    CHK(c->ws[6].ensure(s.cbY * ncol));  // y
+    s.r = c->ws[5].p;
+    s.y = c->ws[6].p;
+  } else {
+    s.r = s.rhi;
+    s.y = s.x;
+  }
+  CHK(c->partials.ensure(sizeof(double) * (size_t)ncol * s.ntiles));
+  CHK(c->counters.ensure(sizeof(unsigned) * ncol));
+  CHK(c->cols.ensure(sizeof(ColState) * ncol));
+  if (c->host_cols_n < (size_t)ncol) {
+    if (c->host_cols) (void)hipHostFree(c->host_cols);
+    c->host_cols = nullptr;
+    c->host_cols_n = 0;
+    HIPCHK(hipHostMalloc((void**)&c->host_cols, sizeof(ColState) * ncol, hipHostMallocDefault));
+    c->host_cols_n = ncol;
+  }
+  s.red.partials = (double*)c->partials.p;
+  s.red.counters = (unsigned*)c->counters.p;
+  s.red.cols = (ColState*)c->cols.p;
+  s.red.action = RA_NONE;
+
+  HIPCHK(hipMemsetAsync(c->counters.p, 0, sizeof(unsigned) * ncol, st));
+  HIPCHK(hipMemsetAsync(s.p, 0, s.cbP * ncol, st));
+  if (s.mixed) HIPCHK(hipMemsetAsync(s.y, 0, s.cbY * ncol, st));
+  if (!use_x0) HIPCHK(hipMemsetAsync(x, 0, s.cb64 * ncol, st));
+
+  ColState* hc = c->host_cols;
+  for (int i = 0; i < ncol; ++i) {
+    ColState& cs = hc[i];
+    memset(&cs, 0, sizeof(cs));
+    double m = sp->mass ? sp->mass[i] : c->params.mass;
+    cs.m0 = m + 4.0 * c->params.wilson_r;
+    cs.maxiter = sp->maxiter;
+    cs.mode = sp->stop_rule == WL_STOP_ITERATED ? 1 : 0;
+    cs.delta = s.mixed ? sp->delta : 0.0;
+    cs.need_update = 1;
+    cs.active = 0;
+    cs.status = WL_STATUS_RUNNING;
+  }
+  HIPCHK(hipMemcpyAsync(c->cols.p, hc, sizeof(ColState) * ncol, hipMemcpyHostToDevice, st));
+  {
+    BlasArgs a = s.bargs(0);
+    a.red.action = RA_B2;
+    a.a = b;
+    a.col_bytes_a = s.cb64;
+    hipError_t e = launch_norm2(F64, nc, a, st);
+    if (e != hipSuccess) return fail(WL_ERR_HIP, "norm: %s", hipGetErrorString(e));
+  }
+  HIPCHK(hipMemcpyAsync(hc, c->cols.p, sizeof(ColState) * ncol, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  const double tol2 = sp->tol * sp->tol;
+  for (int i = 0; i < ncol; ++i) {
+    hc[i].tol2b2 = tol2 * hc[i].b2;
+    hc[i].stop2 = tol2 * hc[i].b2;
+  }
+  HIPCHK(hipMemcpyAsync(c->cols.p, hc, sizeof(ColState) * ncol, hipMemcpyHostToDevice, st));
+  CHK(s.reliable_update());
+
+  const int poll = sp->poll_every > 0 ? sp->poll_every : 8;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  if (sp->use_graph) {
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = WL_OK;
+    for (int i = 0; i < poll && rc == WL_OK; ++i) rc = s.iteration();
+    hipError_t e = hipStreamEndCapture(st, &graph);
+    if (rc != WL_OK) return rc;
+    if (e != hipSuccess) return fail(WL_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      return fail(WL_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+    }
+  }
+  int result = WL_OK;
+  long long guard = 0;
+  const long long guard_max = (long long)sp->maxiter / poll + 64LL + 8LL * sp->maxiter;
+  while (true) {
+    HIPCHK(hipMemcpyAsync(hc, c->cols.p, sizeof(ColState) * ncol, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    bool all_done = true, any_update = false, any_active = false;
+    for (int i = 0; i < ncol; ++i) {
+      if (!hc[i].done) all_done = false;
+      if (hc[i].need_update && !hc[i].done) any_update = true;
+      if (hc[i].active) any_active = true;
+    }
+    if (all_done) break;
+    if (++guard > guard_max) {
+      result = fail(WL_ERR_HIP, "solver made no progress");
+      break;
+    }
+    if (any_update) {
+      result = s.reliable_update();
+      if (result != WL_OK) break;
+      continue;
+    }
+    if (!any_active) {
+      result = fail(WL_ERR_HIP, "solver state inconsistent: no active column");
+      break;
+    }
+    if (gexec) {
+      hipError_t e = hipGraphLaunch(gexec, st);
+      if (e != hipSuccess) {
+        result = fail(WL_ERR_HIP, "graph launch: %s", hipGetErrorString(e));
+        break;
+      }
+    } else {
+      for (int i = 0; i < poll && result == WL_OK; ++i) result = s.iteration();
+      if (result != WL_OK) break;
+    }
+  }
+  if (gexec) (void)hipGraphExecDestroy(gexec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (result != WL_OK) return result;
+  if (s.mixed) {
+    // fold any remaining inner correction (columns stopped by NaN/breakdown)
+    BlasArgs a = s.bargs(0);
+    a.a = x; a.col_bytes_a = s.cb64;
+    a.b = s.y; a.col_bytes_b = s.cbY;
+    hipError_t e = launch_accumulate(F64, s.PY, nc, a, st);
+    if (e != hipSuccess) return fail(WL_ERR_HIP, "accumulate: %s", hipGetErrorString(e));
+  }
+  if (stats) {
+    for (int i = 0; i < ncol; ++i) {
+      const ColState& cs = hc[i];
+      double nb = cs.b2 > 0 ? std::sqrt(cs.b2) : 1.0;
+      stats[i].iters = cs.iter;
+      stats[i].status = cs.status;
+      stats[i].reliable_updates = cs.nupdates;
+      stats[i].pad = 0;
+      stats[i].true_resid = cs.b2 > 0 ? std::sqrt(cs.rtrue2) / nb : std::sqrt(cs.rtrue2);
+      stats[i].iter_resid = cs.b2 > 0 ? std::sqrt(cs.rr) / nb : std::sqrt(cs.rr);
+    }
+  }
+  return WL_OK;
+}
+
+extern "C" {
+
+int wl_cg_normal_field(wl_ctx* c, wl_gauge* g, const wl_field* b, wl_field* x, int use_x0, int nrhs,
+                       const wl_solve_params* sp, wl_solve_stats* stats, void* stream) {
+  CHK(check_ctx(c));
+  if (!g || g->ctx != c || !b || !x || !sp) return fail(WL_ERR_BAD_ARG, "null or foreign handle");
+  if (b->prec != F64 || x->prec != F64) return fail(WL_ERR_BAD_ARG, "b and x must be FP64 fields");
+  if (nrhs < 1 || b->ncol != c->geom.batch * nrhs || x->ncol != b->ncol)
+    return fail(WL_ERR_BAD_ARG, "fields must have batch*nrhs columns");
+  std::lock_guard<std::mutex> lk(c->mu);
+  return cg_fields(c, g, b->buf.p, x->buf.p, use_x0, nrhs, sp, stats, (hipStream_t)stream);
+}
+
+int wl_cg_normal(wl_ctx* c, wl_gauge* g, const void* b, void* x, const void* x0, int nrhs, int dtype,
+                 const wl_solve_params* sp, wl_solve_stats* stats, void* stream) {
+  CHK(check_ctx(c));
+  if (!g || g->ctx != c || !b || !x || !sp) return fail(WL_ERR_BAD_ARG, "null or foreign handle");
+  if (dtype != WL_C128 && dtype != WL_C64) return fail(WL_ERR_BAD_ARG, "bad dtype %d", dtype);
+  if (nrhs < 1) return fail(WL_ERR_BAD_ARG, "nrhs must be >= 1");
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = (hipStream_t)stream;
+  const int nc = c->geom.nc;
+  const int ncol = c->geom.batch * nrhs;
+  const size_t cb = spinor_col_bytes(F64, nc, (size_t)c->V);
+  CHK(c->ws[7].ensure(cb * ncol));
+  CHK(c->ws[8].ensure(cb * ncol));
+  hipError_t e = pack_spinor(F64, nc, b, dtype, c->ws[7].p, c->V, ncol, cb, st);
+  if (e != hipSuccess) return fail(WL_ERR_HIP, "pack: %s", hipGetErrorString(e));
+  if (x0) {
+    e = pack_spinor(F64, nc, x0, dtype, c->ws[8].p, c->V, ncol, cb, st);
+    if (e != hipSuccess) return fail(WL_ERR_HIP, "pack: %s", hipGetErrorString(e));
+  }
+  CHK(cg_fields(c, g, c->ws[7].p, c->ws[8].p, x0 != nullptr, nrhs, sp, stats, st));
+  e = unpack_spinor(F64, nc, c->ws[8].p, x, dtype, c->V, ncol, cb, st);
+  if (e != hipSuccess) return fail(WL_ERR_HIP, "unpack: %s", hipGetErrorString(e));
+  return WL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,205 @@
+// Fused BLAS-1 kernels of the CG iteration, one site per thread, all spinor
+// planes of the site per thread (so the FP16 per-site scale is local), FP64
+// accumulation of every norm and dot product.
+#include "wl_kernels.h"
+#include "wl_reduce.h"
+
+namespace wl {
+
+
+__device__ __forceinline__ bool gate_skip(int gate, const ColState& s) {
+  if (gate == 1) return !s.active;
+  if (gate == 2) return !(s.need_update && !s.done);
+  return false;
+}
+
+template <int P, int NC, class T>
+__device__ __forceinline__ void load_as(const void* base, size_t V, size_t site, cx<T> (&v)[4 * NC]) {
+  using A = typename PT<P>::acc;
+  cx<A> t[4 * NC];
+  SpinorIO<P, NC>::load(base, V, site, t);
+#pragma unroll
+  for (int c = 0; c < 4 * NC; ++c) v[c] = {(T)t[c].re, (T)t[c].im};
+}
+
+template <int P, int NC, class T>
+__device__ __forceinline__ void store_as(void* base, size_t V, size_t site, const cx<T> (&v)[4 * NC]) {
+  using A = typename PT<P>::acc;
+  cx<A> t[4 * NC];
+#pragma unroll
+  for (int c = 0; c < 4 * NC; ++c) t[c] = {(A)v[c].re, (A)v[c].im};
+  SpinorIO<P, NC>::store(base, V, site, t);
+}
+
+#define WL_BLAS_PROLOGUE                                            \
+  const int col = blockIdx.x % a.ncol;                              \
+  const int tile = blockIdx.x / a.ncol;                             \
+  if (gate_skip(a.gate, a.red.cols[col])) return;                   \
+  const size_t V = (size_t)a.V;                                     \
+  const size_t site = (size_t)tile * NT + threadIdx.x;              \
+  const bool valid = site < V;                                      \
+  const size_t s = valid ? site : 0;
+
+// |a|^2 -> action
+template <int P, int NC>
+__global__ __launch_bounds__(NT) void k_norm2(BlasArgs a) {
+  WL_BLAS_PROLOGUE
+  using T = typename PT<P>::acc;
+  cx<T> v[4 * NC];
+  load_as<P, NC>((char*)a.a + col * a.col_bytes_a, V, s, v);
+  double part = 0.0;
+#pragma unroll
+  for (int c = 0; c < 4 * NC; ++c) part += (double)v[c].re * v[c].re + (double)v[c].im * v[c].im;
+  finish_reduction(a.red, col, tile, a.ntiles, valid ? part : 0.0);
+}
+
+// y += alpha p ; r -= alpha Ap ; |r|^2 -> RA_ITER.   a=y (PY), b=p, c=r, d=Ap (P)
+template <int PY, int P, int NC>
+__global__ __launch_bounds__(NT) void k_cg_update(BlasArgs a) {
+  WL_BLAS_PROLOGUE
+  using T = typename PT<P>::acc;
+  using TY = typename PT<PY>::acc;
+  const double alpha = a.red.cols[col].alpha;
+  cx<T> p[4 * NC], r[4 * NC], ap[4 * NC];
+  cx<TY> y[4 * NC];
+  load_as<PY, NC>((char*)a.a + col * a.col_bytes_a, V, s, y);
+  load_as<P, NC>((char*)a.b + col * a.col_bytes_b, V, s, p);
+  load_as<P, NC>((char*)a.c + col * a.col_bytes_c, V, s, r);
+  load_as<P, NC>((char*)a.d + col * a.col_bytes_d, V, s, ap);
+  const TY ay = (TY)alpha;
+  const T ar = (T)alpha;
+  double part = 0.0;
+#pragma unroll
+  for (int c = 0; c < 4 * NC; ++c) {
+    y[c].re += ay * (TY)p[c].re;
+    y[c].im += ay * (TY)p[c].im;
+    r[c].re -= ar * ap[c].re;
+    r[c].im -= ar * ap[c].im;
+    part += (double)r[c].re * r[c].re + (double)r[c].im * r[c].im;
+  }
+  if (valid) {
+    store_as<PY, NC>((char*)a.a + col * a.col_bytes_a, V, s, y);
+    store_as<P, NC>((char*)a.c + col * a.col_bytes_c, V, s, r);
+  }
+  finish_reduction(a.red, col, tile, a.ntiles, valid ? part : 0.0);
+}
+
+// p = r + beta p.   a=p, b=r
+template <int P, int NC>
+__global__ __launch_bounds__(NT) void k_xpay(BlasArgs a) {
+  WL_BLAS_PROLOGUE
+  using T = typename PT<P>::acc;
+  const T beta = (T)a.red.cols[col].beta;
+  cx<T> p[4 * NC], r[4 * NC];
+  load_as<P, NC>((char*)a.a + col * a.col_bytes_a, V, s, p);
+  load_as<P, NC>((char*)a.b + col * a.col_bytes_b, V, s, r);
+#pragma unroll
+  for (int c = 0; c < 4 * NC; ++c) {
+    p[c].re = r[c].re + beta * p[c].re;
+    p[c].im = r[c].im + beta * p[c].im;
+  }
+  if (valid) store_as<P, NC>((char*)a.a + col * a.col_bytes_a, V, s, p);
+}
+
+// x += y ; y = 0.   a=x (PX), b=y (PY)
+template <int PX, int PY, int NC>
+__global__ __launch_bounds__(NT) void k_accumulate(BlasArgs a) {
+  WL_BLAS_PROLOGUE
+  cx<double> x[4 * NC], y[4 * NC];
+  load_as<PX, NC>((char*)a.a + col * a.col_bytes_a, V, s, x);
+  load_as<PY, NC>((char*)a.b + col * a.col_bytes_b, V, s, y);
+  cx<double> z[4 * NC];
+#pragma unroll
+  for (int c = 0; c < 4 * NC; ++c) {
+    x[c].re += y[c].re;
+    x[c].im += y[c].im;
+    z[c] = {0.0, 0.0};
+  }
+  if (valid) {
+    store_as<PX, NC>((char*)a.a + col * a.col_bytes_a, V, s, x);
+    store_as<PY, NC>((char*)a.b + col * a.col_bytes_b, V, s, z);
+  }
+}
+
+// out = in (precision conversion).   a=out (PO), b=in (PI)
+template <int PO, int PI, int NC>
+__global__ __launch_bounds__(NT) void k_convert(BlasArgs a) {
+  WL_BLAS_PROLOGUE
+  cx<double> v[4 * NC];
+  load_as<PI, NC>((char*)a.b + col * a.col_bytes_b, V, s, v);
+  if (valid) store_as<PO, NC>((char*)a.a + col * a.col_bytes_a, V, s, v);
+}
+
+// After a reliable update: restart the iterated residual from the true one.
+__global__ void k_finalize_update(ColState* cols, int ncol) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncol) return;
+  ColState& s = cols[c];
+  if (!s.need_update) return;
+  if (!s.done) {
+    s.rr = s.rtrue2;
+    s.r0norm = sqrt(s.rtrue2);
+    s.nupdates += 1;
+    s.active = 1;
+  }
+  s.need_update = 0;
+}
+
+// ---------------------------------------------------------------------------
+// host dispatch
+// ---------------------------------------------------------------------------
+#define WL_DISPATCH_NC(KERNEL, ...)                                                \
+  ((nc == 3 ? KERNEL<__VA_ARGS__, 3><<<grid, block, 0, stream>>>(a)                 \
+            : KERNEL<__VA_ARGS__, 1><<<grid, block, 0, stream>>>(a)),               \
+   hipGetLastError())
+
+static inline dim3 blas_grid(const BlasArgs& a) { return dim3((unsigned)(a.ntiles * a.ncol)); }
+
+hipError_t launch_norm2(int p, int nc, BlasArgs a, hipStream_t stream) {
+  dim3 grid = blas_grid(a), block(NT);
+  if (p == F64) return WL_DISPATCH_NC(k_norm2, F64);
+  if (p == F32) return WL_DISPATCH_NC(k_norm2, F32);
+  return WL_DISPATCH_NC(k_norm2, F16);
+}
+
+hipError_t launch_cg_update(int py, int p, int nc, BlasArgs a, hipStream_t stream) {
+  dim3 grid = blas_grid(a), block(NT);
+  if (py == F64 && p == F64) return WL_DISPATCH_NC(k_cg_update, F64, F64);
+  if (py == F32 && p == F32) return WL_DISPATCH_NC(k_cg_update, F32, F32);
+  if (py == F32 && p == F16) return WL_DISPATCH_NC(k_cg_update, F32, F16);
+  if (py == F64 && p == F32) return WL_DISPATCH_NC(k_cg_update, F64, F32);
+  if (py == F64 && p == F16) return WL_DISPATCH_NC(k_cg_update, F64, F16);
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_xpay(int p, int nc, BlasArgs a, hipStream_t stream) {
+  dim3 grid = blas_grid(a), block(NT);
+  if (p == F64) return WL_DISPATCH_NC(k_xpay, F64);
+  if (p == F32) return WL_DISPATCH_NC(k_xpay, F32);
+  return WL_DISPATCH_NC(k_xpay, F16);
+}
+
+hipError_t launch_accumulate(int px, int py, int nc, BlasArgs a, hipStream_t stream) {
+  dim3 grid = blas_grid(a), block(NT);
+  if (px == F64 && py == F32) return WL_DISPATCH_NC(k_accumulate, F64, F32);
+  if (px == F64 && py == F64) return WL_DISPATCH_NC(k_accumulate, F64, F64);
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_convert(int po, int pi, int nc, BlasArgs a, hipStream_t stream) {
+  dim3 grid = blas_grid(a), block(NT);
+  if (po == F32 && pi == F64) return WL_DISPATCH_NC(k_convert, F32, F64);
+  if (po == F16 && pi == F64) return WL_DISPATCH_NC(k_convert, F16, F64);
+  if (po == F64 && pi == F64) return WL_DISPATCH_NC(k_convert, F64, F64);
+  if (po == F64 && pi == F32) return WL_DISPATCH_NC(k_convert, F64, F32);
+  if (po == F64 && pi == F16) return WL_DISPATCH_NC(k_convert, F64, F16);
+  if (po == F32 && pi == F32) return WL_DISPATCH_NC(k_convert, F32, F32);
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_finalize_update(ColState* cols, int ncol, hipStream_t stream) {
+  hipLaunchKernelGGL(k_finalize_update, dim3((ncol + 63) / 64), dim3(64), 0, stream, cols, ncol);
+  return hipGetLastError();
+}
+
+}  // namespace wl

@@ -1,0 +1,90 @@
+// Kernel argument blocks and launch wrappers shared by the host code.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace wl {
+
+constexpr int NT = 256;  // threads per workgroup (4 wavefronts)
+
+struct Geom {
+  int L[4];
+  int stride[4];  // flat-index strides, stride[3] = 1 (last axis fastest)
+  long long V;
+};
+
+// Per-column solver scalars, FP64, resident in device memory.
+struct ColState {
+  double rr;        // |r|^2 of the iterated residual
+  double rr_old;    // |r|^2 at the start of the current iteration
+  double pAp;       // Re <p, A p>
+  double alpha, beta;
+  double b2;        // |b|^2
+  double stop2;     // inner stopping threshold on rr
+  double tol2b2;    // true-residual target tol^2 |b|^2
+  double r0norm;    // |r| at the last reliable update
+  double delta;     // reliable-update trigger (0 disables)
+  double rtrue2;    // last true residual |b - A x|^2
+  double m0;        // diagonal m + 4r of this column's operator
+  int iter;
+  int maxiter;
+  int active;       // iterate this column
+  int done;         // finished (see status)
+  int status;       // WL_STATUS_*
+  int need_update;  // 1: reliable update, 2: convergence check
+  int nupdates;
+  int mode;         // 0: stop on true residual, 1: stop on iterated residual (reference rule)
+};
+
+// Which per-column action the last workgroup of a reduction performs.
+enum RedAction {
+  RA_NONE = 0,
+  RA_PAP = 1,     // pAp = sum; alpha = rr / pAp
+  RA_ITER = 2,    // rr update, beta, iteration count, stopping / update triggers
+  RA_B2 = 3,      // b2 = sum
+  RA_TRUERES = 4, // rtrue2 = sum: convergence decision after a reliable update
+};
+
+struct Reduction {
+  double* partials;      // [ncol][nblk_per_col]
+  unsigned* counters;    // [ncol]
+  ColState* cols;        // [ncol]
+  int action;
+};
+
+struct DslashArgs {
+  Geom g;
+  const void* U;          // gauge configurations, internal layout
+  size_t cfg_bytes;       // bytes of one configuration (4 directions)
+  size_t dir_bytes;       // bytes of one direction
+  const void* in;
+  void* out;
+  const void* w;          // epilogue operand (dot / residual), same precision as out
+  size_t col_bytes;       // bytes of one spinor column (in, out and w share precision)
+  int ncol;
+  int cols_per_cfg;
+  int ntiles;
+  double bc_re[4], bc_im[4];
+  double m0;              // diagonal coefficient (used when m0_from_state == 0)
+  double r;               // Wilson r
+  int m0_from_state;      // take m0 per column from red.cols[col].m0
+  int skip_inactive;      // skip columns whose state is not active
+  int only_update;        // run only columns with need_update != 0 (reliable updates)
+  Reduction red;
+};
+
+struct BlasArgs {
+  long long V;
+  int ncol;
+  int ntiles;
+  int gate;  // 0 none, 1 active columns, 2 columns awaiting a reliable update
+  size_t col_bytes_a, col_bytes_b, col_bytes_c, col_bytes_d;
+  void* a;
+  void* b;
+  void* c;
+  void* d;
+  Reduction red;
+};
+
+}  // namespace wl

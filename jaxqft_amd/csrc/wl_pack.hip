@@ -1,0 +1,226 @@
+// Conversion between the reference array layouts and the internal
+// structure-of-arrays layout.
+//
+// Reference spinor: (cols, L0, L1, L2, L3, Ns=4, Nc) complex, C-order
+// (su3_wilson_nf2.py:815-817).  Reference links: BMXYIJ (B, 4, L0..L3, Nc, Nc)
+// or BXYMIJ (B, L0..L3, 4, Nc, Nc); U(1) drops the (Nc, Nc) axes
+// (su3_ym.py:246-249, u1_ym.py:189-192).
+#include "wl_device.h"
+#include "wl_kernels.h"
+
+namespace wl {
+
+template <int P, int NC>
+struct LinkStore;
+
+template <int NC>
+struct LinkStore<F64, NC> {
+  template <int RECON>
+  __device__ static void store(void* base, size_t V, size_t site, const cx<double> (&u)[NC][NC]) {
+    double2* p = reinterpret_cast<double2*>(base);
+    constexpr int ROWS = (NC == 3 && RECON == 12) ? 2 : NC;
+    for (int a = 0; a < ROWS; ++a)
+      for (int b = 0; b < NC; ++b) p[(a * NC + b) * V + site] = make_double2(u[a][b].re, u[a][b].im);
+  }
+};
+
+template <int NC>
+struct LinkStore<F32, NC> {
+  template <int RECON>
+  __device__ static void store(void* base, size_t V, size_t site, const cx<double> (&u)[NC][NC]) {
+    if (NC == 3 && RECON == 12) {
+      float4* p = reinterpret_cast<float4*>(base);
+      for (int j = 0; j < 3; ++j) {
+        int c0 = 2 * j, c1 = 2 * j + 1;
+        p[j * V + site] = make_float4((float)u[c0 / NC][c0 % NC].re, (float)u[c0 / NC][c0 % NC].im,
+                                      (float)u[c1 / NC][c1 % NC].re, (float)u[c1 / NC][c1 % NC].im);
+      }
+    } else {
+      float2* p = reinterpret_cast<float2*>(base);
+      for (int c = 0; c < NC * NC; ++c)
+        p[c * V + site] = make_float2((float)u[c / NC][c % NC].re, (float)u[c / NC][c % NC].im);
+    }
+  }
+};
+
+template <int NC>
+struct LinkStore<F16, NC> {
+  template <int RECON>
+  __device__ static void store(void* base, size_t V, size_t site, const cx<double> (&u)[NC][NC]) {
+    if (NC == 3 && RECON == 12) {
+      uint4* pa = reinterpret_cast<uint4*>(base);
+      uint2* pb = reinterpret_cast<uint2*>(pa + V);
+      uint32_t h[6];
+      for (int c = 0; c < 6; ++c) h[c] = f2h((float)u[c / NC][c % NC].re, (float)u[c / NC][c % NC].im);
+      pa[site] = make_uint4(h[0], h[1], h[2], h[3]);
+      pb[site] = make_uint2(h[4], h[5]);
+    } else {
+      uint32_t* p = reinterpret_cast<uint32_t*>(base);
+      for (int c = 0; c < NC * NC; ++c) p[c * V + site] = f2h((float)u[c / NC][c % NC].re, (float)u[c / NC][c % NC].im);
+    }
+  }
+};
+
+template <int P, int NC, class S>
+__global__ __launch_bounds__(NT) void k_pack_spinor(const S* __restrict__ src, void* __restrict__ dst, long long V,
+                                                    int ncol, size_t col_bytes) {
+  const size_t site = (size_t)blockIdx.x * NT + threadIdx.x;
+  const int col = blockIdx.y;
+  if (site >= (size_t)V) return;
+  using T = typename PT<P>::acc;
+  const S* s = src + ((size_t)col * V + site) * (4 * NC) * 2;
+  cx<T> v[4 * NC];
+#pragma unroll
+  for (int c = 0; c < 4 * NC; ++c) v[c] = {(T)s[2 * c], (T)s[2 * c + 1]};
+  SpinorIO<P, NC>::store(reinterpret_cast<char*>(dst) + col * col_bytes, V, site, v);
+}
+
+template <int P, int NC, class S>
+__global__ __launch_bounds__(NT) void k_unpack_spinor(const void* __restrict__ src, S* __restrict__ dst, long long V,
+                                                      int ncol, size_t col_bytes) {
+  const size_t site = (size_t)blockIdx.x * NT + threadIdx.x;
+  const int col = blockIdx.y;
+  if (site >= (size_t)V) return;
+  using T = typename PT<P>::acc;
+  cx<T> v[4 * NC];
+  SpinorIO<P, NC>::load(reinterpret_cast<const char*>(src) + col * col_bytes, V, site, v);
+  S* d = dst + ((size_t)col * V + site) * (4 * NC) * 2;
+#pragma unroll
+  for (int c = 0; c < 4 * NC; ++c) {
+    d[2 * c] = (S)v[c].re;
+    d[2 * c + 1] = (S)v[c].im;
+  }
+}
+
+// layout: 0 = BMXYIJ, 1 = BXYMIJ
+template <int P, int NC, int RECON, class S>
+__global__ __launch_bounds__(NT) void k_pack_gauge(const S* __restrict__ src, void* __restrict__ dst, long long V,
+                                                   int layout, size_t cfg_bytes, size_t dir_bytes) {
+  const size_t site = (size_t)blockIdx.x * NT + threadIdx.x;
+  const int mu = blockIdx.y;
+  const int cfg = blockIdx.z;
+  if (site >= (size_t)V) return;
+  size_t off = (layout == 0) ? (((size_t)cfg * 4 + mu) * V + site) : (((size_t)cfg * V + site) * 4 + mu);
+  const S* s = src + off * NC * NC * 2;
+  cx<double> u[NC][NC];
+  for (int a = 0; a < NC; ++a)
+    for (int b = 0; b < NC; ++b) u[a][b] = {(double)s[2 * (a * NC + b)], (double)s[2 * (a * NC + b) + 1]};
+  char* d = reinterpret_cast<char*>(dst) + cfg * cfg_bytes + mu * dir_bytes;
+  LinkStore<P, NC>::template store<RECON>(d, V, site, u);
+}
+
+// Re-encode an internal FP64 full gauge field into another precision/recon.
+template <int P, int NC, int RECON>
+__global__ __launch_bounds__(NT) void k_convert_gauge(const void* __restrict__ src, void* __restrict__ dst, long long V,
+                                                      size_t src_cfg_bytes, size_t src_dir_bytes, size_t cfg_bytes,
+                                                      size_t dir_bytes) {
+  const size_t site = (size_t)blockIdx.x * NT + threadIdx.x;
+  const int mu = blockIdx.y;
+  const int cfg = blockIdx.z;
+  if (site >= (size_t)V) return;
+  const double2* s =
+      reinterpret_cast<const double2*>(reinterpret_cast<const char*>(src) + cfg * src_cfg_bytes + mu * src_dir_bytes);
+  cx<double> u[NC][NC];
+  for (int a = 0; a < NC; ++a)
+    for (int b = 0; b < NC; ++b) {
+      double2 t = s[(a * NC + b) * V + site];
+      u[a][b] = {t.x, t.y};
+    }
+  char* d = reinterpret_cast<char*>(dst) + cfg * cfg_bytes + mu * dir_bytes;
+  LinkStore<P, NC>::template store<RECON>(d, V, site, u);
+}
+
+// ---------------------------------------------------------------------------
+// host dispatch.  dtype: 0 complex128, 1 complex64
+// ---------------------------------------------------------------------------
+template <int P, int NC>
+static hipError_t pack_spinor_t(const void* src, int dtype, void* dst, long long V, int ncol, size_t col_bytes,
+                                hipStream_t st) {
+  dim3 grid((unsigned)((V + NT - 1) / NT), (unsigned)ncol), block(NT);
+  if (dtype == 0)
+    hipLaunchKernelGGL((k_pack_spinor<P, NC, double>), grid, block, 0, st, (const double*)src, dst, V, ncol, col_bytes);
+  else
+    hipLaunchKernelGGL((k_pack_spinor<P, NC, float>), grid, block, 0, st, (const float*)src, dst, V, ncol, col_bytes);
+  return hipGetLastError();
+}
+
+template <int P, int NC>
+static hipError_t unpack_spinor_t(const void* src, void* dst, int dtype, long long V, int ncol, size_t col_bytes,
+                                  hipStream_t st) {
+  dim3 grid((unsigned)((V + NT - 1) / NT), (unsigned)ncol), block(NT);
+  if (dtype == 0)
+    hipLaunchKernelGGL((k_unpack_spinor<P, NC, double>), grid, block, 0, st, src, (double*)dst, V, ncol, col_bytes);
+  else
+    hipLaunchKernelGGL((k_unpack_spinor<P, NC, float>), grid, block, 0, st, src, (float*)dst, V, ncol, col_bytes);
+  return hipGetLastError();
+}
+
+hipError_t pack_spinor(int prec, int nc, const void* src, int dtype, void* dst, long long V, int ncol,
+                       size_t col_bytes, hipStream_t st) {
+  if (nc == 3) {
+    if (prec == F64) return pack_spinor_t<F64, 3>(src, dtype, dst, V, ncol, col_bytes, st);
+    if (prec == F32) return pack_spinor_t<F32, 3>(src, dtype, dst, V, ncol, col_bytes, st);
+    return pack_spinor_t<F16, 3>(src, dtype, dst, V, ncol, col_bytes, st);
+  }
+  if (prec == F64) return pack_spinor_t<F64, 1>(src, dtype, dst, V, ncol, col_bytes, st);
+  if (prec == F32) return pack_spinor_t<F32, 1>(src, dtype, dst, V, ncol, col_bytes, st);
+  return pack_spinor_t<F16, 1>(src, dtype, dst, V, ncol, col_bytes, st);
+}
+
+hipError_t unpack_spinor(int prec, int nc, const void* src, void* dst, int dtype, long long V, int ncol,
+                         size_t col_bytes, hipStream_t st) {
+  if (nc == 3) {
+    if (prec == F64) return unpack_spinor_t<F64, 3>(src, dst, dtype, V, ncol, col_bytes, st);
+    if (prec == F32) return unpack_spinor_t<F32, 3>(src, dst, dtype, V, ncol, col_bytes, st);
+    return unpack_spinor_t<F16, 3>(src, dst, dtype, V, ncol, col_bytes, st);
+  }
+  if (prec == F64) return unpack_spinor_t<F64, 1>(src, dst, dtype, V, ncol, col_bytes, st);
+  if (prec == F32) return unpack_spinor_t<F32, 1>(src, dst, dtype, V, ncol, col_bytes, st);
+  return unpack_spinor_t<F16, 1>(src, dst, dtype, V, ncol, col_bytes, st);
+}
+
+hipError_t pack_gauge_f64(int nc, const void* src, int dtype, int layout, void* dst, long long V, int ncfg,
+                          size_t cfg_bytes, size_t dir_bytes, hipStream_t st) {
+  dim3 grid((unsigned)((V + NT - 1) / NT), 4, (unsigned)ncfg), block(NT);
+  if (nc == 3) {
+    if (dtype == 0)
+      hipLaunchKernelGGL((k_pack_gauge<F64, 3, 18, double>), grid, block, 0, st, (const double*)src, dst, V, layout,
+                         cfg_bytes, dir_bytes);
+    else
+      hipLaunchKernelGGL((k_pack_gauge<F64, 3, 18, float>), grid, block, 0, st, (const float*)src, dst, V, layout,
+                         cfg_bytes, dir_bytes);
+  } else {
+    if (dtype == 0)
+      hipLaunchKernelGGL((k_pack_gauge<F64, 1, 18, double>), grid, block, 0, st, (const double*)src, dst, V, layout,
+                         cfg_bytes, dir_bytes);
+    else
+      hipLaunchKernelGGL((k_pack_gauge<F64, 1, 18, float>), grid, block, 0, st, (const float*)src, dst, V, layout,
+                         cfg_bytes, dir_bytes);
+  }
+  return hipGetLastError();
+}
+
+hipError_t convert_gauge(int prec, int nc, int recon, const void* src, void* dst, long long V, int ncfg,
+                         size_t src_cfg_bytes, size_t src_dir_bytes, size_t cfg_bytes, size_t dir_bytes,
+                         hipStream_t st) {
+  dim3 grid((unsigned)((V + NT - 1) / NT), 4, (unsigned)ncfg), block(NT);
+#define WL_CG(P, NCV, R)                                                                                        \
+  hipLaunchKernelGGL((k_convert_gauge<P, NCV, R>), grid, block, 0, st, src, dst, V, src_cfg_bytes, src_dir_bytes, \
+                     cfg_bytes, dir_bytes)
+  if (nc == 3) {
+    if (prec == F64 && recon == 12) WL_CG(F64, 3, 12);
+    else if (prec == F64) WL_CG(F64, 3, 18);
+    else if (prec == F32 && recon == 12) WL_CG(F32, 3, 12);
+    else if (prec == F32) WL_CG(F32, 3, 18);
+    else if (prec == F16 && recon == 12) WL_CG(F16, 3, 12);
+    else WL_CG(F16, 3, 18);
+  } else {
+    if (prec == F64) WL_CG(F64, 1, 18);
+    else if (prec == F32) WL_CG(F32, 1, 18);
+    else WL_CG(F16, 1, 18);
+  }
+#undef WL_CG
+  return hipGetLastError();
+}
+
+}  // namespace wl

@@ -1,0 +1,109 @@
+// Deterministic grid reductions for the CG scalars.
+//
+// Every workgroup writes one FP64 partial (wave64 shuffle tree + LDS across
+// the 4 wavefronts); the last workgroup to arrive (agent-scope atomic
+// counter) sums the partials in a fixed order and performs the per-column
+// scalar update.  The result is independent of workgroup scheduling, so a
+// column gives the same bits alone or batched with other columns.
+#pragma once
+#include "wl_device.h"
+#include "wl_kernels.h"
+
+#define WL_STATUS_RUNNING (-1)
+#define WL_STATUS_CONVERGED 0
+#define WL_STATUS_MAXITER 1
+#define WL_STATUS_NAN 2
+#define WL_STATUS_BREAKDOWN 3
+
+namespace wl {
+
+__device__ __forceinline__ void col_finish(ColState& s, int status) {
+  s.status = status;
+  s.done = 1;
+  s.active = 0;
+}
+
+__device__ __forceinline__ void apply_action(int action, ColState& s, double sum) {
+  switch (action) {
+    case RA_PAP: {
+      s.pAp = sum;
+      if (!(sum > 0.0) || !isfinite(sum)) {
+        col_finish(s, isfinite(sum) ? WL_STATUS_BREAKDOWN : WL_STATUS_NAN);
+        s.alpha = 0.0;
+      } else {
+        s.alpha = s.rr / sum;
+      }
+      break;
+    }
+    case RA_ITER: {
+      s.rr_old = s.rr;
+      s.rr = sum;
+      s.beta = sum / s.rr_old;
+      s.iter += 1;
+      if (!isfinite(sum)) {
+        col_finish(s, WL_STATUS_NAN);
+      } else if (sum <= s.stop2) {
+        if (s.mode == 1) {
+          col_finish(s, WL_STATUS_CONVERGED);
+        } else {
+          s.need_update = 2;
+          s.active = 0;
+        }
+      } else if (s.delta > 0.0 && sqrt(sum) < s.delta * s.r0norm) {
+        s.need_update = 1;
+        s.active = 0;
+      } else if (s.iter >= s.maxiter) {
+        // one last true-residual evaluation decides the status
+        s.need_update = 2;
+        s.active = 0;
+      }
+      break;
+    }
+    case RA_B2:
+      s.b2 = sum;
+      break;
+    case RA_TRUERES: {
+      s.rtrue2 = sum;
+      if (!isfinite(sum)) {
+        col_finish(s, WL_STATUS_NAN);
+      } else if (sum <= s.tol2b2) {
+        col_finish(s, WL_STATUS_CONVERGED);
+      } else if (s.iter >= s.maxiter) {
+        col_finish(s, WL_STATUS_MAXITER);
+      } else {
+        // continue from the true residual, keeping the search direction
+        s.beta = (s.rr_old > 0.0) ? sum / s.rr_old : 0.0;
+      }
+      break;
+    }
+    default:
+      break;
+  }
+}
+
+// Called by every thread of the workgroup; v is this thread's contribution.
+__device__ __forceinline__ void finish_reduction(const Reduction& red, int col, int blk, int nblk, double v) {
+  __shared__ double lds[NT / 64];
+  __shared__ int am_last;
+  double bs = block_sum<NT>(v, lds);
+  double* parts = red.partials + (size_t)col * nblk;
+  if (threadIdx.x == 0) {
+    __hip_atomic_store(parts + blk, bs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    unsigned prev = atomicAdd(red.counters + col, 1u);
+    am_last = (prev == (unsigned)(nblk - 1));
+  }
+  __syncthreads();
+  if (!am_last) return;
+  __threadfence();
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += NT) s += __hip_atomic_load(parts + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  s = block_sum<NT>(s, lds);
+  if (threadIdx.x == 0) {
+    apply_action(red.action, red.cols[col], s);
+    red.counters[col] = 0;
+    __threadfence();
+  }
+}
+
+}  // namespace wl

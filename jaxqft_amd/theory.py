@@ -1,0 +1,326 @@
+"""Theory-level drop-in for the Wilson-fermion operator and normal-equation CG.
+
+Mirrors the fermion-operator surface of the reference theory objects
+``SU3WilsonNf2`` (jaxqft/models/su3_wilson_nf2.py:594-1635) and
+``U1WilsonNf2`` (jaxqft/models/u1_wilson_nf2.py:296-1025): the same dataclass
+field names, shapes, layouts, boundary-phase and mass conventions and method
+names (``apply_D``, ``apply_Ddag``, ``apply_normal``, ``apply_offdiag``,
+``solve_normal``, ``solve_normal_with_guess``, ``sample_pseudofermion``,
+``pseudofermion_action``, ``normal_residual_stats``, ``fermion_shape``, ...).
+
+Every operator application and solve runs in libwilson_amd on the GPU.
+Inputs may be NumPy arrays or torch tensors; outputs come back as the same
+kind (and dtype as the fermion input).  Computation is FP64 regardless of the
+input dtype (complex64 inputs are widened on the device).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Any, Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from ._arrays import from_device, to_device
+from .fermions.gamma import build_euclidean_gamma, gamma5
+from .fermions.wilson import WilsonDiracOperator
+from . import fields as _fields
+
+
+def parse_fermion_bc(bc, nd: int, default_antiperiodic_t: bool) -> Tuple[complex, ...]:
+    """su3_wilson_nf2.py:711-724 / :1645-1660 and u1_wilson_nf2.py:461-471."""
+    if bc is None:
+        vals = [1.0 + 0.0j] * nd
+        if default_antiperiodic_t:
+            vals[-1] = -1.0 + 0.0j
+    elif isinstance(bc, str):
+        txt = bc.strip().lower()
+        if txt in ("", "periodic", "p"):
+            vals = [1.0 + 0.0j] * nd
+        elif txt in ("antiperiodic-t", "anti-t", "ap-t", "apt", "chroma"):
+            vals = [1.0 + 0.0j] * nd
+            vals[-1] = -1.0 + 0.0j
+        else:
+            vals = [complex(t.strip().replace("I", "j").replace("i", "j")) for t in bc.split(",") if t.strip()]
+    else:
+        vals = [complex(v) for v in bc]
+    if len(vals) != nd:
+        raise ValueError(f"fermion_bc must have Nd={nd} entries; got {len(vals)}")
+    return tuple(vals)
+
+
+class ResidentGauge:
+    """Gauge configurations kept on the device; pass it wherever ``U`` is taken
+    to skip the per-call upload.  Links are stored without the boundary phase."""
+
+    def __init__(self, theory: "WilsonNf2Theory", U: Any):
+        theory._check_links(U)
+        Ut, _, _ = to_device(U)
+        self.theory = theory
+        self.ctx = theory._context()
+        self.gauge = nat.Gauge(self.ctx, Ut, theory._layout_code())
+        self.shape = tuple(Ut.shape)
+
+
+@dataclass
+class WilsonNf2Theory:
+    lattice_shape: Tuple[int, ...]
+    beta: float = 0.0
+    batch_size: int = 1
+    layout: str = "BMXYIJ"
+    seed: int = 0
+    dtype: Any = np.complex64
+    mass: float = 0.0
+    wilson_r: float = 1.0
+    cg_tol: float = 1e-8
+    cg_maxiter: int = 500
+    solver_kind: str = "cg"
+    solver_form: str = "normal"
+    preconditioner_kind: str = "none"
+    fermion_bc: Optional[Any] = None
+    dirac_kernel: str = "optimized"
+    # native solver policy (no counterpart in the reference)
+    solver_precision: str = "f64"   # inner precision: f64 (plain CG) | f32 | f16 (reliable updates)
+    solver_recon: int = 18          # 18 | 12: link reconstruction of the inner operator
+    reliable_delta: float = 0.1     # reliable-update trigger on the residual drop
+    solver_stop: str = "true"       # "true": FP64 true residual <= cg_tol; "iterated": reference rule
+    poll_every: int = 8
+    use_graph: bool = False
+    Nc: int = field(default=3, init=False)
+    last_solve_stats: List[Dict[str, Any]] = field(default_factory=list, init=False, repr=False)
+
+    _default_antiperiodic_t = False
+
+    def __post_init__(self):
+        self.lattice_shape = tuple(int(v) for v in self.lattice_shape)
+        self.Nd = len(self.lattice_shape)
+        self.Bs = int(self.batch_size)
+        lay = str(self.layout).upper()
+        lay = {"BM...IJ": "BMXYIJ", "B...MIJ": "BXYMIJ"}.get(lay, lay)
+        if lay not in ("BMXYIJ", "BXYMIJ"):
+            raise ValueError("layout must be one of: BMXYIJ, BXYMIJ, BM...IJ, B...MIJ")
+        self.layout = lay
+        self.solver_kind = str(self.solver_kind).lower()
+        self.solver_form = str(self.solver_form).lower()
+        self.preconditioner_kind = str(self.preconditioner_kind).lower()
+        self.dirac_kernel = str(self.dirac_kernel).lower()
+        if self.solver_kind != "cg":
+            raise ValueError("the native solver path provides solver_kind='cg' (CG on D^dag D) only")
+        if self.solver_form != "normal":
+            raise ValueError("the native solver path provides solver_form='normal' only")
+        # A scalar Jacobi preconditioner 1/m0^2 leaves the CG iterates and the
+        # residual-based stopping rule unchanged, so both kinds map to plain CG.
+        if self.preconditioner_kind not in ("none", "jacobi"):
+            raise ValueError("preconditioner_kind must be one of: none, jacobi")
+        if self.dirac_kernel not in ("optimized", "reference"):
+            raise ValueError("dirac_kernel must be one of: optimized, reference")
+        if self.solver_precision not in nat.PREC:
+            raise ValueError("solver_precision must be one of: f64, f32, f16")
+        if int(self.solver_recon) not in (18, 12):
+            raise ValueError("solver_recon must be 18 or 12")
+        if self.solver_stop not in ("true", "iterated"):
+            raise ValueError("solver_stop must be 'true' or 'iterated'")
+        self.fermion_bc = parse_fermion_bc(self.fermion_bc, self.Nd, self._default_antiperiodic_t)
+        self.dirac = WilsonDiracOperator(ndim=self.Nd, mass=self.mass, wilson_r=self.wilson_r, dtype=self.dtype)
+        self.gamma = self.dirac.gamma
+        self.Ns = self.dirac.Ns
+        self._m0 = float(self.mass + self.wilson_r * self.Nd)
+        self._rng = _fields.rng_for(self.seed)
+        self._ctx: Optional[nat.Context] = None
+
+    # ------------------------------------------------------------------ shapes
+    def fermion_shape(self) -> Tuple[int, ...]:
+        return (self.Bs, *self.lattice_shape, self.Ns, self.Nc)
+
+    def field_shape(self) -> Tuple[int, ...]:
+        raise NotImplementedError
+
+    def _layout_code(self) -> int:
+        return nat.WL_LAYOUT_BMXYIJ if self.layout == "BMXYIJ" else nat.WL_LAYOUT_BXYMIJ
+
+    def _check_links(self, U) -> None:
+        if isinstance(U, ResidentGauge):
+            return
+        shp = tuple(U.shape)
+        if shp != tuple(self.field_shape()):
+            raise ValueError(f"link field has shape {shp}, expected {self.field_shape()}")
+
+    def _check_fermion(self, psi, name="psi") -> None:
+        if tuple(psi.shape) != tuple(self.fermion_shape()):
+            raise ValueError(f"{name} has shape {tuple(psi.shape)}, expected {self.fermion_shape()}")
+
+    # ------------------------------------------------------------ native plumbing
+    def _context(self) -> nat.Context:
+        if self._ctx is None:
+            self._ctx = nat.Context(self.lattice_shape, self.Nc, self.Bs, self.mass, self.wilson_r, self.fermion_bc)
+        if self._ctx.mass != float(self.mass):
+            self._ctx.set_mass(float(self.mass))
+        return self._ctx
+
+    def _gauge(self, U) -> Tuple[nat.Context, Any]:
+        if isinstance(U, ResidentGauge):
+            if U.theory is not self and U.ctx is not self._context():
+                raise ValueError("resident gauge belongs to another theory")
+            return U.ctx, U.gauge
+        self._check_links(U)
+        ctx = self._context()
+        Ut, _, _ = to_device(U)
+        return ctx, nat.Gauge(ctx, Ut, self._layout_code())
+
+    def resident_gauge(self, U) -> ResidentGauge:
+        return ResidentGauge(self, U)
+
+    def _apply(self, U, psi, op: int):
+        self._check_fermion(psi)
+        ctx, g = self._gauge(U)
+        pt, kind, orig = to_device(psi)
+        out = torch.empty_like(pt)
+        dt = nat.WL_C128 if pt.dtype == torch.complex128 else nat.WL_C64
+        nat.check(ctx._lib.wl_apply(ctx.h, g.h, int(op), nat.c_void_p(pt.data_ptr()), nat.c_void_p(out.data_ptr()), 1,
+                                    dt, nat.c_void_p(nat.current_stream())))
+        return from_device(out, kind, orig)
+
+    # -------------------------------------------------------------- operator API
+    def apply_D(self, U, psi):
+        return self._apply(U, psi, nat.WL_OP_D)
+
+    def apply_Ddag(self, U, psi):
+        return self._apply(U, psi, nat.WL_OP_DDAG)
+
+    def apply_normal(self, U, psi):
+        return self._apply(U, psi, nat.WL_OP_DDAGD)
+
+    def apply_offdiag(self, U, psi):
+        return self._apply(U, psi, nat.WL_OP_K)
+
+    def apply_offdiag_dagger(self, U, psi):
+        return self._apply(U, psi, nat.WL_OP_KDAG)
+
+    def apply_g5D(self, U, psi):
+        """gamma5 D (the composition used by the reference's convention test)."""
+        return self._apply(U, psi, nat.WL_OP_G5D)
+
+    def apply_D_with_kernel(self, U, psi, kernel: str, use_sparse_gamma: bool = True):
+        if str(kernel).lower() not in ("optimized", "reference"):
+            raise ValueError("kernel must be one of: optimized, reference")
+        return self.apply_D(U, psi)
+
+    def apply_normal_with_kernel(self, U, psi, kernel: str, use_sparse_gamma: bool = True):
+        if str(kernel).lower() not in ("optimized", "reference"):
+            raise ValueError("kernel must be one of: optimized, reference")
+        return self.apply_normal(U, psi)
+
+    # ----------------------------------------------------------------- solvers
+    def _solve_params(self, masses=None, **overrides):
+        pol = dict(
+            tol=self.cg_tol,
+            maxiter=self.cg_maxiter,
+            stop_rule=nat.WL_STOP_ITERATED if self.solver_stop == "iterated" else nat.WL_STOP_TRUE_RESIDUAL,
+            inner_prec=nat.PREC[self.solver_precision],
+            inner_recon=int(self.solver_recon),
+            delta=self.reliable_delta,
+            poll_every=self.poll_every,
+            use_graph=self.use_graph,
+            masses=masses,
+        )
+        pol.update(overrides)
+        return nat.solve_params(**pol)
+
+    def _solve(self, U, phi, x0, nrhs: int = 1, masses=None, **overrides):
+        ctx, g = self._gauge(U)
+        pt, kind, orig = to_device(phi)
+        x = torch.empty_like(pt)
+        x0t = None
+        if x0 is not None:
+            x0t, _, _ = to_device(x0, dtype=pt.dtype)
+        ncol = self.Bs * nrhs
+        sp, keep = self._solve_params(masses=masses, **overrides)
+        stats = (nat.WlSolveStats * ncol)()
+        dt = nat.WL_C128 if pt.dtype == torch.complex128 else nat.WL_C64
+        stream = nat.current_stream()
+        if sp.use_graph and stream == 0:
+            side = torch.cuda.Stream()
+            side.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(side):
+                stream = side.cuda_stream
+                nat.check(ctx._lib.wl_cg_normal(ctx.h, g.h, nat.c_void_p(pt.data_ptr()), nat.c_void_p(x.data_ptr()),
+                                                nat.c_void_p(x0t.data_ptr() if x0t is not None else 0), nrhs, dt,
+                                                nat.byref(sp), stats, nat.c_void_p(stream)))
+            torch.cuda.current_stream().wait_stream(side)
+        else:
+            nat.check(ctx._lib.wl_cg_normal(ctx.h, g.h, nat.c_void_p(pt.data_ptr()), nat.c_void_p(x.data_ptr()),
+                                            nat.c_void_p(x0t.data_ptr() if x0t is not None else 0), nrhs, dt,
+                                            nat.byref(sp), stats, nat.c_void_p(stream)))
+        del keep
+        self.last_solve_stats = nat.stats_list(stats, ncol)
+        return from_device(x, kind, orig)
+
+    def solve_normal(self, U, phi):
+        """x = (D^dag D)^-1 phi per batch element (su3_wilson_nf2.py:1410-1429)."""
+        self._check_fermion(phi, "phi")
+        return self._solve(U, phi, None)
+
+    def solve_normal_with_guess(self, U, phi, x0):
+        """Same with initial guess x0 (su3_wilson_nf2.py:1431-1432)."""
+        self._check_fermion(phi, "phi")
+        self._check_fermion(x0, "x0")
+        return self._solve(U, phi, x0)
+
+    def solve_normal_multi(self, U, rhs, masses: Optional[Sequence[float]] = None, **overrides):
+        """Solve k right-hand sides per configuration at once.
+
+        rhs: (B, k, *L, Ns, Nc).  masses: optional per-(b, k) masses, flattened
+        config-major.  Each column converges independently (per-column
+        scalars and masking); the links are shared by all k columns."""
+        shp = tuple(rhs.shape)
+        if len(shp) != self.Nd + 4 or shp[0] != self.Bs or shp[2:] != tuple(self.fermion_shape()[1:]):
+            raise ValueError(f"rhs must have shape (B, k, *L, Ns, Nc); got {shp}")
+        k = int(shp[1])
+        flat = rhs.reshape((self.Bs * k, *shp[2:]))
+        x = self._solve(U, flat, None, nrhs=k, masses=masses, **overrides)
+        return x.reshape(shp)
+
+    # ------------------------------------------------------ fields and actions
+    def hot_start(self, scale: float = 0.1):
+        raise NotImplementedError
+
+    def random_fermion(self):
+        return _fields.random_fermion(self._rng, self.fermion_shape()).astype(self.dtype)
+
+    def sample_pseudofermion(self, U):
+        """phi = D^dag eta (su3_wilson_nf2.py:1177-1181)."""
+        chi = self.random_fermion()
+        return self.apply_Ddag(U, chi)
+
+    def pseudofermion_action(self, U, phi):
+        """S = Re <phi, (D^dag D)^-1 phi> per batch element (su3_wilson_nf2.py:1183-1189)."""
+        x = self.solve_normal(U, phi)
+        ph = np.asarray(phi.cpu() if isinstance(phi, torch.Tensor) else phi)
+        xx = np.asarray(x.cpu() if isinstance(x, torch.Tensor) else x)
+        return np.real(np.sum(np.conj(ph) * xx, axis=tuple(range(1, ph.ndim))))
+
+    def apply_gamma5(self, psi):
+        """(gamma5 psi) on the spin axis; host-side helper for tests."""
+        g5 = gamma5(self.gamma)
+        return np.einsum("st,...tc->...sc", g5, np.asarray(psi))
+
+    def normal_residual_stats(self, U, phi, x) -> Dict[str, float]:
+        """su3_wilson_nf2.py:1456-1466, with D^dag D evaluated on the GPU in FP64."""
+        Ax = self.apply_normal(U, x)
+        r = np.asarray(Ax.cpu() if isinstance(Ax, torch.Tensor) else Ax) - np.asarray(
+            phi.cpu() if isinstance(phi, torch.Tensor) else phi)
+        ph = np.asarray(phi.cpu() if isinstance(phi, torch.Tensor) else phi)
+        rel = [float(np.linalg.norm(r[b]) / (np.linalg.norm(ph[b]) + 1e-30)) for b in range(r.shape[0])]
+        absn = [float(np.linalg.norm(r[b])) for b in range(r.shape[0])]
+        return {"mean_rel_residual": float(np.mean(rel)), "max_rel_residual": float(np.max(rel)),
+                "mean_abs_residual": float(np.mean(absn)), "max_abs_residual": float(np.max(absn))}
+
+    def solver_info_stats(self) -> Dict[str, float]:
+        """Iteration statistics of the last solve (the reference can only report
+        NaN here because jax cg returns info=None, su3_wilson_nf2.py:1468-1554)."""
+        it = [s["iters"] for s in self.last_solve_stats]
+        if not it:
+            return {"mean_iters": float("nan"), "max_iters": float("nan")}
+        return {"mean_iters": float(np.mean(it)), "max_iters": float(np.max(it)), "min_iters": float(np.min(it))}

@@ -1,0 +1,141 @@
+"""GPU parity of the normal-equation CG (plain FP64 and reliable-update mixed
+precision) against the oracle: every solve must reach a true residual
+||b - D^dag D x|| / ||b|| <= 1e-10 evaluated by the FP64 oracle operator, and
+agree with the oracle's converged solution."""
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+def true_resid(L, nc, mass, bc, U, b, x):
+    from oracle.wilson_oracle import WilsonOracle
+
+    ora = WilsonOracle(L, nc, mass, fermion_bc=bc)
+    r = ora.apply_normal(U, x) - b
+    return float(np.linalg.norm(r) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("policy", [
+    dict(solver_precision="f64"),
+    dict(solver_precision="f64", solver_stop="iterated"),
+    dict(solver_precision="f64", use_graph=True),
+    dict(solver_precision="f32"),
+    dict(solver_precision="f32", solver_recon=12),
+    dict(solver_precision="f16"),
+    dict(solver_precision="f16", solver_recon=12, use_graph=True),
+])
+def test_config1_cg_4x4x4x4(policy):
+    print(f"[gpu] cg config-1 {policy}", flush=True)
+    from jaxqft_amd.su3_wilson_nf2 import SU3WilsonNf2
+
+    z = np.load(GOLD / "cg_su3_4x4x4x4.npz")
+    L = (4, 4, 4, 4)
+    th = SU3WilsonNf2(lattice_shape=L, mass=0.1, fermion_bc=(1, 1, 1, -1), dtype=np.complex128, cg_tol=1e-10,
+                      cg_maxiter=3000, **policy)
+    x = th.solve_normal(z["U"], z["b"])
+    st = th.last_solve_stats[0]
+    assert st["status"] == "converged", st
+    res = true_resid(L, 3, 0.1, (1, 1, 1, -1), z["U"], z["b"], x)
+    if policy.get("solver_stop") == "iterated":
+        assert res <= 2e-10, res  # reference rule stops on the iterated residual
+    else:
+        assert res <= 1e-10, (res, st)
+        assert abs(res - st["true_resid"]) <= 1e-3 * res + 1e-14
+    assert np.linalg.norm(x - z["x"]) / np.linalg.norm(z["x"]) < 1e-8
+
+
+@pytest.mark.parametrize("nc", [3, 1])
+def test_batch_and_guess(nc):
+    print(f"[gpu] cg batch/guess nc={nc}", flush=True)
+    from jaxqft_amd import fields
+    from jaxqft_amd.su3_wilson_nf2 import SU3WilsonNf2
+    from jaxqft_amd.u1_wilson_nf2 import U1WilsonNf2
+
+    L = (4, 4, 2, 8)
+    cls = SU3WilsonNf2 if nc == 3 else U1WilsonNf2
+    th = cls(lattice_shape=L, mass=0.05, batch_size=2, dtype=np.complex128, cg_tol=1e-10, cg_maxiter=4000)
+    rng = fields.rng_for(500 + nc)
+    U = fields.su3_hot_start(rng, 2, L, 0.5) if nc == 3 else fields.u1_hot_start(rng, 2, L, 0.5)
+    b = fields.random_fermion(rng, th.fermion_shape())
+    x = th.solve_normal(U, b)
+    it_cold = [s["iters"] for s in th.last_solve_stats]
+    for k in range(2):
+        assert true_resid(L, nc, 0.05, th.fermion_bc, U[k:k + 1], b[k:k + 1], x[k:k + 1]) <= 1e-10
+    x2 = th.solve_normal_with_guess(U, b, x + 1e-6 * b)
+    it_warm = [s["iters"] for s in th.last_solve_stats]
+    assert all(w < c for w, c in zip(it_warm, it_cold)), (it_warm, it_cold)
+    for k in range(2):
+        assert true_resid(L, nc, 0.05, th.fermion_bc, U[k:k + 1], b[k:k + 1], x2[k:k + 1]) <= 1e-10
+
+
+def test_pseudofermion_identity_on_gpu():
+    print("[gpu] pfid", flush=True)
+    from jaxqft_amd import fields
+    from jaxqft_amd.su3_wilson_nf2 import SU3WilsonNf2
+
+    L = (4, 4, 4, 4)
+    th = SU3WilsonNf2(lattice_shape=L, mass=0.1, fermion_bc=(1, 1, 1, -1), dtype=np.complex128, cg_tol=1e-12,
+                      cg_maxiter=3000, seed=9)
+    U = th.hot_start(scale=0.3)
+    eta = th.random_fermion()
+    phi = th.apply_Ddag(U, eta)
+    S = th.pseudofermion_action(U, phi)
+    assert abs(S[0] - np.vdot(eta, eta).real) / np.vdot(eta, eta).real < 1e-10
+    st = th.normal_residual_stats(U, phi, th.solve_normal(U, phi))
+    assert st["max_rel_residual"] <= 1e-12
+
+
+def test_status_maxiter_and_nan():
+    print("[gpu] cg status", flush=True)
+    from jaxqft_amd import fields
+    from jaxqft_amd.su3_wilson_nf2 import SU3WilsonNf2
+
+    L = (4, 4, 4, 4)
+    rng = fields.rng_for(600)
+    U = fields.su3_hot_start(rng, 1, L, 0.5)
+    b = fields.random_fermion(rng, (1, *L, 4, 3))
+    th = SU3WilsonNf2(lattice_shape=L, mass=0.1, dtype=np.complex128, cg_tol=1e-10, cg_maxiter=7)
+    th.solve_normal(U, b)
+    assert th.last_solve_stats[0]["status"] == "maxiter"
+    assert th.last_solve_stats[0]["iters"] == 7
+    bad = b.copy()
+    bad[0, 1, 2, 3, 0, 2, 1] = np.nan
+    th2 = SU3WilsonNf2(lattice_shape=L, mass=0.1, dtype=np.complex128, cg_tol=1e-10, cg_maxiter=100)
+    th2.solve_normal(U, bad)
+    assert th2.last_solve_stats[0]["status"] == "nan"
+    # zero right-hand side converges immediately to x = 0
+    x0 = th2.solve_normal(U, np.zeros_like(b))
+    assert th2.last_solve_stats[0]["status"] == "converged" and np.all(x0 == 0)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32", "f16"])
+def test_multi_rhs_same_alone_and_in_the_mix(prec):
+    """Per-column masses and convergence masking: every column of a batched
+    multi-RHS solve is bit-identical to the same column solved alone."""
+    print(f"[gpu] multi-rhs {prec}", flush=True)
+    from jaxqft_amd import fields
+    from jaxqft_amd.su3_wilson_nf2 import SU3WilsonNf2
+
+    L = (4, 4, 4, 8)
+    rng = fields.rng_for(700)
+    U = fields.su3_hot_start(rng, 1, L, 0.6)
+    k = 5
+    rhs = fields.random_fermion(rng, (1, k, *L, 4, 3))
+    rhs[0, 2] = fields.point_sources(L, (1, 2, 3, 4))[7]
+    masses = [0.02, 0.2, 0.05, 0.1, 0.03]
+    th = SU3WilsonNf2(lattice_shape=L, mass=0.1, fermion_bc=(1, 1, 1, -1), dtype=np.complex128, cg_tol=1e-10,
+                      cg_maxiter=5000, solver_precision=prec)
+    X = th.solve_normal_multi(U, rhs, masses=masses)
+    stats = list(th.last_solve_stats)
+    assert all(s["status"] == "converged" for s in stats), stats
+    for j in range(k):
+        xj = th.solve_normal_multi(U, rhs[:, j:j + 1], masses=[masses[j]])
+        assert np.array_equal(xj[:, 0], X[:, j]), j
+        assert th.last_solve_stats[0]["iters"] == stats[j]["iters"]
+        assert true_resid(L, 3, masses[j], (1, 1, 1, -1), U, rhs[:, j], X[:, j]) <= 1e-10

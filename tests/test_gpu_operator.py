@@ -1,0 +1,150 @@
+"""GPU parity of the Wilson operator (through the C ABI) against the oracle's
+golden fixtures and against the oracle directly, plus size-independent
+properties at the benchmark sizes.  FP64 tolerance: 1e-13 relative to |D psi|."""
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLD = Path(__file__).resolve().parent / "golden"
+TOL = 1e-13
+
+
+def rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def theory_for(L, nc, mass, bc, **kw):
+    from jaxqft_amd.su3_wilson_nf2 import SU3WilsonNf2
+    from jaxqft_amd.u1_wilson_nf2 import U1WilsonNf2
+
+    cls = SU3WilsonNf2 if nc == 3 else U1WilsonNf2
+    return cls(lattice_shape=L, mass=mass, fermion_bc=bc, dtype=np.complex128, **kw)
+
+
+@pytest.mark.parametrize("name", ["op_su3_2x2x2x2", "op_su3_4x4x4x4", "op_su3_2x3x4x5", "op_u1_2x2x2x2",
+                                  "op_u1_4x4x4x4"])
+def test_operator_matches_golden(name):
+    print(f"[gpu] operator golden {name}", flush=True)
+    z = np.load(GOLD / f"{name}.npz")
+    L = tuple(int(v) for v in z["L"])
+    nc = z["psi"].shape[-1]
+    th = theory_for(L, nc, float(z["mass"]), tuple(z["bc"]))
+    U, psi = z["U"], z["psi"]
+    assert rel(th.apply_D(U, psi), z["D"]) < TOL
+    assert rel(th.apply_Ddag(U, psi), z["Ddag"]) < TOL
+    assert rel(th.apply_g5D(U, psi), z["g5D"]) < TOL
+    assert rel(th.apply_normal(U, psi), z["DdagD"]) < TOL
+    assert rel(th.apply_offdiag(U, psi), z["K"]) < TOL
+    assert rel(th.apply_offdiag_dagger(U, psi), z["Kdag"]) < TOL
+
+
+@pytest.mark.parametrize("nc", [3, 1])
+@pytest.mark.parametrize("L", [(8, 8, 8, 8), (4, 6, 2, 10), (1, 2, 3, 4)])
+def test_operator_matches_oracle_random(nc, L):
+    print(f"[gpu] operator oracle nc={nc} L={L}", flush=True)
+    from jaxqft_amd import fields
+    from oracle.wilson_oracle import WilsonOracle
+
+    bc = (np.exp(0.7j), -1, 1, -1)
+    rng = fields.rng_for(200 + nc)
+    th = theory_for(L, nc, 0.05, bc, batch_size=2)
+    U = fields.su3_hot_start(rng, 2, L, 0.6) if nc == 3 else fields.u1_hot_start(rng, 2, L, 0.6)
+    psi = fields.random_fermion(rng, th.fermion_shape())
+    ora = WilsonOracle(L, nc, 0.05, fermion_bc=bc)
+    assert rel(th.apply_D(U, psi), ora.apply_D(U, psi)) < TOL
+    assert rel(th.apply_Ddag(U, psi), ora.apply_Ddag(U, psi)) < TOL
+    assert rel(th.apply_normal(U, psi), ora.apply_normal(U, psi)) < TOL
+
+
+def test_layout_bxymij_and_complex64():
+    print("[gpu] layout/complex64", flush=True)
+    from jaxqft_amd import fields
+    from jaxqft_amd.su3_wilson_nf2 import SU3WilsonNf2
+    from oracle.wilson_oracle import WilsonOracle
+
+    L = (4, 4, 2, 6)
+    rng = fields.rng_for(300)
+    U = fields.su3_hot_start(rng, 1, L, 0.5)
+    psi = fields.random_fermion(rng, (1, *L, 4, 3))
+    ref = WilsonOracle(L, 3, 0.1, fermion_bc=(1, 1, 1, -1)).apply_D(U, psi)
+    th = SU3WilsonNf2(lattice_shape=L, mass=0.1, fermion_bc=(1, 1, 1, -1), layout="BXYMIJ", dtype=np.complex128)
+    assert rel(th.apply_D(np.moveaxis(U, 1, 5).copy(), psi), ref) < TOL
+    th32 = SU3WilsonNf2(lattice_shape=L, mass=0.1, fermion_bc=(1, 1, 1, -1))
+    out = th32.apply_D(U.astype(np.complex64), psi.astype(np.complex64))
+    assert out.dtype == np.complex64
+    ref32 = WilsonOracle(L, 3, 0.1, fermion_bc=(1, 1, 1, -1)).apply_D(U.astype(np.complex64), psi.astype(np.complex64))
+    assert rel(out, ref32) < 1e-6
+
+
+def test_torch_inputs_stay_on_device():
+    print("[gpu] torch inputs", flush=True)
+    import torch
+
+    from jaxqft_amd import fields
+    from oracle.wilson_oracle import WilsonOracle
+
+    L = (4, 4, 4, 4)
+    rng = fields.rng_for(301)
+    U = fields.su3_hot_start(rng, 1, L, 0.4)
+    psi = fields.random_fermion(rng, (1, *L, 4, 3))
+    th = theory_for(L, 3, 0.2, (1, 1, 1, -1))
+    out = th.apply_Ddag(torch.from_numpy(U).cuda(), torch.from_numpy(psi).cuda())
+    assert isinstance(out, torch.Tensor) and out.is_cuda
+    assert rel(out.cpu().numpy(), WilsonOracle(L, 3, 0.2, fermion_bc=(1, 1, 1, -1)).apply_Ddag(U, psi)) < TOL
+
+
+def test_l2_operator_with_prephased_links():
+    """WilsonDiracOperator receives links with the phase baked in, like the
+    reference's L2 operator (wilson.py:171-192)."""
+    print("[gpu] L2 operator", flush=True)
+    from jaxqft_amd import fields
+    from jaxqft_amd.fermions import WilsonDiracOperator
+    from oracle import wilson_oracle as wo
+
+    L = (4, 4, 4, 8)
+    rng = fields.rng_for(302)
+    U = fields.su3_hot_start(rng, 1, L, 0.5)
+    psi = fields.random_fermion(rng, (1, *L, 4, 3))
+    Ul = wo.links_with_fermion_bc(U, (1, 1, 1, -1))
+    op = WilsonDiracOperator(ndim=4, mass=0.3)
+    m0 = 4.3
+    assert rel(op.apply(Ul, psi, sign=-1), m0 * psi + wo.dslash_reference(Ul, psi, -1)) < TOL
+    assert rel(op.apply_dagger(Ul, psi), m0 * psi + wo.dslash_reference(Ul, psi, +1)) < TOL
+    assert rel(op.apply_dslash(Ul, psi, sign=-1), wo.dslash_optimized(Ul, psi, -1)) < TOL
+
+
+@pytest.mark.parametrize("L", [(16, 16, 16, 16), (24, 24, 24, 48), (32, 32, 32, 32)])
+def test_properties_at_benchmark_sizes(L):
+    """gamma5-hermiticity, adjointness, linearity and gauge covariance at full
+    size (no oracle: these hold exactly for the true operator)."""
+    print(f"[gpu] properties L={L}", flush=True)
+    import torch
+
+    from jaxqft_amd import fields
+    from jaxqft_amd.su3_wilson_nf2 import SU3WilsonNf2
+
+    rng = fields.rng_for(400)
+    th = SU3WilsonNf2(lattice_shape=L, mass=0.02, fermion_bc=(1, 1, 1, -1), dtype=np.complex128)
+    U = fields.su3_hot_start(rng, 1, L, 0.6)
+    G = th.resident_gauge(U)
+    psi = torch.from_numpy(fields.random_fermion(rng, th.fermion_shape())).cuda()
+    chi = torch.from_numpy(fields.random_fermion(rng, th.fermion_shape())).cuda()
+    g5 = lambda v: torch.stack([v[..., 2, :], -v[..., 3, :], v[..., 0, :], -v[..., 1, :]], dim=-2)
+    Dpsi = th.apply_D(G, psi)
+    Ddag_psi = th.apply_Ddag(G, psi)
+    assert float(torch.linalg.norm(Ddag_psi - g5(th.apply_D(G, g5(psi)))) / torch.linalg.norm(Ddag_psi)) < 1e-14
+    lhs = torch.vdot(chi.flatten(), Dpsi.flatten())
+    rhs = torch.vdot(th.apply_Ddag(G, chi).flatten(), psi.flatten())
+    assert float(abs(lhs - rhs) / abs(lhs)) < 1e-12
+    a, b = 0.3 - 1.2j, 2.1 + 0.4j
+    lin = th.apply_D(G, a * psi + b * chi) - (a * Dpsi + b * th.apply_D(G, chi))
+    assert float(torch.linalg.norm(lin) / torch.linalg.norm(Dpsi)) < 1e-14
+    n = th.apply_normal(G, psi)
+    q = torch.vdot(psi.flatten(), n.flatten())
+    assert abs(float(q.imag)) < 1e-10 * float(q.real) and float(q.real) > 0

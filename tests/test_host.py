@@ -1,0 +1,132 @@
+"""CPU-side checks: the C ABI library loads and exports every declared entry
+point, its argument validation (no device needed), the Python drop-in's
+validation and shapes, the field generators, and the golden fixtures."""
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from jaxqft_amd import _native as nat
+from jaxqft_amd import fields
+from oracle import wilson_oracle as wo
+
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+def test_library_exports_declared_symbols():
+    lib = nat.load()
+    names = nat.declared_symbols()
+    assert len(names) >= 15
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert lib.wl_version() >= 1
+
+
+def _geom(L=(4, 4, 4, 4), nc=3, batch=1, nd=4):
+    return nat.WlGeom(nd, (ctypes.c_int * 4)(*L), nc, batch)
+
+
+def _params(r=1.0):
+    return nat.WlParams(0.1, r, (ctypes.c_double * 4)(1, 1, 1, -1), (ctypes.c_double * 4)(0, 0, 0, 0))
+
+
+@pytest.mark.parametrize("geom,params,code,msg", [
+    (dict(nd=3), {}, nat.WL_ERR_UNSUPPORTED, "Nd=4"),
+    (dict(nc=2), {}, nat.WL_ERR_UNSUPPORTED, "Nc"),
+    (dict(batch=0), {}, nat.WL_ERR_BAD_ARG, "batch"),
+    (dict(L=(4, 0, 4, 4)), {}, nat.WL_ERR_BAD_ARG, "extent"),
+    ({}, dict(r=0.5), nat.WL_ERR_UNSUPPORTED, "wilson_r"),
+])
+def test_ctx_create_validation(geom, params, code, msg):
+    lib = nat.load()
+    h = ctypes.c_void_p()
+    rc = lib.wl_ctx_create(ctypes.byref(_geom(**geom)), ctypes.byref(_params(**params)), ctypes.byref(h))
+    assert rc == code
+    assert msg in lib.wl_last_error().decode()
+
+
+def test_null_handles_rejected():
+    lib = nat.load()
+    assert lib.wl_apply(None, None, 0, None, None, 1, 0, None) == nat.WL_ERR_BAD_ARG
+    assert lib.wl_field_import(None, None, 0, None) == nat.WL_ERR_BAD_ARG
+    sp = nat.WlSolveParams()
+    assert lib.wl_cg_normal(None, None, None, None, None, 1, 0, ctypes.byref(sp), None, None) == nat.WL_ERR_BAD_ARG
+    lib.wl_ctx_destroy(None)
+    lib.wl_gauge_release(None)
+    lib.wl_field_destroy(None)
+
+
+def test_theory_validation_and_shapes():
+    from jaxqft_amd.su3_wilson_nf2 import SU3WilsonNf2
+    from jaxqft_amd.u1_wilson_nf2 import U1WilsonNf2
+
+    th = SU3WilsonNf2(lattice_shape=(4, 4, 4, 8), batch_size=2)
+    assert th.fermion_shape() == (2, 4, 4, 4, 8, 4, 3)
+    assert th.field_shape() == (2, 4, 4, 4, 4, 8, 3, 3)
+    assert th.fermion_bc == (1, 1, 1, 1)
+    assert SU3WilsonNf2(lattice_shape=(4,) * 4, layout="B...MIJ").field_shape() == (1, 4, 4, 4, 4, 4, 3, 3)
+    u1 = U1WilsonNf2(lattice_shape=(4, 4, 4, 8))
+    assert u1.fermion_shape() == (1, 4, 4, 4, 8, 4, 1)
+    assert u1.field_shape() == (1, 4, 4, 4, 4, 8)
+    assert u1.fermion_bc == (1, 1, 1, -1)
+    for bad in (dict(layout="XYZ"), dict(solver_kind="gmres"), dict(solver_form="split"),
+                dict(preconditioner_kind="ilu"), dict(dirac_kernel="fast"), dict(fermion_bc=(1, 1)),
+                dict(solver_precision="f8"), dict(solver_recon=8)):
+        with pytest.raises(ValueError):
+            SU3WilsonNf2(lattice_shape=(4,) * 4, **bad)
+    assert th.dirac.flops_per_site_matvec(nc=3) == 3600  # the reference's own estimator at Nc=3
+    assert th.dirac.flops_per_site_dslash(nc=3) == 3552
+
+
+def test_su3_field_generator():
+    rng = fields.rng_for(5)
+    U = fields.su3_hot_start(rng, 1, (4, 4, 4, 4), 0.5).reshape(-1, 3, 3)
+    np.testing.assert_allclose(np.einsum("nij,nkj->nik", U, np.conj(U)), np.broadcast_to(np.eye(3), U.shape), atol=1e-13)
+    np.testing.assert_allclose(np.linalg.det(U), 1.0, atol=1e-13)
+    # exp of the same algebra element via scipy
+    from scipy.linalg import expm
+
+    rng = fields.rng_for(6)
+    a = rng.standard_normal((3, 3)) + 1j * rng.standard_normal((3, 3))
+    X = fields.project_su3_algebra(0.8 * a[None])
+    np.testing.assert_allclose(fields.expm_su3_algebra(X)[0], expm(X[0]), atol=1e-13)
+
+
+def test_point_sources():
+    src = fields.point_sources((2, 2, 2, 4), (1, 0, 1, 3))
+    assert src.shape == (12, 2, 2, 2, 4, 4, 3)
+    assert np.count_nonzero(src) == 12
+    for k in range(12):
+        assert src[k, 1, 0, 1, 3, k // 3, k % 3] == 1
+
+
+@pytest.mark.parametrize("name", ["op_su3_2x2x2x2", "op_su3_4x4x4x4", "op_su3_2x3x4x5", "op_u1_2x2x2x2",
+                                  "op_u1_4x4x4x4"])
+def test_golden_operator_fixtures_reproduce(name):
+    z = np.load(GOLD / f"{name}.npz")
+    L = tuple(int(v) for v in z["L"])
+    nc = z["psi"].shape[-1]
+    ora = wo.WilsonOracle(L, nc, float(z["mass"]), fermion_bc=tuple(z["bc"]))
+    U, psi = z["U"], z["psi"]
+    for key, fn in (("D", ora.apply_D), ("Ddag", ora.apply_Ddag), ("g5D", ora.apply_g5D),
+                    ("DdagD", ora.apply_normal), ("K", ora.apply_offdiag), ("Kdag", ora.apply_offdiag_dagger)):
+        ref = z[key]
+        assert np.linalg.norm(fn(U, psi) - ref) / np.linalg.norm(ref) < 1e-14, key
+
+
+def test_golden_cg_fixture():
+    z = np.load(GOLD / "cg_su3_4x4x4x4.npz")
+    ora = wo.WilsonOracle((4, 4, 4, 4), 3, 0.1, fermion_bc=(1, 1, 1, -1))
+    r = ora.apply_normal(z["U"], z["x"]) - z["b"]
+    assert np.linalg.norm(r) / np.linalg.norm(z["b"]) <= 1e-11
+
+
+@pytest.mark.parametrize("L", [(4, 4, 4, 4), (8, 8, 8, 8), (4, 4, 4, 8)])
+def test_golden_index_tables(L):
+    z = np.load(GOLD / f"index_{'x'.join(map(str, L))}.npz")
+    t = wo.checkerboard_neighbor_maps(L)
+    for k in t:
+        np.testing.assert_array_equal(t[k], z[k])
+        assert t[k].dtype == np.int32
