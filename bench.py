@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="run only the Dslash variants and a calibration copy (for rocprofv3 counter passes)")
     return ap.parse_args()
 
 
@@ -98,6 +100,9 @@ def main():
         torch.cuda.synchronize()
 
         # ---------------------------------------------------------- Dslash
+        # autotune the march length per (precision, recon), then time the best
+        from jaxqft_amd.autotune import time_dslash, tune_march
+
         dslash = {}
         variants = [("f64", 18), ("f64", 12), ("f32", 18), ("f32", 12), ("f16", 18), ("f16", 12)]
         for prec, recon in variants:
@@ -105,25 +110,28 @@ def main():
             fi = nat.Field(ctx, pc, 1)
             fo = nat.Field(ctx, pc, 1)
             fi.import_(bt)
-            gauge.prepare(pc, recon)
-            for _ in range(3):
-                nat.check(lib.wl_dslash(ctx.h, gauge.h, nat.WL_OP_D, recon, fi.h, fo.h, 1, c_void_p(sp_handle)))
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-            for _ in range(args.dslash_reps):
-                nat.check(lib.wl_dslash(ctx.h, gauge.h, nat.WL_OP_D, recon, fi.h, fo.h, 1, c_void_p(sp_handle)))
-            e1.record(stream)
-            e1.synchronize()
-            ms = e0.elapsed_time(e1) / args.dslash_reps
+            tune = tune_march(ctx, gauge, pc, recon, fi, fo, reps=max(3, args.dslash_reps // 5), stream=stream)
+            us = time_dslash(ctx, gauge, pc, recon, fi, fo, 1, args.dslash_reps, stream)
             bps = dslash_bytes_per_site(prec, recon)
-            gbs = bps * V / (ms * 1e-3) / 1e9
+            gbs = bps * V / (us * 1e-6) / 1e9
             dslash[f"{prec}_r{recon}"] = {
-                "us": round(ms * 1e3, 2), "GB/s": round(gbs, 1), "frac_hbm_peak": round(gbs / PEAK_HBM_GBS, 4),
-                "GFLOP/s": round(FLOP_PER_SITE * V / (ms * 1e-3) / 1e9, 1), "bytes_per_site": bps,
+                "us": round(us, 2), "GB/s": round(gbs, 1), "frac_hbm_peak": round(gbs / PEAK_HBM_GBS, 4),
+                "GFLOP/s": round(FLOP_PER_SITE * V / (us * 1e-6) / 1e9, 1), "bytes_per_site": bps,
+                "march": tune["best"], "tune_us": {str(k): round(v, 2) for k, v in tune["us"].items()},
             }
-            print(f"[bench] dslash {prec} r{recon}: {ms*1e3:.1f} us  {gbs:.0f} GB/s", file=sys.stderr, flush=True)
+            print(f"[bench] dslash {prec} r{recon}: {us:.1f} us  {gbs:.0f} GB/s  march {tune}", file=sys.stderr,
+                  flush=True)
             del fi, fo
+
+        if args.profile_only:
+            # calibration: a plain device copy of known size (256 MiB read + 256 MiB written)
+            src = torch.empty(32 << 20, dtype=torch.float64, device="cuda")
+            dst = torch.empty_like(src)
+            for _ in range(5):
+                dst.copy_(src)
+            torch.cuda.synchronize()
+            print(json.dumps({"profile_only": True, "dslash": dslash, "copy_bytes_each_way": src.numel() * 8}))
+            return
 
         # ---------------------------------------------------------- solves
         def solve(field_x, prec, recon, graph):
@@ -203,39 +211,28 @@ def main():
 
 
 def cpu_baseline(U, b, L, args, gpu_iters):
-    """The oracle (NumPy FP64, the reference algorithm restated) timed on this
-    host: CG iterations on the same fields for about args.cpu_seconds, then
-    extrapolated to the GPU FP64 iteration count."""
-    from oracle.wilson_oracle import WilsonOracle
+    """The C/OpenMP oracle (the reference algorithm restated) timed on this
+    host's cores: FP64 CG iterations on D^dag D on the same fields for about
+    args.cpu_seconds, extrapolated to the GPU FP64 iteration count."""
+    from oracle import oracle_c
 
-    ora = WilsonOracle(L, 3, args.mass, fermion_bc=(1, 1, 1, -1), kernel="optimized")
-    A = lambda v: ora.apply_normal(U, v)
-    x = np.zeros_like(b)
-    r = b.copy()  # x0 = 0
-    p = r.copy()
-    rr = np.vdot(r, r).real
+    threads = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+    Ub, bb = U[0], b[0]
     t0 = time.perf_counter()
-    n = 0
-    while True:
-        Ap = A(p)
-        alpha = rr / np.vdot(p, Ap).real
-        x += alpha * p
-        r -= alpha * Ap
-        rr_new = np.vdot(r, r).real
-        p = r + (rr_new / rr) * p
-        rr = rr_new
-        n += 1
-        if time.perf_counter() - t0 > args.cpu_seconds:
-            break
-    s_per_iter = (time.perf_counter() - t0) / n
+    oracle_c.cg(Ub, bb, L, 3, args.mass, (1, 1, 1, -1), 0.0, 1)
+    one = time.perf_counter() - t0
+    n = max(1, int(args.cpu_seconds / max(one, 1e-3)))
+    t0 = time.perf_counter()
+    _, k = oracle_c.cg(Ub, bb, L, 3, args.mass, (1, 1, 1, -1), 0.0, n)
+    s_per_iter = (time.perf_counter() - t0) / k
     return {
         "value": round(1.0 / (s_per_iter * max(1, gpu_iters)), 6),
         "unit": "solves/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": f"{n} FP64 CG iterations on D^dag D at {'x'.join(map(str, L))} (NumPy oracle, one core), "
-                  f"{s_per_iter:.2f} s/iteration, extrapolated to the GPU FP64 iteration count {gpu_iters}",
-        "affinity_cores": len(os.sched_getaffinity(0)),
+        "sample": f"{k} FP64 CG iterations on D^dag D at {'x'.join(map(str, L))} with the C/OpenMP oracle "
+                  f"({threads} threads): {s_per_iter*1e3:.1f} ms/iteration, extrapolated to the GPU FP64 "
+                  f"iteration count {gpu_iters}",
     }
 
 

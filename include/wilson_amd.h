@@ -124,6 +124,12 @@ const char* wl_last_error(void);
 int wl_ctx_create(const wl_geom* geom, const wl_params* params, wl_ctx** out);
 void wl_ctx_destroy(wl_ctx* ctx);
 int wl_ctx_set_mass(wl_ctx* ctx, double mass);
+/* Tuning knob (no counterpart in the reference): number of x0 sites each GPU
+ * thread marches over in the stencil kernels of precision prec (0 =
+ * automatic; otherwise it must divide L0).  wl_ctx_get_march returns the
+ * value in effect. */
+int wl_ctx_set_march(wl_ctx* ctx, int prec, int march);
+int wl_ctx_get_march(wl_ctx* ctx, int prec);
 
 /* where: 0 = device pointer, 1 = host pointer */
 int wl_gauge_upload(wl_ctx* ctx, const void* U, int layout, int dtype, int where, wl_gauge** out);

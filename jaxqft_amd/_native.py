@@ -94,6 +94,8 @@ def load() -> ctypes.CDLL:
         "wl_ctx_create": (c_int, [POINTER(WlGeom), POINTER(WlParams), POINTER(vp)]),
         "wl_ctx_destroy": (None, [vp]),
         "wl_ctx_set_mass": (c_int, [vp, c_double]),
+        "wl_ctx_set_march": (c_int, [vp, c_int, c_int]),
+        "wl_ctx_get_march": (c_int, [vp, c_int]),
         "wl_gauge_upload": (c_int, [vp, vp, c_int, c_int, c_int, POINTER(vp)]),
         "wl_gauge_release": (None, [vp]),
         "wl_gauge_prepare": (c_int, [vp, c_int, c_int, vp]),
@@ -153,6 +155,12 @@ class Context:
         self.mass = float(mass)
         self.volume = L[0] * L[1] * L[2] * L[3]
         self._lib = lib
+
+    def set_march(self, prec: int, march: int) -> None:
+        check(self._lib.wl_ctx_set_march(self.h, int(prec), int(march)))
+
+    def get_march(self, prec: int) -> int:
+        return int(self._lib.wl_ctx_get_march(self.h, int(prec)))
 
     def set_mass(self, mass: float) -> None:
         check(self._lib.wl_ctx_set_mass(self.h, float(mass)))

@@ -23,13 +23,13 @@ hipError_t launch_xpay(int p, int nc, BlasArgs a, hipStream_t stream);
 hipError_t launch_accumulate(int px, int py, int nc, BlasArgs a, hipStream_t stream);
 hipError_t launch_convert(int po, int pi, int nc, BlasArgs a, hipStream_t stream);
 hipError_t launch_finalize_update(ColState* cols, int ncol, hipStream_t stream);
-hipError_t pack_spinor(int prec, int nc, const void* src, int dtype, void* dst, long long V, int ncol,
+hipError_t pack_spinor(int prec, int nc, const void* src, int dtype, void* dst, long long V, long long Vp, int ncol,
                        size_t col_bytes, hipStream_t st);
-hipError_t unpack_spinor(int prec, int nc, const void* src, void* dst, int dtype, long long V, int ncol,
+hipError_t unpack_spinor(int prec, int nc, const void* src, void* dst, int dtype, long long V, long long Vp, int ncol,
                          size_t col_bytes, hipStream_t st);
-hipError_t pack_gauge_f64(int nc, const void* src, int dtype, int layout, void* dst, long long V, int ncfg,
-                          size_t cfg_bytes, size_t dir_bytes, hipStream_t st);
-hipError_t convert_gauge(int prec, int nc, int recon, const void* src, void* dst, long long V, int ncfg,
+hipError_t pack_gauge_f64(int nc, const void* src, int dtype, int layout, void* dst, long long V, long long Vp,
+                          int ncfg, size_t cfg_bytes, size_t dir_bytes, hipStream_t st);
+hipError_t convert_gauge(int prec, int nc, int recon, const void* src, void* dst, long long V, long long Vp, int ncfg,
                          size_t src_cfg_bytes, size_t src_dir_bytes, size_t cfg_bytes, size_t dir_bytes,
                          hipStream_t st);
 }  // namespace wl
@@ -60,25 +60,31 @@ static int fail(int code, const char* fmt, ...) {
     if (_r != WL_OK) return _r; \
   } while (0)
 
+// Device buffer.  Each allocation starts at a staggered offset (a multiple
+// of 9 cache lines) so that the same site of different fields does not map
+// to the same L2 set.
+static unsigned g_alloc_color = 0;
 struct DevBuf {
-  void* p = nullptr;
+  void* p = nullptr;    // usable base
+  void* raw = nullptr;  // allocation
   size_t n = 0;
   int ensure(size_t bytes) {
     if (bytes <= n) return WL_OK;
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-    hipError_t e = hipMalloc(&p, bytes);
+    release();
+    const size_t off = (size_t)(g_alloc_color++ % 8) * 1152;
+    hipError_t e = hipMalloc(&raw, bytes + 8 * 1152);
     if (e != hipSuccess) {
       (void)hipGetLastError();
-      p = nullptr;
+      raw = nullptr;
       return fail(e == hipErrorOutOfMemory ? WL_ERR_OOM : WL_ERR_HIP, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
     }
+    p = static_cast<char*>(raw) + off;
     n = bytes;
     return WL_OK;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (raw) (void)hipFree(raw);
+    raw = nullptr;
     p = nullptr;
     n = 0;
   }
@@ -89,13 +95,15 @@ struct wl_ctx {
   wl_params params;
   Geom g;
   long long V;
+  long long Vp;  // padded plane stride
   int device;
   std::mutex mu;
   // workspace
-  DevBuf partials, counters, cols;
+  DevBuf partials, cols;
   DevBuf ws[10];
   ColState* host_cols = nullptr;  // pinned mirror
   size_t host_cols_n = 0;
+  int march[3] = {0, 0, 0};       // x0 sites per thread per precision (0 = automatic)
 };
 
 struct wl_gauge {
@@ -114,13 +122,36 @@ struct wl_field {
 };
 
 static inline size_t cfg_bytes(const wl_ctx* c, int prec, int recon) {
-  return 4 * link_dir_bytes(prec, c->geom.nc, recon, (size_t)c->V);
+  return 4 * link_dir_bytes(prec, c->geom.nc, recon, (size_t)c->Vp);
 }
+static inline size_t col_bytes_of(const wl_ctx* c, int prec) { return spinor_col_bytes(prec, c->geom.nc, (size_t)c->Vp); }
 
 static int check_ctx(const wl_ctx* c) {
   if (!c) return fail(WL_ERR_BAD_ARG, "null context");
   return WL_OK;
 }
+
+static inline int nchunk_of(const wl_ctx* c) { return (int)((c->g.stride[0] + NT - 1) / NT); }
+
+// x0 sites marched per thread: the largest of 4, 2, 1 dividing L0 that still
+// leaves at least 512 workgroups per spinor column.
+static int march_for(const wl_ctx* c, int prec) {
+  int m = c->march[prec];
+  if (m > 0) return m;
+  const int L0 = c->geom.L[0];
+  for (int cand : {4, 2}) {
+    if (L0 % cand == 0 && (long long)nchunk_of(c) * (L0 / cand) >= 512) return cand;
+  }
+  return 1;
+}
+
+// upper bound of reduction partials per column over every kernel
+static inline size_t max_tiles(const wl_ctx* c) {
+  size_t a = (size_t)nchunk_of(c) * c->geom.L[0];
+  size_t b = (size_t)((c->V + NT - 1) / NT);
+  return a > b ? a : b;
+}
+
 
 extern "C" {
 
@@ -145,12 +176,14 @@ int wl_ctx_create(const wl_geom* geom, const wl_params* params, wl_ctx** out) {
   c->geom = *geom;
   c->params = *params;
   c->V = V;
+  c->Vp = V + PLANE_PAD;
   for (int i = 0; i < 4; ++i) c->g.L[i] = geom->L[i];
   c->g.stride[3] = 1;
   c->g.stride[2] = geom->L[3];
   c->g.stride[1] = geom->L[3] * geom->L[2];
   c->g.stride[0] = geom->L[3] * geom->L[2] * geom->L[1];
   c->g.V = V;
+  c->g.Vp = c->Vp;
   if (hipGetDevice(&c->device) != hipSuccess) {
     delete c;
     return fail(WL_ERR_HIP, "no HIP device");
@@ -162,7 +195,6 @@ int wl_ctx_create(const wl_geom* geom, const wl_params* params, wl_ctx** out) {
 void wl_ctx_destroy(wl_ctx* c) {
   if (!c) return;
   c->partials.release();
-  c->counters.release();
   c->cols.release();
   for (auto& b : c->ws) b.release();
   if (c->host_cols) (void)hipHostFree(c->host_cols);
@@ -174,6 +206,21 @@ int wl_ctx_set_mass(wl_ctx* c, double mass) {
   std::lock_guard<std::mutex> lk(c->mu);
   c->params.mass = mass;
   return WL_OK;
+}
+
+int wl_ctx_set_march(wl_ctx* c, int prec, int march) {
+  CHK(check_ctx(c));
+  if (prec < F64 || prec > F16) return fail(WL_ERR_BAD_ARG, "bad precision %d", prec);
+  if (march < 0 || (march > 0 && c->geom.L[0] % march != 0))
+    return fail(WL_ERR_BAD_ARG, "march %d must be 0 (automatic) or divide L0=%d", march, c->geom.L[0]);
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->march[prec] = march;
+  return WL_OK;
+}
+
+int wl_ctx_get_march(wl_ctx* c, int prec) {
+  if (!c || prec < F64 || prec > F16) return 0;
+  return march_for(c, prec);
 }
 
 int wl_gauge_upload(wl_ctx* c, const void* U, int layout, int dtype, int where, wl_gauge** out) {
@@ -212,7 +259,7 @@ int wl_gauge_upload(wl_ctx* c, const void* U, int layout, int dtype, int where, 
     }
     src = staging.p;
   }
-  hipError_t e = pack_gauge_f64(nc, src, dtype, layout, g->master.p, c->V, ncfg, cb, cb / 4, 0);
+  hipError_t e = pack_gauge_f64(nc, src, dtype, layout, g->master.p, c->V, c->Vp, ncfg, cb, cb / 4, 0);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   staging.release();
   if (e != hipSuccess) {
@@ -252,7 +299,7 @@ static int gauge_encoding(wl_gauge* g, int prec, int recon, hipStream_t st, cons
   size_t cb = cfg_bytes(c, prec, recon);
   CHK(b.ensure(cb * g->ncfg));
   size_t mcb = cfg_bytes(c, F64, 18);
-  hipError_t e = convert_gauge(prec, c->geom.nc, recon, g->master.p, b.p, c->V, g->ncfg, mcb, mcb / 4, cb, cb / 4, st);
+  hipError_t e = convert_gauge(prec, c->geom.nc, recon, g->master.p, b.p, c->V, c->Vp, g->ncfg, mcb, mcb / 4, cb, cb / 4, st);
   if (e != hipSuccess) {
     b.release();
     return fail(WL_ERR_HIP, "gauge convert: %s", hipGetErrorString(e));
@@ -262,7 +309,7 @@ static int gauge_encoding(wl_gauge* g, int prec, int recon, hipStream_t st, cons
   return WL_OK;
 }
 
-static void fill_dslash_common(const wl_ctx* c, DslashArgs& a) {
+static void fill_dslash_common(const wl_ctx* c, DslashArgs& a, int prec) {
   memset(&a, 0, sizeof(a));
   a.g = c->g;
   for (int i = 0; i < 4; ++i) {
@@ -271,7 +318,10 @@ static void fill_dslash_common(const wl_ctx* c, DslashArgs& a) {
   }
   a.m0 = c->params.mass + 4.0 * c->params.wilson_r;
   a.r = c->params.wilson_r;
-  a.ntiles = (int)((c->V + NT - 1) / NT);
+  a.march = march_for(c, prec);
+  a.nseg = c->geom.L[0] / a.march;
+  // march 1 runs the one-site-per-thread kernel over flat tiles of NT sites
+  a.ntiles = a.march == 1 ? (int)((c->V + NT - 1) / NT) : nchunk_of(c) * a.nseg;
 }
 
 // op on internal fields of precision prec (no gating, no reduction)
@@ -280,7 +330,7 @@ static int dslash_internal(wl_ctx* c, wl_gauge* g, int op, int prec, int recon, 
   const void* U;
   CHK(gauge_encoding(g, prec, recon, st, &U));
   DslashArgs a;
-  fill_dslash_common(c, a);
+  fill_dslash_common(c, a, prec);
   int rc = (c->geom.nc == 1) ? 18 : recon;
   a.U = U;
   a.cfg_bytes = cfg_bytes(c, prec, rc);
@@ -322,14 +372,14 @@ int wl_apply(wl_ctx* c, wl_gauge* g, int op, const void* in, void* out, int nrhs
   hipStream_t st = (hipStream_t)stream;
   const int nc = c->geom.nc;
   const int ncol = c->geom.batch * nrhs;
-  const size_t cb = spinor_col_bytes(F64, nc, (size_t)c->V);
+  const size_t cb = col_bytes_of(c, F64);
   CHK(c->ws[0].ensure(cb * ncol));
   CHK(c->ws[1].ensure(cb * ncol));
   CHK(c->ws[2].ensure(cb * ncol));
-  hipError_t e = pack_spinor(F64, nc, in, dtype, c->ws[0].p, c->V, ncol, cb, st);
+  hipError_t e = pack_spinor(F64, nc, in, dtype, c->ws[0].p, c->V, c->Vp, ncol, cb, st);
   if (e != hipSuccess) return fail(WL_ERR_HIP, "pack: %s", hipGetErrorString(e));
   CHK(op_internal(c, g, op, F64, 18, c->ws[0].p, c->ws[1].p, c->ws[2].p, ncol, nrhs, cb, st));
-  e = unpack_spinor(F64, nc, c->ws[1].p, out, dtype, c->V, ncol, cb, st);
+  e = unpack_spinor(F64, nc, c->ws[1].p, out, dtype, c->V, c->Vp, ncol, cb, st);
   if (e != hipSuccess) return fail(WL_ERR_HIP, "unpack: %s", hipGetErrorString(e));
   return WL_OK;
 }
@@ -350,7 +400,7 @@ int wl_field_create(wl_ctx* c, int prec, int ncol, wl_field** out) {
   f->ctx = c;
   f->prec = prec;
   f->ncol = ncol;
-  f->col_bytes = spinor_col_bytes(prec, c->geom.nc, (size_t)c->V);
+  f->col_bytes = col_bytes_of(c, prec);
   int rc = f->buf.ensure(f->col_bytes * ncol);
   if (rc) {
     delete f;
@@ -378,7 +428,7 @@ int wl_field_import(wl_field* f, const void* src, int dtype, void* stream) {
   if (!f || !src) return fail(WL_ERR_BAD_ARG, "null argument");
   if (dtype != WL_C128 && dtype != WL_C64) return fail(WL_ERR_BAD_ARG, "bad dtype %d", dtype);
   wl_ctx* c = f->ctx;
-  hipError_t e = pack_spinor(f->prec, c->geom.nc, src, dtype, f->buf.p, c->V, f->ncol, f->col_bytes, (hipStream_t)stream);
+  hipError_t e = pack_spinor(f->prec, c->geom.nc, src, dtype, f->buf.p, c->V, c->Vp, f->ncol, f->col_bytes, (hipStream_t)stream);
   if (e != hipSuccess) return fail(WL_ERR_HIP, "import: %s", hipGetErrorString(e));
   return WL_OK;
 }
@@ -387,7 +437,7 @@ int wl_field_export(const wl_field* f, void* dst, int dtype, void* stream) {
   if (!f || !dst) return fail(WL_ERR_BAD_ARG, "null argument");
   if (dtype != WL_C128 && dtype != WL_C64) return fail(WL_ERR_BAD_ARG, "bad dtype %d", dtype);
   wl_ctx* c = f->ctx;
-  hipError_t e = unpack_spinor(f->prec, c->geom.nc, f->buf.p, dst, dtype, c->V, f->ncol, f->col_bytes, (hipStream_t)stream);
+  hipError_t e = unpack_spinor(f->prec, c->geom.nc, f->buf.p, dst, dtype, c->V, c->Vp, f->ncol, f->col_bytes, (hipStream_t)stream);
   if (e != hipSuccess) return fail(WL_ERR_HIP, "export: %s", hipGetErrorString(e));
   return WL_OK;
 }
@@ -434,7 +484,7 @@ struct Solver {
   Reduction red;
 
   void dargs(DslashArgs& a, int prec, const void* U, int recon) {
-    fill_dslash_common(c, a);
+    fill_dslash_common(c, a, prec);
     a.U = U;
     a.cfg_bytes = cfg_bytes(c, prec, nc == 1 ? 18 : recon);
     a.dir_bytes = a.cfg_bytes / 4;
@@ -447,6 +497,7 @@ struct Solver {
     BlasArgs a;
     memset(&a, 0, sizeof(a));
     a.V = c->V;
+    a.Vp = c->Vp;
     a.ncol = ncol;
     a.ntiles = ntiles;
     a.gate = gate;
@@ -466,7 +517,7 @@ struct Solver {
     }
     DslashArgs d;
     dargs(d, F64, U64, 18);
-    d.only_update = 1;
+    d.gate = 2;
     d.in = x; d.out = thi; d.col_bytes = cb64;
     e = launch_dslash(F64, nc, 18, WL_OP_D, 0, d, st);
     if (e != hipSuccess) return fail(WL_ERR_HIP, "dslash: %s", hipGetErrorString(e));
@@ -497,7 +548,7 @@ struct Solver {
     hipError_t e;
     DslashArgs d;
     dargs(d, P, UP, R);
-    d.skip_inactive = 1;
+    d.gate = 1;
     d.in = p; d.out = t; d.col_bytes = cbP;
     e = launch_dslash(P, nc, nc == 1 ? 18 : R, WL_OP_D, 0, d, st);
     if (e != hipSuccess) return fail(WL_ERR_HIP, "dslash: %s", hipGetErrorString(e));
@@ -548,9 +599,9 @@ static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int n
   s.R = (nc == 1) ? 18 : (sp->inner_recon == 12 ? 12 : 18);
   s.mixed = s.P != F64;
   s.PY = s.mixed ? F32 : F64;
-  s.cb64 = spinor_col_bytes(F64, nc, (size_t)c->V);
-  s.cbP = spinor_col_bytes(s.P, nc, (size_t)c->V);
-  s.cbY = spinor_col_bytes(s.PY, nc, (size_t)c->V);
+  s.cb64 = col_bytes_of(c, F64);
+  s.cbP = col_bytes_of(c, s.P);
+  s.cbY = col_bytes_of(c, s.PY);
   CHK(gauge_encoding(g, F64, 18, st, &s.U64));
   CHK(gauge_encoding(g, s.P, s.R, st, &s.UP));
   s.b = b;
@@ -575,8 +626,7 @@ static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int n
     s.r = s.rhi;
     s.y = s.x;
   }
-  CHK(c->partials.ensure(sizeof(double) * (size_t)ncol * s.ntiles));
-  CHK(c->counters.ensure(sizeof(unsigned) * ncol));
+  CHK(c->partials.ensure(sizeof(double) * (size_t)ncol * max_tiles(c)));
   CHK(c->cols.ensure(sizeof(ColState) * ncol));
   if (c->host_cols_n < (size_t)ncol) {
     if (c->host_cols) (void)hipHostFree(c->host_cols);
@@ -586,11 +636,9 @@ static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int n
     c->host_cols_n = ncol;
   }
   s.red.partials = (double*)c->partials.p;
-  s.red.counters = (unsigned*)c->counters.p;
   s.red.cols = (ColState*)c->cols.p;
   s.red.action = RA_NONE;
 
-  HIPCHK(hipMemsetAsync(c->counters.p, 0, sizeof(unsigned) * ncol, st));
   HIPCHK(hipMemsetAsync(s.p, 0, s.cbP * ncol, st));
   if (s.mixed) HIPCHK(hipMemsetAsync(s.y, 0, s.cbY * ncol, st));
   if (!use_x0) HIPCHK(hipMemsetAsync(x, 0, s.cb64 * ncol, st));
@@ -729,17 +777,17 @@ int wl_cg_normal(wl_ctx* c, wl_gauge* g, const void* b, void* x, const void* x0,
   hipStream_t st = (hipStream_t)stream;
   const int nc = c->geom.nc;
   const int ncol = c->geom.batch * nrhs;
-  const size_t cb = spinor_col_bytes(F64, nc, (size_t)c->V);
+  const size_t cb = col_bytes_of(c, F64);
   CHK(c->ws[7].ensure(cb * ncol));
   CHK(c->ws[8].ensure(cb * ncol));
-  hipError_t e = pack_spinor(F64, nc, b, dtype, c->ws[7].p, c->V, ncol, cb, st);
+  hipError_t e = pack_spinor(F64, nc, b, dtype, c->ws[7].p, c->V, c->Vp, ncol, cb, st);
   if (e != hipSuccess) return fail(WL_ERR_HIP, "pack: %s", hipGetErrorString(e));
   if (x0) {
-    e = pack_spinor(F64, nc, x0, dtype, c->ws[8].p, c->V, ncol, cb, st);
+    e = pack_spinor(F64, nc, x0, dtype, c->ws[8].p, c->V, c->Vp, ncol, cb, st);
     if (e != hipSuccess) return fail(WL_ERR_HIP, "pack: %s", hipGetErrorString(e));
   }
   CHK(cg_fields(c, g, c->ws[7].p, c->ws[8].p, x0 != nullptr, nrhs, sp, stats, st));
-  e = unpack_spinor(F64, nc, c->ws[8].p, x, dtype, c->V, ncol, cb, st);
+  e = unpack_spinor(F64, nc, c->ws[8].p, x, dtype, c->V, c->Vp, ncol, cb, st);
   if (e != hipSuccess) return fail(WL_ERR_HIP, "unpack: %s", hipGetErrorString(e));
   return WL_OK;
 }

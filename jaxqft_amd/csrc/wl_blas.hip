@@ -7,11 +7,6 @@
 namespace wl {
 
 
-__device__ __forceinline__ bool gate_skip(int gate, const ColState& s) {
-  if (gate == 1) return !s.active;
-  if (gate == 2) return !(s.need_update && !s.done);
-  return false;
-}
 
 template <int P, int NC, class T>
 __device__ __forceinline__ void load_as(const void* base, size_t V, size_t site, cx<T> (&v)[4 * NC]) {
@@ -35,9 +30,9 @@ __device__ __forceinline__ void store_as(void* base, size_t V, size_t site, cons
   const int col = blockIdx.x % a.ncol;                              \
   const int tile = blockIdx.x / a.ncol;                             \
   if (gate_skip(a.gate, a.red.cols[col])) return;                   \
-  const size_t V = (size_t)a.V;                                     \
+  const size_t V = (size_t)a.Vp;                                    \
   const size_t site = (size_t)tile * NT + threadIdx.x;              \
-  const bool valid = site < V;                                      \
+  const bool valid = site < (size_t)a.V;                            \
   const size_t s = valid ? site : 0;
 
 // |a|^2 -> action
@@ -50,7 +45,7 @@ __global__ __launch_bounds__(NT) void k_norm2(BlasArgs a) {
   double part = 0.0;
 #pragma unroll
   for (int c = 0; c < 4 * NC; ++c) part += (double)v[c].re * v[c].re + (double)v[c].im * v[c].im;
-  finish_reduction(a.red, col, tile, a.ntiles, valid ? part : 0.0);
+  write_partial(a.red, col, tile, a.ntiles, valid ? part : 0.0);
 }
 
 // y += alpha p ; r -= alpha Ap ; |r|^2 -> RA_ITER.   a=y (PY), b=p, c=r, d=Ap (P)
@@ -81,7 +76,7 @@ __global__ __launch_bounds__(NT) void k_cg_update(BlasArgs a) {
     store_as<PY, NC>((char*)a.a + col * a.col_bytes_a, V, s, y);
     store_as<P, NC>((char*)a.c + col * a.col_bytes_c, V, s, r);
   }
-  finish_reduction(a.red, col, tile, a.ntiles, valid ? part : 0.0);
+  write_partial(a.red, col, tile, a.ntiles, valid ? part : 0.0);
 }
 
 // p = r + beta p.   a=p, b=r
@@ -130,6 +125,22 @@ __global__ __launch_bounds__(NT) void k_convert(BlasArgs a) {
   if (valid) store_as<PO, NC>((char*)a.a + col * a.col_bytes_a, V, s, v);
 }
 
+__global__ __launch_bounds__(NT) void k_reduce_finish(Reduction red, int nblk, int gate) {
+  const int col = blockIdx.x;
+  if (gate_skip(gate, red.cols[col])) return;
+  __shared__ double lds[NT / 64];
+  const double* parts = red.partials + (size_t)col * nblk;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += NT) s += parts[i];
+  s = block_sum<NT>(s, lds);
+  if (threadIdx.x == 0) apply_action(red.action, red.cols[col], s);
+}
+
+hipError_t launch_reduce_finish(const Reduction& red, int ncol, int nblk, int gate, hipStream_t stream) {
+  hipLaunchKernelGGL(k_reduce_finish, dim3(ncol), dim3(NT), 0, stream, red, nblk, gate);
+  return hipGetLastError();
+}
+
 // After a reliable update: restart the iterated residual from the true one.
 __global__ void k_finalize_update(ColState* cols, int ncol) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -155,14 +166,20 @@ __global__ void k_finalize_update(ColState* cols, int ncol) {
 
 static inline dim3 blas_grid(const BlasArgs& a) { return dim3((unsigned)(a.ntiles * a.ncol)); }
 
-hipError_t launch_norm2(int p, int nc, BlasArgs a, hipStream_t stream) {
+static hipError_t norm2_main(int p, int nc, BlasArgs a, hipStream_t stream) {
   dim3 grid = blas_grid(a), block(NT);
   if (p == F64) return WL_DISPATCH_NC(k_norm2, F64);
   if (p == F32) return WL_DISPATCH_NC(k_norm2, F32);
   return WL_DISPATCH_NC(k_norm2, F16);
 }
 
-hipError_t launch_cg_update(int py, int p, int nc, BlasArgs a, hipStream_t stream) {
+hipError_t launch_norm2(int p, int nc, BlasArgs a, hipStream_t stream) {
+  hipError_t e = norm2_main(p, nc, a, stream);
+  if (e != hipSuccess) return e;
+  return launch_reduce_finish(a.red, a.ncol, a.ntiles, a.gate, stream);
+}
+
+static hipError_t cg_update_main(int py, int p, int nc, BlasArgs a, hipStream_t stream) {
   dim3 grid = blas_grid(a), block(NT);
   if (py == F64 && p == F64) return WL_DISPATCH_NC(k_cg_update, F64, F64);
   if (py == F32 && p == F32) return WL_DISPATCH_NC(k_cg_update, F32, F32);
@@ -170,6 +187,12 @@ hipError_t launch_cg_update(int py, int p, int nc, BlasArgs a, hipStream_t strea
   if (py == F64 && p == F32) return WL_DISPATCH_NC(k_cg_update, F64, F32);
   if (py == F64 && p == F16) return WL_DISPATCH_NC(k_cg_update, F64, F16);
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_cg_update(int py, int p, int nc, BlasArgs a, hipStream_t stream) {
+  hipError_t e = cg_update_main(py, p, nc, a, stream);
+  if (e != hipSuccess) return e;
+  return launch_reduce_finish(a.red, a.ncol, a.ntiles, a.gate, stream);
 }
 
 hipError_t launch_xpay(int p, int nc, BlasArgs a, hipStream_t stream) {

@@ -11,8 +11,14 @@ constexpr int NT = 256;  // threads per workgroup (4 wavefronts)
 struct Geom {
   int L[4];
   int stride[4];  // flat-index strides, stride[3] = 1 (last axis fastest)
-  long long V;
+  long long V;    // sites
+  long long Vp;   // plane stride of the internal layouts (V + padding)
 };
+
+// Planes are padded so that the planes of one site (and the same site in
+// different fields) do not all fall in one L2 set: a power-of-two plane
+// stride maps every plane of a site to the same set index.
+constexpr long long PLANE_PAD = 288;
 
 // Per-column solver scalars, FP64, resident in device memory.
 struct ColState {
@@ -48,7 +54,6 @@ enum RedAction {
 
 struct Reduction {
   double* partials;      // [ncol][nblk_per_col]
-  unsigned* counters;    // [ncol]
   ColState* cols;        // [ncol]
   int action;
 };
@@ -64,18 +69,20 @@ struct DslashArgs {
   size_t col_bytes;       // bytes of one spinor column (in, out and w share precision)
   int ncol;
   int cols_per_cfg;
-  int ntiles;
+  int ntiles;             // nchunk * nseg
+  int march;              // x0 sites per thread (divides L0)
+  int nseg;               // L0 / march
   double bc_re[4], bc_im[4];
   double m0;              // diagonal coefficient (used when m0_from_state == 0)
   double r;               // Wilson r
   int m0_from_state;      // take m0 per column from red.cols[col].m0
-  int skip_inactive;      // skip columns whose state is not active
-  int only_update;        // run only columns with need_update != 0 (reliable updates)
+  int gate;               // 0 all columns, 1 active columns, 2 columns awaiting a reliable update
   Reduction red;
 };
 
 struct BlasArgs {
   long long V;
+  long long Vp;
   int ncol;
   int ntiles;
   int gate;  // 0 none, 1 active columns, 2 columns awaiting a reliable update

@@ -63,7 +63,7 @@ struct LinkStore<F16, NC> {
 
 template <int P, int NC, class S>
 __global__ __launch_bounds__(NT) void k_pack_spinor(const S* __restrict__ src, void* __restrict__ dst, long long V,
-                                                    int ncol, size_t col_bytes) {
+                                                    long long Vp, int ncol, size_t col_bytes) {
   const size_t site = (size_t)blockIdx.x * NT + threadIdx.x;
   const int col = blockIdx.y;
   if (site >= (size_t)V) return;
@@ -72,18 +72,18 @@ __global__ __launch_bounds__(NT) void k_pack_spinor(const S* __restrict__ src, v
   cx<T> v[4 * NC];
 #pragma unroll
   for (int c = 0; c < 4 * NC; ++c) v[c] = {(T)s[2 * c], (T)s[2 * c + 1]};
-  SpinorIO<P, NC>::store(reinterpret_cast<char*>(dst) + col * col_bytes, V, site, v);
+  SpinorIO<P, NC>::store(reinterpret_cast<char*>(dst) + col * col_bytes, Vp, site, v);
 }
 
 template <int P, int NC, class S>
 __global__ __launch_bounds__(NT) void k_unpack_spinor(const void* __restrict__ src, S* __restrict__ dst, long long V,
-                                                      int ncol, size_t col_bytes) {
+                                                      long long Vp, int ncol, size_t col_bytes) {
   const size_t site = (size_t)blockIdx.x * NT + threadIdx.x;
   const int col = blockIdx.y;
   if (site >= (size_t)V) return;
   using T = typename PT<P>::acc;
   cx<T> v[4 * NC];
-  SpinorIO<P, NC>::load(reinterpret_cast<const char*>(src) + col * col_bytes, V, site, v);
+  SpinorIO<P, NC>::load(reinterpret_cast<const char*>(src) + col * col_bytes, Vp, site, v);
   S* d = dst + ((size_t)col * V + site) * (4 * NC) * 2;
 #pragma unroll
   for (int c = 0; c < 4 * NC; ++c) {
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(NT) void k_unpack_spinor(const void* __restrict__ s
 // layout: 0 = BMXYIJ, 1 = BXYMIJ
 template <int P, int NC, int RECON, class S>
 __global__ __launch_bounds__(NT) void k_pack_gauge(const S* __restrict__ src, void* __restrict__ dst, long long V,
-                                                   int layout, size_t cfg_bytes, size_t dir_bytes) {
+                                                   long long Vp, int layout, size_t cfg_bytes, size_t dir_bytes) {
   const size_t site = (size_t)blockIdx.x * NT + threadIdx.x;
   const int mu = blockIdx.y;
   const int cfg = blockIdx.z;
@@ -106,13 +106,13 @@ __global__ __launch_bounds__(NT) void k_pack_gauge(const S* __restrict__ src, vo
   for (int a = 0; a < NC; ++a)
     for (int b = 0; b < NC; ++b) u[a][b] = {(double)s[2 * (a * NC + b)], (double)s[2 * (a * NC + b) + 1]};
   char* d = reinterpret_cast<char*>(dst) + cfg * cfg_bytes + mu * dir_bytes;
-  LinkStore<P, NC>::template store<RECON>(d, V, site, u);
+  LinkStore<P, NC>::template store<RECON>(d, Vp, site, u);
 }
 
 // Re-encode an internal FP64 full gauge field into another precision/recon.
 template <int P, int NC, int RECON>
 __global__ __launch_bounds__(NT) void k_convert_gauge(const void* __restrict__ src, void* __restrict__ dst, long long V,
-                                                      size_t src_cfg_bytes, size_t src_dir_bytes, size_t cfg_bytes,
+                                                      long long Vp, size_t src_cfg_bytes, size_t src_dir_bytes, size_t cfg_bytes,
                                                       size_t dir_bytes) {
   const size_t site = (size_t)blockIdx.x * NT + threadIdx.x;
   const int mu = blockIdx.y;
@@ -123,89 +123,89 @@ __global__ __launch_bounds__(NT) void k_convert_gauge(const void* __restrict__ s
   cx<double> u[NC][NC];
   for (int a = 0; a < NC; ++a)
     for (int b = 0; b < NC; ++b) {
-      double2 t = s[(a * NC + b) * V + site];
+      double2 t = s[(a * NC + b) * Vp + site];
       u[a][b] = {t.x, t.y};
     }
   char* d = reinterpret_cast<char*>(dst) + cfg * cfg_bytes + mu * dir_bytes;
-  LinkStore<P, NC>::template store<RECON>(d, V, site, u);
+  LinkStore<P, NC>::template store<RECON>(d, Vp, site, u);
 }
 
 // ---------------------------------------------------------------------------
 // host dispatch.  dtype: 0 complex128, 1 complex64
 // ---------------------------------------------------------------------------
 template <int P, int NC>
-static hipError_t pack_spinor_t(const void* src, int dtype, void* dst, long long V, int ncol, size_t col_bytes,
+static hipError_t pack_spinor_t(const void* src, int dtype, void* dst, long long V, long long Vp, int ncol, size_t col_bytes,
                                 hipStream_t st) {
   dim3 grid((unsigned)((V + NT - 1) / NT), (unsigned)ncol), block(NT);
   if (dtype == 0)
-    hipLaunchKernelGGL((k_pack_spinor<P, NC, double>), grid, block, 0, st, (const double*)src, dst, V, ncol, col_bytes);
+    hipLaunchKernelGGL((k_pack_spinor<P, NC, double>), grid, block, 0, st, (const double*)src, dst, V, Vp, ncol, col_bytes);
   else
-    hipLaunchKernelGGL((k_pack_spinor<P, NC, float>), grid, block, 0, st, (const float*)src, dst, V, ncol, col_bytes);
+    hipLaunchKernelGGL((k_pack_spinor<P, NC, float>), grid, block, 0, st, (const float*)src, dst, V, Vp, ncol, col_bytes);
   return hipGetLastError();
 }
 
 template <int P, int NC>
-static hipError_t unpack_spinor_t(const void* src, void* dst, int dtype, long long V, int ncol, size_t col_bytes,
+static hipError_t unpack_spinor_t(const void* src, void* dst, int dtype, long long V, long long Vp, int ncol, size_t col_bytes,
                                   hipStream_t st) {
   dim3 grid((unsigned)((V + NT - 1) / NT), (unsigned)ncol), block(NT);
   if (dtype == 0)
-    hipLaunchKernelGGL((k_unpack_spinor<P, NC, double>), grid, block, 0, st, src, (double*)dst, V, ncol, col_bytes);
+    hipLaunchKernelGGL((k_unpack_spinor<P, NC, double>), grid, block, 0, st, src, (double*)dst, V, Vp, ncol, col_bytes);
   else
-    hipLaunchKernelGGL((k_unpack_spinor<P, NC, float>), grid, block, 0, st, src, (float*)dst, V, ncol, col_bytes);
+    hipLaunchKernelGGL((k_unpack_spinor<P, NC, float>), grid, block, 0, st, src, (float*)dst, V, Vp, ncol, col_bytes);
   return hipGetLastError();
 }
 
-hipError_t pack_spinor(int prec, int nc, const void* src, int dtype, void* dst, long long V, int ncol,
+hipError_t pack_spinor(int prec, int nc, const void* src, int dtype, void* dst, long long V, long long Vp, int ncol,
                        size_t col_bytes, hipStream_t st) {
   if (nc == 3) {
-    if (prec == F64) return pack_spinor_t<F64, 3>(src, dtype, dst, V, ncol, col_bytes, st);
-    if (prec == F32) return pack_spinor_t<F32, 3>(src, dtype, dst, V, ncol, col_bytes, st);
-    return pack_spinor_t<F16, 3>(src, dtype, dst, V, ncol, col_bytes, st);
+    if (prec == F64) return pack_spinor_t<F64, 3>(src, dtype, dst, V, Vp, ncol, col_bytes, st);
+    if (prec == F32) return pack_spinor_t<F32, 3>(src, dtype, dst, V, Vp, ncol, col_bytes, st);
+    return pack_spinor_t<F16, 3>(src, dtype, dst, V, Vp, ncol, col_bytes, st);
   }
-  if (prec == F64) return pack_spinor_t<F64, 1>(src, dtype, dst, V, ncol, col_bytes, st);
-  if (prec == F32) return pack_spinor_t<F32, 1>(src, dtype, dst, V, ncol, col_bytes, st);
-  return pack_spinor_t<F16, 1>(src, dtype, dst, V, ncol, col_bytes, st);
+  if (prec == F64) return pack_spinor_t<F64, 1>(src, dtype, dst, V, Vp, ncol, col_bytes, st);
+  if (prec == F32) return pack_spinor_t<F32, 1>(src, dtype, dst, V, Vp, ncol, col_bytes, st);
+  return pack_spinor_t<F16, 1>(src, dtype, dst, V, Vp, ncol, col_bytes, st);
 }
 
-hipError_t unpack_spinor(int prec, int nc, const void* src, void* dst, int dtype, long long V, int ncol,
+hipError_t unpack_spinor(int prec, int nc, const void* src, void* dst, int dtype, long long V, long long Vp, int ncol,
                          size_t col_bytes, hipStream_t st) {
   if (nc == 3) {
-    if (prec == F64) return unpack_spinor_t<F64, 3>(src, dst, dtype, V, ncol, col_bytes, st);
-    if (prec == F32) return unpack_spinor_t<F32, 3>(src, dst, dtype, V, ncol, col_bytes, st);
-    return unpack_spinor_t<F16, 3>(src, dst, dtype, V, ncol, col_bytes, st);
+    if (prec == F64) return unpack_spinor_t<F64, 3>(src, dst, dtype, V, Vp, ncol, col_bytes, st);
+    if (prec == F32) return unpack_spinor_t<F32, 3>(src, dst, dtype, V, Vp, ncol, col_bytes, st);
+    return unpack_spinor_t<F16, 3>(src, dst, dtype, V, Vp, ncol, col_bytes, st);
   }
-  if (prec == F64) return unpack_spinor_t<F64, 1>(src, dst, dtype, V, ncol, col_bytes, st);
-  if (prec == F32) return unpack_spinor_t<F32, 1>(src, dst, dtype, V, ncol, col_bytes, st);
-  return unpack_spinor_t<F16, 1>(src, dst, dtype, V, ncol, col_bytes, st);
+  if (prec == F64) return unpack_spinor_t<F64, 1>(src, dst, dtype, V, Vp, ncol, col_bytes, st);
+  if (prec == F32) return unpack_spinor_t<F32, 1>(src, dst, dtype, V, Vp, ncol, col_bytes, st);
+  return unpack_spinor_t<F16, 1>(src, dst, dtype, V, Vp, ncol, col_bytes, st);
 }
 
-hipError_t pack_gauge_f64(int nc, const void* src, int dtype, int layout, void* dst, long long V, int ncfg,
+hipError_t pack_gauge_f64(int nc, const void* src, int dtype, int layout, void* dst, long long V, long long Vp, int ncfg,
                           size_t cfg_bytes, size_t dir_bytes, hipStream_t st) {
   dim3 grid((unsigned)((V + NT - 1) / NT), 4, (unsigned)ncfg), block(NT);
   if (nc == 3) {
     if (dtype == 0)
-      hipLaunchKernelGGL((k_pack_gauge<F64, 3, 18, double>), grid, block, 0, st, (const double*)src, dst, V, layout,
+      hipLaunchKernelGGL((k_pack_gauge<F64, 3, 18, double>), grid, block, 0, st, (const double*)src, dst, V, Vp, layout,
                          cfg_bytes, dir_bytes);
     else
-      hipLaunchKernelGGL((k_pack_gauge<F64, 3, 18, float>), grid, block, 0, st, (const float*)src, dst, V, layout,
+      hipLaunchKernelGGL((k_pack_gauge<F64, 3, 18, float>), grid, block, 0, st, (const float*)src, dst, V, Vp, layout,
                          cfg_bytes, dir_bytes);
   } else {
     if (dtype == 0)
-      hipLaunchKernelGGL((k_pack_gauge<F64, 1, 18, double>), grid, block, 0, st, (const double*)src, dst, V, layout,
+      hipLaunchKernelGGL((k_pack_gauge<F64, 1, 18, double>), grid, block, 0, st, (const double*)src, dst, V, Vp, layout,
                          cfg_bytes, dir_bytes);
     else
-      hipLaunchKernelGGL((k_pack_gauge<F64, 1, 18, float>), grid, block, 0, st, (const float*)src, dst, V, layout,
+      hipLaunchKernelGGL((k_pack_gauge<F64, 1, 18, float>), grid, block, 0, st, (const float*)src, dst, V, Vp, layout,
                          cfg_bytes, dir_bytes);
   }
   return hipGetLastError();
 }
 
-hipError_t convert_gauge(int prec, int nc, int recon, const void* src, void* dst, long long V, int ncfg,
+hipError_t convert_gauge(int prec, int nc, int recon, const void* src, void* dst, long long V, long long Vp, int ncfg,
                          size_t src_cfg_bytes, size_t src_dir_bytes, size_t cfg_bytes, size_t dir_bytes,
                          hipStream_t st) {
   dim3 grid((unsigned)((V + NT - 1) / NT), 4, (unsigned)ncfg), block(NT);
 #define WL_CG(P, NCV, R)                                                                                        \
-  hipLaunchKernelGGL((k_convert_gauge<P, NCV, R>), grid, block, 0, st, src, dst, V, src_cfg_bytes, src_dir_bytes, \
+  hipLaunchKernelGGL((k_convert_gauge<P, NCV, R>), grid, block, 0, st, src, dst, V, Vp, src_cfg_bytes, src_dir_bytes, \
                      cfg_bytes, dir_bytes)
   if (nc == 3) {
     if (prec == F64 && recon == 12) WL_CG(F64, 3, 12);

@@ -1,10 +1,12 @@
 // Deterministic grid reductions for the CG scalars.
 //
 // Every workgroup writes one FP64 partial (wave64 shuffle tree + LDS across
-// the 4 wavefronts); the last workgroup to arrive (agent-scope atomic
-// counter) sums the partials in a fixed order and performs the per-column
-// scalar update.  The result is independent of workgroup scheduling, so a
-// column gives the same bits alone or batched with other columns.
+// the 4 wavefronts) with a plain store; a one-workgroup-per-column finish
+// kernel then sums the partials in a fixed order and performs the per-column
+// scalar update.  No atomics or fences (an agent-scope fence per workgroup
+// writes back the XCD's L2 and costs more than the stencil itself).  The
+// result is independent of workgroup scheduling, so a column gives the same
+// bits alone or batched with other columns.
 #pragma once
 #include "wl_device.h"
 #include "wl_kernels.h"
@@ -16,6 +18,12 @@
 #define WL_STATUS_BREAKDOWN 3
 
 namespace wl {
+
+__device__ __forceinline__ bool gate_skip(int gate, const ColState& s) {
+  if (gate == 1) return !s.active;
+  if (gate == 2) return !(s.need_update && !s.done);
+  return false;
+}
 
 __device__ __forceinline__ void col_finish(ColState& s, int status) {
   s.status = status;
@@ -81,29 +89,18 @@ __device__ __forceinline__ void apply_action(int action, ColState& s, double sum
   }
 }
 
-// Called by every thread of the workgroup; v is this thread's contribution.
-__device__ __forceinline__ void finish_reduction(const Reduction& red, int col, int blk, int nblk, double v) {
+// Phase 1, called by every thread of the workgroup: write this workgroup's
+// FP64 partial (plain store; the kernel boundary publishes it).
+__device__ __forceinline__ void write_partial(const Reduction& red, int col, int blk, int nblk, double v) {
   __shared__ double lds[NT / 64];
-  __shared__ int am_last;
   double bs = block_sum<NT>(v, lds);
-  double* parts = red.partials + (size_t)col * nblk;
-  if (threadIdx.x == 0) {
-    __hip_atomic_store(parts + blk, bs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    unsigned prev = atomicAdd(red.counters + col, 1u);
-    am_last = (prev == (unsigned)(nblk - 1));
-  }
-  __syncthreads();
-  if (!am_last) return;
-  __threadfence();
-  double s = 0.0;
-  for (int i = threadIdx.x; i < nblk; i += NT) s += __hip_atomic_load(parts + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  s = block_sum<NT>(s, lds);
-  if (threadIdx.x == 0) {
-    apply_action(red.action, red.cols[col], s);
-    red.counters[col] = 0;
-    __threadfence();
-  }
+  if (threadIdx.x == 0) red.partials[(size_t)col * nblk + blk] = bs;
 }
+
+// Phase 2 kernel (one workgroup per column): fixed-order sum of the partials
+// and the per-column scalar update.  gate: 0 all columns, 1 active columns,
+// 2 columns awaiting a reliable update -- the same gate as the phase-1 kernel,
+// evaluated on the same (unchanged) state.
+__global__ __launch_bounds__(NT) void k_reduce_finish(Reduction red, int nblk, int gate);
 
 }  // namespace wl

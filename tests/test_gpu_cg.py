@@ -139,3 +139,26 @@ def test_multi_rhs_same_alone_and_in_the_mix(prec):
         assert np.array_equal(xj[:, 0], X[:, j]), j
         assert th.last_solve_stats[0]["iters"] == stats[j]["iters"]
         assert true_resid(L, 3, masses[j], (1, 1, 1, -1), U, rhs[:, j], X[:, j]) <= 1e-10
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32", "f16"])
+@pytest.mark.parametrize("march", [2, 8])
+def test_cg_with_marching_stencil(prec, march):
+    """The fused-reduction stencils of the marching kernel drive CG to the same
+    FP64 true residual."""
+    print(f"[gpu] cg march={march} {prec}", flush=True)
+    from jaxqft_amd import _native as nat
+    from jaxqft_amd import fields
+    from jaxqft_amd.su3_wilson_nf2 import SU3WilsonNf2
+
+    L = (8, 4, 4, 8)
+    rng = fields.rng_for(1000)
+    U = fields.su3_hot_start(rng, 1, L, 0.5)
+    b = fields.random_fermion(rng, (1, *L, 4, 3))
+    th = SU3WilsonNf2(lattice_shape=L, mass=0.05, fermion_bc=(1, 1, 1, -1), dtype=np.complex128, cg_tol=1e-10,
+                      cg_maxiter=4000, solver_precision=prec)
+    for p in (nat.WL_F64, nat.WL_F32, nat.WL_F16):
+        th._context().set_march(p, march)
+    x = th.solve_normal(U, b)
+    assert th.last_solve_stats[0]["status"] == "converged"
+    assert true_resid(L, 3, 0.05, (1, 1, 1, -1), U, b, x) <= 1e-10

@@ -148,3 +148,62 @@ def test_properties_at_benchmark_sizes(L):
     n = th.apply_normal(G, psi)
     q = torch.vdot(psi.flatten(), n.flatten())
     assert abs(float(q.imag)) < 1e-10 * float(q.real) and float(q.real) > 0
+
+
+@pytest.mark.parametrize("march", [1, 2, 4, 8])
+@pytest.mark.parametrize("nc", [3, 1])
+def test_march_variants_match_oracle(march, nc):
+    """Every stencil launch shape (x0 sites per thread) gives the oracle's D and D^dag."""
+    print(f"[gpu] march={march} nc={nc}", flush=True)
+    import torch
+
+    from jaxqft_amd import _native as nat
+    from jaxqft_amd import fields
+    from oracle.wilson_oracle import WilsonOracle
+
+    L = (8, 4, 6, 4)
+    bc = (-1, np.exp(0.3j), 1, -1)
+    rng = fields.rng_for(800 + march)
+    th = theory_for(L, nc, 0.07, bc, batch_size=2)
+    th._context().set_march(nat.WL_F64, march)
+    assert th._context().get_march(nat.WL_F64) == march
+    U = fields.su3_hot_start(rng, 2, L, 0.6) if nc == 3 else fields.u1_hot_start(rng, 2, L, 0.6)
+    psi = fields.random_fermion(rng, th.fermion_shape())
+    ora = WilsonOracle(L, nc, 0.07, fermion_bc=bc)
+    assert rel(th.apply_D(U, psi), ora.apply_D(U, psi)) < TOL
+    assert rel(th.apply_Ddag(U, psi), ora.apply_Ddag(U, psi)) < TOL
+    assert rel(th.apply_g5D(U, psi), ora.apply_g5D(U, psi)) < TOL
+    assert rel(th.apply_offdiag(U, psi), ora.apply_offdiag(U, psi)) < TOL
+
+
+@pytest.mark.parametrize("prec,recon,tol", [("f64", 12, 1e-13), ("f32", 18, 2e-6), ("f32", 12, 2e-6),
+                                            ("f16", 18, 3e-3), ("f16", 12, 3e-3)])
+@pytest.mark.parametrize("march", [1, 4])
+def test_low_precision_stencils(prec, recon, tol, march):
+    """Resident-field stencils in every storage precision against the FP64
+    oracle: FP32 within 2e-6 and FP16 (per-site scaled) within 3e-3 relative;
+    FP64 with 12-real reconstruction within 1e-13 (links are exactly SU(3))."""
+    print(f"[gpu] stencil {prec} r{recon} march={march}", flush=True)
+    import torch
+
+    from jaxqft_amd import _native as nat
+    from jaxqft_amd import fields
+    from oracle.wilson_oracle import WilsonOracle
+
+    L = (8, 8, 4, 8)
+    bc = (1, 1, 1, -1)
+    rng = fields.rng_for(900)
+    U = fields.su3_hot_start(rng, 1, L, 0.5)
+    psi = fields.random_fermion(rng, (1, *L, 4, 3))
+    ctx = nat.Context(L, 3, 1, 0.05, 1.0, bc)
+    pc = nat.PREC[prec]
+    ctx.set_march(pc, march)
+    g = nat.Gauge(ctx, torch.from_numpy(U).cuda(), nat.WL_LAYOUT_BMXYIJ)
+    fi, fo = nat.Field(ctx, pc, 1), nat.Field(ctx, pc, 1)
+    fi.import_(torch.from_numpy(psi).cuda())
+    ora = WilsonOracle(L, 3, 0.05, fermion_bc=bc)
+    out = torch.empty((1, *L, 4, 3), dtype=torch.complex128, device="cuda")
+    for op, ref in ((nat.WL_OP_D, ora.apply_D(U, psi)), (nat.WL_OP_DDAG, ora.apply_Ddag(U, psi))):
+        nat.check(ctx._lib.wl_dslash(ctx.h, g.h, op, recon, fi.h, fo.h, 1, nat.c_void_p(nat.current_stream())))
+        fo.export(out)
+        assert rel(out.cpu().numpy(), ref) < tol, (op, rel(out.cpu().numpy(), ref))
