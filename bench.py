@@ -1,9 +1,15 @@
 """Benchmark: CG solves/s and Dslash HBM GB/s on a 32^4 SU(3) Wilson lattice,
-mixed-precision reliable-update CG vs plain FP64, true residual 1e-10.
+mixed-precision multi-RHS CG vs the plain FP64 single-RHS path, true residual
+1e-10 (recomputed in FP64).
 
-One step = one solve of D^dag D x = b on the resident 32^4 gauge field
-(eps = 0.6, m = 0.02, boundary phases (1,1,1,-1), complex Gaussian source),
-inputs already in HBM.  Prints one JSON line.
+Workload (one step): one batched solve of D^dag D x = b for k = 12 complex
+Gaussian right-hand sides on a resident 32^4 gauge field (eps = 0.6,
+m = 0.02, boundary phases (1,1,1,-1)), inner operator FP16 storage with
+12-real link reconstruction, FP64 reliable updates; inputs already in HBM.
+``value`` = right-hand sides solved per second.  Beside it: the plain FP64
+single-RHS solve rate, every Dslash storage variant (autotuned launch shape),
+the 24^3x48 multi-RHS case k in {1, 4, 12, 24}, and the CPU baseline.
+Prints one JSON line.
 """
 
 from __future__ import annotations
@@ -25,16 +31,16 @@ METRIC = "CG solves/s + Dslash HBM GB/s, 32^4 SU(3) Wilson; mixed vs FP64 at 1e-
 PEAK_HBM_GBS = 8000.0
 FLOP_PER_SITE = 1320
 
-# compulsory HBM bytes per site of one Dslash application (links read once,
-# input spinor read once, output written once)
-def dslash_bytes_per_site(prec: str, recon: int, nc: int = 3) -> int:
+
+def dslash_bytes_per_site(prec: str, recon: int, nrhs: int = 1, nc: int = 3) -> int:
+    """Compulsory HBM bytes per site of one application: links read once,
+    each right-hand side's input read once and output written once (FP16
+    adds the per-site FP32 scale of input and output)."""
     spin_reals = 24 if nc == 3 else 8
     link_reals = (12 if recon == 12 else 18) if nc == 3 else 2
-    if prec == "f64":
-        return (4 * link_reals + 2 * spin_reals) * 8
-    if prec == "f32":
-        return (4 * link_reals + 2 * spin_reals) * 4
-    return (4 * link_reals + 2 * spin_reals) * 2 + 2 * 4  # + per-site scale in and out
+    b = {"f64": 8, "f32": 4, "f16": 2}[prec]
+    spin = 2 * spin_reals * b + (8 if prec == "f16" else 0)
+    return 4 * link_reals * b + nrhs * spin
 
 
 def parse():
@@ -46,18 +52,69 @@ def parse():
     ap.add_argument("--eps", type=float, default=0.6)
     ap.add_argument("--mass", type=float, default=0.02)
     ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--nrhs", type=int, default=12)
     ap.add_argument("--inner", type=str, default="f16", help="inner precision of the mixed solve")
     ap.add_argument("--recon", type=int, default=12)
-    ap.add_argument("--delta", type=float, default=0.1)
+    ap.add_argument("--delta", type=float, default=0.01, help="reliable-update trigger of the headline solve")
     ap.add_argument("--poll", type=int, default=16)
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--dslash-reps", type=int, default=50)
-    ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--no-config4", action="store_true", help="skip the 24^3x48 multi-RHS case")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving scenario")
+    ap.add_argument("--serving-requests", type=int, default=60)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
                     help="run only the Dslash variants and a calibration copy (for rocprofv3 counter passes)")
     return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+class Bench:
+    def __init__(self, args):
+        import torch
+
+        from jaxqft_amd import _native as nat
+
+        self.torch = torch
+        self.nat = nat
+        self.lib = nat.load()
+        self.args = args
+        self.stream = torch.cuda.Stream()  # non-default stream (graph capture needs one)
+        self.h = c_void_p(self.stream.cuda_stream)
+
+    def solve(self, ctx, gauge, b, x, nrhs, prec, recon, masses=None, delta=None):
+        nat = self.nat
+        a = self.args
+        sp, keep = nat.solve_params(a.tol, 20000, nat.WL_STOP_TRUE_RESIDUAL, nat.PREC[prec], recon,
+                                    a.delta if delta is None else delta, a.poll, bool(a.graph), masses)
+        st = (nat.WlSolveStats * b.ncol)()
+        nat.check(self.lib.wl_cg_normal_field(ctx.h, gauge.h, b.h, x.h, 0, nrhs, byref(sp), st, self.h))
+        del keep
+        return nat.stats_list(st, b.ncol)
+
+    def timed_solves(self, n, *sargs):
+        torch = self.torch
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            st = self.solve(*sargs)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / n, st
+
+
+def summarize(stats):
+    return {
+        "iters_max": max(s["iters"] for s in stats),
+        "iters_mean": round(float(np.mean([s["iters"] for s in stats])), 1),
+        "status": sorted(set(s["status"] for s in stats)),
+        "true_resid_max": max(s["true_resid"] for s in stats),
+        "reliable_updates_max": max(s["reliable_updates"] for s in stats),
+    }
 
 
 def main():
@@ -70,46 +127,37 @@ def main():
 
     from jaxqft_amd import _native as nat
     from jaxqft_amd import fields
+    from jaxqft_amd.autotune import time_dslash, tune_march
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+    t_start = time.perf_counter()
+    B = Bench(args)
+    stream = B.stream
     L = tuple(int(v) for v in args.lattice.split(","))
     V = int(np.prod(L))
+    k = args.nrhs
     bc = (1, 1, 1, -1)
-    stream = torch.cuda.Stream()  # non-default stream (graph capture needs one)
-    sp_handle = stream.cuda_stream
 
     t0 = time.perf_counter()
     rng = fields.rng_for(fields.SEED_BASE + 2)
     U = fields.su3_hot_start(rng, 1, L, args.eps)
-    b_host = fields.random_fermion(rng, (1, *L, 4, 3))
-    t_gen = time.perf_counter() - t0
-    print(f"[bench] generated fields {L} in {t_gen:.1f}s", file=sys.stderr, flush=True)
+    b_host = fields.random_fermion(rng, (k, *L, 4, 3))
+    log(f"generated fields {L} in {time.perf_counter() - t0:.1f}s")
 
-    lib = nat.load()
     with torch.cuda.stream(stream):
         ctx = nat.Context(L, 3, 1, args.mass, 1.0, bc)
-        Ut = torch.from_numpy(U).cuda()
-        gauge = nat.Gauge(ctx, Ut, nat.WL_LAYOUT_BMXYIJ)
-        del Ut
-        bt = torch.from_numpy(b_host).cuda()
-        b = nat.Field(ctx, nat.WL_F64, 1)
-        b.import_(bt)
-        x = nat.Field(ctx, nat.WL_F64, 1)
-        x64 = nat.Field(ctx, nat.WL_F64, 1)
+        gauge = nat.Gauge(ctx, torch.from_numpy(U).cuda(), nat.WL_LAYOUT_BMXYIJ)
+        bk_t = torch.from_numpy(b_host).cuda()
+        b1_t = bk_t[:1].contiguous()
         torch.cuda.synchronize()
 
-        # ---------------------------------------------------------- Dslash
-        # autotune the march length per (precision, recon), then time the best
-        from jaxqft_amd.autotune import time_dslash, tune_march
-
+        # ------------------------------------------------ Dslash (one RHS)
         dslash = {}
-        variants = [("f64", 18), ("f64", 12), ("f32", 18), ("f32", 12), ("f16", 18), ("f16", 12)]
-        for prec, recon in variants:
+        for prec, recon in [("f64", 18), ("f64", 12), ("f32", 18), ("f32", 12), ("f16", 18), ("f16", 12)]:
             pc = nat.PREC[prec]
-            fi = nat.Field(ctx, pc, 1)
-            fo = nat.Field(ctx, pc, 1)
-            fi.import_(bt)
+            fi, fo = nat.Field(ctx, pc, 1), nat.Field(ctx, pc, 1)
+            fi.import_(b1_t)
             tune = tune_march(ctx, gauge, pc, recon, fi, fo, reps=max(3, args.dslash_reps // 5), stream=stream)
             us = time_dslash(ctx, gauge, pc, recon, fi, fo, 1, args.dslash_reps, stream)
             bps = dslash_bytes_per_site(prec, recon)
@@ -117,14 +165,12 @@ def main():
             dslash[f"{prec}_r{recon}"] = {
                 "us": round(us, 2), "GB/s": round(gbs, 1), "frac_hbm_peak": round(gbs / PEAK_HBM_GBS, 4),
                 "GFLOP/s": round(FLOP_PER_SITE * V / (us * 1e-6) / 1e9, 1), "bytes_per_site": bps,
-                "march": tune["best"], "tune_us": {str(k): round(v, 2) for k, v in tune["us"].items()},
+                "march": tune["best"], "xcd_remap": tune["xcd_remap"], "tune_us": {m: round(t, 2) for m, t in tune["us"].items()},
             }
-            print(f"[bench] dslash {prec} r{recon}: {us:.1f} us  {gbs:.0f} GB/s  march {tune}", file=sys.stderr,
-                  flush=True)
+            log(f"dslash {prec} r{recon}: {us:.1f} us {gbs:.0f} GB/s march {tune['best']}")
             del fi, fo
 
         if args.profile_only:
-            # calibration: a plain device copy of known size (256 MiB read + 256 MiB written)
             src = torch.empty(32 << 20, dtype=torch.float64, device="cuda")
             dst = torch.empty_like(src)
             for _ in range(5):
@@ -133,81 +179,223 @@ def main():
             print(json.dumps({"profile_only": True, "dslash": dslash, "copy_bytes_each_way": src.numel() * 8}))
             return
 
-        # ---------------------------------------------------------- solves
-        def solve(field_x, prec, recon, graph):
-            sp, keep = nat.solve_params(args.tol, 20000, nat.WL_STOP_TRUE_RESIDUAL, nat.PREC[prec], recon, args.delta,
-                                        args.poll, bool(graph))
-            st = (nat.WlSolveStats * 1)()
-            nat.check(lib.wl_cg_normal_field(ctx.h, gauge.h, b.h, field_x.h, 0, 1, byref(sp), st, c_void_p(sp_handle)))
-            return nat.stats_list(st, 1)[0]
+        # ---------------------------------------- Dslash (k RHS, shared links)
+        dslash_k = {}
+        for prec, recon in [("f32", 12), ("f16", 12), ("f64", 18)]:
+            pc = nat.PREC[prec]
+            fi, fo = nat.Field(ctx, pc, k), nat.Field(ctx, pc, k)
+            fi.import_(bk_t)
+            gauge.prepare(pc, recon)
+            best = None
+            tried = {}
+            for kk in (1, 2, 4):
+                ctx.set_mrhs(pc, kk)
+                us = time_dslash(ctx, gauge, pc, recon, fi, fo, k, max(5, args.dslash_reps // 5), stream)
+                tried[str(kk)] = round(us, 2)
+                if best is None or us < best[1]:
+                    best = (kk, us)
+            ctx.set_mrhs(pc, best[0])
+            bps = dslash_bytes_per_site(prec, recon, k)
+            gbs = bps * V / (best[1] * 1e-6) / 1e9
+            dslash_k[f"{prec}_r{recon}"] = {
+                "nrhs": k, "rhs_per_thread": best[0], "us": round(best[1], 2), "us_per_rhs": round(best[1] / k, 2),
+                "GB/s": round(gbs, 1), "frac_hbm_peak": round(gbs / PEAK_HBM_GBS, 4), "bytes_per_site": bps,
+                "tune_us": tried,
+            }
+            log(f"dslash {prec} r{recon} k={k}: {best[1]:.1f} us ({best[1]/k:.1f}/rhs) {gbs:.0f} GB/s {tried}")
+            del fi, fo
 
-        # plain FP64 reference solve
-        st64 = solve(x64, "f64", 18, args.graph)  # warm
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.fp64_steps):
-            st64 = solve(x64, "f64", 18, args.graph)
-        torch.cuda.synchronize()
-        fp64_s = (time.perf_counter() - t0) / args.fp64_steps
-        print(f"[bench] fp64 solve {fp64_s:.3f}s {st64}", file=sys.stderr, flush=True)
+        # ------------------------------------------------------------ solves
+        b1 = nat.Field(ctx, nat.WL_F64, 1)
+        b1.import_(b1_t)
+        x1 = nat.Field(ctx, nat.WL_F64, 1)
+        B.solve(ctx, gauge, b1, x1, 1, "f64", 18)
+        fp64_s, st64 = B.timed_solves(args.fp64_steps, ctx, gauge, b1, x1, 1, "f64", 18)
+        log(f"fp64 single-RHS solve {fp64_s*1e3:.1f} ms {summarize(st64)}")
+        x1m = nat.Field(ctx, nat.WL_F64, 1)
+        B.solve(ctx, gauge, b1, x1m, 1, args.inner, args.recon)
+        mix1_s, stm1 = B.timed_solves(args.fp64_steps, ctx, gauge, b1, x1m, 1, args.inner, args.recon)
+        log(f"mixed single-RHS solve {mix1_s*1e3:.1f} ms {summarize(stm1)}")
+        xref = torch.empty_like(b1_t)
+        x1.export(xref)
 
+        # config 3: FP32 / FP16 inner, reconstruction on/off, delta = 0.1
+        config3 = {}
+        xc = nat.Field(ctx, nat.WL_F64, 1)
+        xct = torch.empty_like(b1_t)
+        for prec, recon in [("f32", 18), ("f32", 12), ("f16", 18), ("f16", 12)]:
+            B.solve(ctx, gauge, b1, xc, 1, prec, recon, delta=0.1)
+            t, st = B.timed_solves(1, ctx, gauge, b1, xc, 1, prec, recon, None, 0.1)
+            xc.export(xct)
+            torch.cuda.synchronize()
+            err = float(torch.linalg.norm(xct - xref) / torch.linalg.norm(xref))
+            config3[f"{prec}_r{recon}"] = {**summarize(st), "s_per_solve": round(t, 4), "rel_error_vs_fp64": err}
+            log(f"config3 {prec} r{recon} delta 0.1: {t*1e3:.1f} ms {summarize(st)} err {err:.2e}")
+        del xc
+
+        bk = nat.Field(ctx, nat.WL_F64, k)
+        bk.import_(bk_t)
+        xk = nat.Field(ctx, nat.WL_F64, k)
         for _ in range(args.warmup):
-            stm = solve(x, args.inner, args.recon, args.graph)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            stm = solve(x, args.inner, args.recon, args.graph)
-        torch.cuda.synchronize()
-        elapsed = time.perf_counter() - t0
-        ms_per_step = elapsed / args.steps * 1e3
-        print(f"[bench] mixed solve {ms_per_step:.1f} ms {stm}", file=sys.stderr, flush=True)
+            B.solve(ctx, gauge, bk, xk, k, args.inner, args.recon)
+        mixk_s, stmk = B.timed_solves(args.steps, ctx, gauge, bk, xk, k, args.inner, args.recon)
+        log(f"mixed {k}-RHS solve {mixk_s*1e3:.1f} ms {summarize(stmk)}")
 
-        xa = torch.empty_like(bt)
-        xb = torch.empty_like(bt)
-        x.export(xa)
-        x64.export(xb)
+        # error of the mixed solutions against the plain FP64 solve
+        xa = torch.empty_like(b1_t)
+        xb = torch.empty_like(b1_t)
+        xkt = torch.empty_like(bk_t)
+        x1m.export(xa)
+        x1.export(xb)
+        xk.export(xkt)
         torch.cuda.synchronize()
-        err_vs_fp64 = float(torch.linalg.norm(xa - xb) / torch.linalg.norm(xb))
+        err1 = float(torch.linalg.norm(xa - xb) / torch.linalg.norm(xb))
+        errk = float(torch.linalg.norm(xkt[:1] - xb) / torch.linalg.norm(xb))
 
-    inner_key = f"{args.inner}_r{args.recon}"
-    dom = dslash[inner_key]
+        # inner stencil of the headline solve, timed on its own (HIP events)
+        pc = nat.PREC[args.inner]
+        fi, fo = nat.Field(ctx, pc, k), nat.Field(ctx, pc, k)
+        fi.import_(bk_t)
+        dom_us = time_dslash(ctx, gauge, pc, args.recon, fi, fo, k, args.dslash_reps, stream)
+        dom_bytes = dslash_bytes_per_site(args.inner, args.recon, k) * V
+        dom_gbs = dom_bytes / (dom_us * 1e-6) / 1e9
+        rhs_per_thread = ctx.get_mrhs(pc, k)
+        del fi, fo, bk, xk, b1, x1, x1m
+        torch.cuda.synchronize()
+
     out = {
         "metric": METRIC,
-        "value": round(1e3 / ms_per_step, 4),
+        "value": round(k / mixk_s, 4),
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": round(mixk_s * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": f"{args.inner}-inner/f64-reliable",
-        "data": "synthetic (seeded PCG64 SU(3) links eps=%g, complex Gaussian source)" % args.eps,
-        "config": {"workload": "SU(3) Wilson D^dag D CG, %s, m=%g, bc=(1,1,1,-1), true residual %g" % (
-            "x".join(map(str, L)), args.mass, args.tol), "lattice": list(L), "nrhs": 1,
-                   "inner": args.inner, "recon": args.recon, "delta": args.delta, "graph": bool(args.graph)},
+        "dtype": f"{args.inner}-storage/f32-arith inner, f64 reliable updates",
+        "data": "synthetic (seeded PCG64 SU(3) links eps=%g, complex Gaussian sources)" % args.eps,
+        "config": {
+            "workload": "SU(3) Wilson D^dag D CG, %s, m=%g, bc=(1,1,1,-1), %d RHS per solve, FP64 true residual %g"
+                        % ("x".join(map(str, L)), args.mass, k, args.tol),
+            "lattice": list(L), "nrhs": k, "inner": args.inner, "recon": args.recon, "delta": args.delta,
+            "graph": bool(args.graph),
+        },
         "roofline": {
             "bound": "hbm",
-            "kernel": f"k_dslash {inner_key}",
-            "achieved": dom["GB/s"],
+            "kernel": f"k_dslash_mrhs {args.inner} r{args.recon}, {k} RHS, {rhs_per_thread} per thread "
+                      "(links counted once)",
+            "achieved": round(dom_gbs, 1),
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
-            "frac": dom["frac_hbm_peak"],
+            "frac": round(dom_gbs / PEAK_HBM_GBS, 4),
             "traffic": None,
+            "kernel_us": round(dom_us, 2),
+            "algorithmic_bytes": dom_bytes,
         },
         "solve": {
-            "mixed": {**stm, "s_per_solve": round(ms_per_step / 1e3, 4)},
-            "fp64": {**st64, "s_per_solve": round(fp64_s, 4), "solves_per_s": round(1.0 / fp64_s, 4)},
-            "speedup_mixed_vs_fp64": round(fp64_s / (ms_per_step / 1e3), 3),
-            "rel_error_mixed_vs_fp64": err_vs_fp64,
+            "fp64_single_rhs": {**summarize(st64), "s_per_solve": round(fp64_s, 4), "solves_per_s": round(1 / fp64_s, 3)},
+            "mixed_single_rhs": {**summarize(stm1), "s_per_solve": round(mix1_s, 4),
+                                 "solves_per_s": round(1 / mix1_s, 3), "rel_error_vs_fp64": err1},
+            "mixed_multi_rhs": {**summarize(stmk), "nrhs": k, "s_per_batch": round(mixk_s, 4),
+                                "solves_per_s": round(k / mixk_s, 3), "rel_error_vs_fp64_col0": errk},
+            "speedup_mixed_multi_vs_fp64_single": round((k / mixk_s) / (1 / fp64_s), 3),
+            "speedup_mixed_single_vs_fp64_single": round(fp64_s / mix1_s, 3),
         },
-        "dslash": dslash,
+        "config3_single_rhs_delta0.1": config3,
+        "dslash_single_rhs": dslash,
+        "dslash_multi_rhs": dslash_k,
     }
-
+    if not args.no_config4:
+        out["multi_rhs_24x24x24x48"] = config4(B, args)
+    if not args.no_serving:
+        out["serving"] = serving(B, args)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(U, b_host, L, args, st64["iters"])
+        out["cpu_baseline"] = cpu_baseline(U, b_host[:1], L, args, st64[0]["iters"])
+    out["bench_wall_s"] = round(time.perf_counter() - t_start, 1)
     print(json.dumps(out))
+
+
+def config4(B, args):
+    """24^3x48, eps=0.6, m=0.03: k in {1, 4, 12, 24} right-hand sides (k=12:
+    D^dag applied to the 12 unit point sources at a random site, as the
+    propagator loop of measurements.py:333-345 does; otherwise complex
+    Gaussian), FP32 inner stencil, FP64 reliable updates."""
+    torch, nat = B.torch, B.nat
+    from jaxqft_amd import fields
+
+    L = (24, 24, 24, 48)
+    rng = fields.rng_for(fields.SEED_BASE + 3)
+    U = fields.su3_hot_start(rng, 1, L, 0.6)
+    res = {}
+    with torch.cuda.stream(B.stream):
+        ctx = nat.Context(L, 3, 1, 0.03, 1.0, (1, 1, 1, -1))
+        gauge = nat.Gauge(ctx, torch.from_numpy(U).cuda(), nat.WL_LAYOUT_BMXYIJ)
+        for k in (1, 4, 12, 24):
+            if k == 12:
+                site = tuple(int(v) for v in rng.integers(0, L))
+                src = torch.from_numpy(fields.point_sources(L, site)).cuda()
+                bt = torch.empty_like(src)
+                nat.check(B.lib.wl_apply(ctx.h, gauge.h, nat.WL_OP_DDAG, c_void_p(src.data_ptr()),
+                                         c_void_p(bt.data_ptr()), 12, nat.WL_C128, B.h))
+                kind = "D^dag point sources"
+            else:
+                bt = torch.from_numpy(fields.random_fermion(rng, (k, *L, 4, 3))).cuda()
+                kind = "Gaussian"
+            b = nat.Field(ctx, nat.WL_F64, k)
+            b.import_(bt)
+            x = nat.Field(ctx, nat.WL_F64, k)
+            B.solve(ctx, gauge, b, x, k, "f32", 12)
+            t, st = B.timed_solves(1, ctx, gauge, b, x, k, "f32", 12)
+            res[str(k)] = {**summarize(st), "sources": kind, "s_per_batch": round(t, 4),
+                           "solves_per_s": round(k / t, 3)}
+            log(f"24^3x48 k={k}: {t*1e3:.1f} ms, {k/t:.2f} solves/s {summarize(st)}")
+            del b, x, bt
+    return res
+
+
+def serving(B, args):
+    """Poisson requests (mean 200/s) over {8^4, 16^4, 24^3x48, 32^4}, 4
+    device-resident fields per size (eps = 0.5), m ~ U[0.02, 0.2], k ~ U{1..12}
+    right-hand sides; continuous batching with FP16 inner / FP64 reliable
+    updates against an unbatched FP64 queue on the same requests."""
+    torch = B.torch
+    from jaxqft_amd import fields
+    from jaxqft_amd.serving import Request, Server, fp64_queue, poisson_requests
+
+    lattices = [(8, 8, 8, 8), (16, 16, 16, 16), (24, 24, 24, 48), (32, 32, 32, 32)]
+    t0 = time.perf_counter()
+    U = {}
+    for i, L in enumerate(lattices):
+        for f in range(4):
+            U[(L, f)] = fields.su3_hot_start_device(fields.SEED_BASE + 100 + 10 * i + f, 1, L, 0.5)
+    torch.cuda.synchronize()
+    log(f"serving: generated {len(U)} gauge fields in {time.perf_counter() - t0:.1f}s")
+    free, _ = torch.cuda.mem_get_info()
+    budget = 0.6 * free
+    reqs = poisson_requests(200.0, args.serving_requests, lattices, 4, seed=fields.SEED_BASE + 5)
+    srv = Server(U, capacity=12, inner="f16", recon=12, tol=args.tol, delta=args.delta, poll_every=8,
+                 hbm_budget_bytes=budget)
+    del U
+    # warm the lanes (graph capture, gauge encodings) on a short burst
+    warm = poisson_requests(1000.0, 8, lattices, 4, seed=1)
+    srv.run(warm, realtime=False)
+    res = srv.run(reqs, realtime=True)
+    log(f"serving continuous batching: {res}")
+    base = [Request(rid=r.rid, key=r.key, mass=r.mass, rhs=r.rhs, arrival=r.arrival) for r in reqs]
+    res64 = fp64_queue(base, srv.backend.gauges, srv.backend.ctxs, tol=args.tol, realtime=True)
+    log(f"serving unbatched fp64 queue: {res64}")
+    diff = max(float(torch.linalg.norm(a.x - b.x) / torch.linalg.norm(b.x)) for a, b in zip(reqs, base))
+    return {
+        "workload": f"{len(reqs)} Poisson requests at 200/s over 8^4,16^4,24^3x48,32^4 (4 fields each, eps=0.5), "
+                    "m~U[0.02,0.2], k~U{1..12}",
+        "hbm_budget_GB": round(budget / 1e9, 1),
+        "continuous_batching": res,
+        "fp64_unbatched_queue": res64,
+        "throughput_ratio": round(res["rhs_per_s"] / res64["rhs_per_s"], 3),
+        "max_rel_diff_vs_fp64_queue": diff,
+    }
 
 
 def cpu_baseline(U, b, L, args, gpu_iters):
@@ -223,16 +411,16 @@ def cpu_baseline(U, b, L, args, gpu_iters):
     one = time.perf_counter() - t0
     n = max(1, int(args.cpu_seconds / max(one, 1e-3)))
     t0 = time.perf_counter()
-    _, k = oracle_c.cg(Ub, bb, L, 3, args.mass, (1, 1, 1, -1), 0.0, n)
-    s_per_iter = (time.perf_counter() - t0) / k
+    _, it = oracle_c.cg(Ub, bb, L, 3, args.mass, (1, 1, 1, -1), 0.0, n)
+    s_per_iter = (time.perf_counter() - t0) / it
     return {
         "value": round(1.0 / (s_per_iter * max(1, gpu_iters)), 6),
         "unit": "solves/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{k} FP64 CG iterations on D^dag D at {'x'.join(map(str, L))} with the C/OpenMP oracle "
+        "sample": f"{it} FP64 CG iterations on D^dag D at {'x'.join(map(str, L))} with the C/OpenMP oracle "
                   f"({threads} threads): {s_per_iter*1e3:.1f} ms/iteration, extrapolated to the GPU FP64 "
-                  f"iteration count {gpu_iters}",
+                  f"single-RHS iteration count {gpu_iters}",
     }
 
 

@@ -130,6 +130,17 @@ int wl_ctx_set_mass(wl_ctx* ctx, double mass);
  * value in effect. */
 int wl_ctx_set_march(wl_ctx* ctx, int prec, int march);
 int wl_ctx_get_march(wl_ctx* ctx, int prec);
+/* Tuning knob (no counterpart in the reference): right-hand sides handled per
+ * GPU thread by the multi-RHS stencil (each link load is shared by them):
+ * 1, 2 or 4, or 0 = automatic.  It applies when it divides the number of
+ * right-hand sides per configuration; wl_ctx_get_mrhs returns the value used
+ * for nrhs right-hand sides. */
+int wl_ctx_set_mrhs(wl_ctx* ctx, int prec, int k);
+/* Tuning knob (no counterpart in the reference): 1 (default) maps the stencil
+ * workgroups of precision prec so that each XCD walks one contiguous range of
+ * lattice tiles; 0 keeps the hardware dispatch order. */
+int wl_ctx_set_xcd_remap(wl_ctx* ctx, int prec, int on);
+int wl_ctx_get_mrhs(wl_ctx* ctx, int prec, int nrhs);
 
 /* where: 0 = device pointer, 1 = host pointer */
 int wl_gauge_upload(wl_ctx* ctx, const void* U, int layout, int dtype, int where, wl_gauge** out);
@@ -158,6 +169,26 @@ int wl_cg_normal(wl_ctx* ctx, wl_gauge* g, const void* b, void* x, const void* x
 /* Same on resident FP64 fields; x is used as the initial guess when use_x0. */
 int wl_cg_normal_field(wl_ctx* ctx, wl_gauge* g, const wl_field* b, wl_field* x, int use_x0, int nrhs,
                        const wl_solve_params* sp, wl_solve_stats* stats, void* stream);
+
+/* Continuous-batching solve engine (no counterpart in the reference; it serves
+ * the point-source propagator loop of core/measurements.py:277-371 and request
+ * streams).  A batch holds `capacity` columns on one resident configuration
+ * (context batch must be 1).  wl_batch_admit packs the right-hand side b
+ * (device, reference layout, one spinor) into a free column with its own mass;
+ * wl_batch_step queues poll_every CG iterations for every occupied column
+ * (asynchronous, one HIP graph when use_graph); wl_batch_query waits for the
+ * stream and reports, per column, state (0 free, 1 running, 2 finished) and
+ * stats; wl_batch_retire writes a column's solution x (device, reference
+ * layout) and frees the column.  Columns are independent: a right-hand side
+ * gives the same solution alone or batched. */
+typedef struct wl_batch wl_batch;
+size_t wl_batch_bytes(wl_ctx* ctx, int capacity, int inner_prec);
+int wl_batch_create(wl_ctx* ctx, wl_gauge* g, int capacity, const wl_solve_params* sp, void* stream, wl_batch** out);
+void wl_batch_destroy(wl_batch* b);
+int wl_batch_admit(wl_batch* b, int slot, const void* rhs, int dtype, double mass);
+int wl_batch_step(wl_batch* b);
+int wl_batch_query(wl_batch* b, wl_solve_stats* stats, int* state);
+int wl_batch_retire(wl_batch* b, int slot, void* x, int dtype);
 
 #ifdef __cplusplus
 }

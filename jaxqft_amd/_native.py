@@ -96,6 +96,9 @@ def load() -> ctypes.CDLL:
         "wl_ctx_set_mass": (c_int, [vp, c_double]),
         "wl_ctx_set_march": (c_int, [vp, c_int, c_int]),
         "wl_ctx_get_march": (c_int, [vp, c_int]),
+        "wl_ctx_set_mrhs": (c_int, [vp, c_int, c_int]),
+        "wl_ctx_set_xcd_remap": (c_int, [vp, c_int, c_int]),
+        "wl_ctx_get_mrhs": (c_int, [vp, c_int, c_int]),
         "wl_gauge_upload": (c_int, [vp, vp, c_int, c_int, c_int, POINTER(vp)]),
         "wl_gauge_release": (None, [vp]),
         "wl_gauge_prepare": (c_int, [vp, c_int, c_int, vp]),
@@ -109,6 +112,15 @@ def load() -> ctypes.CDLL:
         "wl_cg_normal": (c_int, [vp, vp, vp, vp, vp, c_int, c_int, POINTER(WlSolveParams), POINTER(WlSolveStats), vp]),
         "wl_cg_normal_field": (c_int, [vp, vp, vp, vp, c_int, c_int, POINTER(WlSolveParams), POINTER(WlSolveStats), vp]),
     }
+    sig.update({
+        "wl_batch_bytes": (c_size_t, [vp, c_int, c_int]),
+        "wl_batch_create": (c_int, [vp, vp, c_int, POINTER(WlSolveParams), vp, POINTER(vp)]),
+        "wl_batch_destroy": (None, [vp]),
+        "wl_batch_admit": (c_int, [vp, c_int, vp, c_int, c_double]),
+        "wl_batch_step": (c_int, [vp]),
+        "wl_batch_query": (c_int, [vp, POINTER(WlSolveStats), POINTER(c_int)]),
+        "wl_batch_retire": (c_int, [vp, c_int, vp, c_int]),
+    })
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
         fn.restype = res
@@ -161,6 +173,15 @@ class Context:
 
     def get_march(self, prec: int) -> int:
         return int(self._lib.wl_ctx_get_march(self.h, int(prec)))
+
+    def set_mrhs(self, prec: int, k: int) -> None:
+        check(self._lib.wl_ctx_set_mrhs(self.h, int(prec), int(k)))
+
+    def get_mrhs(self, prec: int, nrhs: int) -> int:
+        return int(self._lib.wl_ctx_get_mrhs(self.h, int(prec), int(nrhs)))
+
+    def set_xcd_remap(self, prec: int, on: bool) -> None:
+        check(self._lib.wl_ctx_set_xcd_remap(self.h, int(prec), 1 if on else 0))
 
     def set_mass(self, mass: float) -> None:
         check(self._lib.wl_ctx_set_mass(self.h, float(mass)))

@@ -2,6 +2,7 @@
 // fields, the operator entry point and the normal-equation CG engine.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -23,6 +24,7 @@ hipError_t launch_xpay(int p, int nc, BlasArgs a, hipStream_t stream);
 hipError_t launch_accumulate(int px, int py, int nc, BlasArgs a, hipStream_t stream);
 hipError_t launch_convert(int po, int pi, int nc, BlasArgs a, hipStream_t stream);
 hipError_t launch_finalize_update(ColState* cols, int ncol, hipStream_t stream);
+hipError_t launch_admit_finish(ColState* cols, int col, double tol2, hipStream_t stream);
 hipError_t pack_spinor(int prec, int nc, const void* src, int dtype, void* dst, long long V, long long Vp, int ncol,
                        size_t col_bytes, hipStream_t st);
 hipError_t unpack_spinor(int prec, int nc, const void* src, void* dst, int dtype, long long V, long long Vp, int ncol,
@@ -104,6 +106,8 @@ struct wl_ctx {
   ColState* host_cols = nullptr;  // pinned mirror
   size_t host_cols_n = 0;
   int march[3] = {0, 0, 0};       // x0 sites per thread per precision (0 = automatic)
+  int mrhs[3] = {0, 0, 0};        // right-hand sides per thread per precision (0 = automatic)
+  int xcd_remap[3] = {1, 1, 1};   // XCD-contiguous workgroup order per precision
 };
 
 struct wl_gauge {
@@ -143,6 +147,19 @@ static int march_for(const wl_ctx* c, int prec) {
     if (L0 % cand == 0 && (long long)nchunk_of(c) * (L0 / cand) >= 512) return cand;
   }
   return 1;
+}
+
+// right-hand sides per thread: the largest of (4, 2) dividing the number of
+// right-hand sides per configuration (FP64: 2), unless set explicitly.
+static int mrhs_for(const wl_ctx* c, int prec, int nrhs) {
+  if (c->geom.nc != 3) return 1;
+  int k = c->mrhs[prec];
+  if (k == 0) {
+    for (int cand : {4, 2})
+      if ((prec != F64 || cand == 2) && nrhs % cand == 0) return cand;
+    return 1;
+  }
+  return (k > 1 && nrhs % k == 0) ? k : 1;
 }
 
 // upper bound of reduction partials per column over every kernel
@@ -215,6 +232,28 @@ int wl_ctx_set_march(wl_ctx* c, int prec, int march) {
     return fail(WL_ERR_BAD_ARG, "march %d must be 0 (automatic) or divide L0=%d", march, c->geom.L[0]);
   std::lock_guard<std::mutex> lk(c->mu);
   c->march[prec] = march;
+  return WL_OK;
+}
+
+int wl_ctx_set_mrhs(wl_ctx* c, int prec, int k) {
+  CHK(check_ctx(c));
+  if (prec < F64 || prec > F16) return fail(WL_ERR_BAD_ARG, "bad precision %d", prec);
+  if (k != 0 && k != 1 && k != 2 && k != 4) return fail(WL_ERR_BAD_ARG, "mrhs must be 0 (automatic), 1, 2 or 4");
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->mrhs[prec] = k;
+  return WL_OK;
+}
+
+int wl_ctx_get_mrhs(wl_ctx* c, int prec, int nrhs) {
+  if (!c || prec < F64 || prec > F16 || nrhs < 1) return 0;
+  return mrhs_for(c, prec, nrhs);
+}
+
+int wl_ctx_set_xcd_remap(wl_ctx* c, int prec, int on) {
+  CHK(check_ctx(c));
+  if (prec < F64 || prec > F16) return fail(WL_ERR_BAD_ARG, "bad precision %d", prec);
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->xcd_remap[prec] = on ? 1 : 0;
   return WL_OK;
 }
 
@@ -309,6 +348,14 @@ static int gauge_encoding(wl_gauge* g, int prec, int recon, hipStream_t st, cons
   return WL_OK;
 }
 
+// launch geometry once ncol and cols_per_cfg are set
+static void set_launch_shape(const wl_ctx* c, DslashArgs& a, int prec) {
+  a.mrhs = mrhs_for(c, prec, a.cols_per_cfg);
+  if (a.mrhs > 1) a.march = 1;
+  a.nseg = c->geom.L[0] / a.march;
+  a.ntiles = a.march == 1 ? (int)((c->V + NT - 1) / NT) : nchunk_of(c) * a.nseg;
+}
+
 static void fill_dslash_common(const wl_ctx* c, DslashArgs& a, int prec) {
   memset(&a, 0, sizeof(a));
   a.g = c->g;
@@ -319,6 +366,7 @@ static void fill_dslash_common(const wl_ctx* c, DslashArgs& a, int prec) {
   a.m0 = c->params.mass + 4.0 * c->params.wilson_r;
   a.r = c->params.wilson_r;
   a.march = march_for(c, prec);
+  a.xcd_remap = c->xcd_remap[prec];
   a.nseg = c->geom.L[0] / a.march;
   // march 1 runs the one-site-per-thread kernel over flat tiles of NT sites
   a.ntiles = a.march == 1 ? (int)((c->V + NT - 1) / NT) : nchunk_of(c) * a.nseg;
@@ -338,6 +386,7 @@ static int dslash_internal(wl_ctx* c, wl_gauge* g, int op, int prec, int recon, 
   a.col_bytes = col_bytes;
   a.ncol = ncol;
   a.cols_per_cfg = nrhs;
+  set_launch_shape(c, a, prec);
   a.in = in;
   a.out = out;
   hipError_t e = launch_dslash(prec, c->geom.nc, rc, op, 0, a, st);
@@ -475,6 +524,7 @@ struct Solver {
   int P, R;       // inner precision, recon
   int PY;         // precision of the inner solution accumulator
   bool mixed;
+  bool fused;     // reliable updates run inside every iteration (device-driven)
   size_t cb64, cbP, cbY;
   const void* U64;
   const void* UP;
@@ -490,6 +540,7 @@ struct Solver {
     a.dir_bytes = a.cfg_bytes / 4;
     a.ncol = ncol;
     a.cols_per_cfg = nrhs;
+    set_launch_shape(c, a, prec);
     a.m0_from_state = 1;
     a.red = red;
   }
@@ -573,6 +624,10 @@ struct Solver {
       e = launch_xpay(P, nc, a, st);
       if (e != hipSuccess) return fail(WL_ERR_HIP, "xpay: %s", hipGetErrorString(e));
     }
+    // Mixed precision: the reliable update runs on the device right after the
+    // iteration that requested it (all its kernels are gated on the column's
+    // need_update flag), so no host round trip is needed.
+    if (fused) return reliable_update();
     return WL_OK;
   }
 };
@@ -598,6 +653,7 @@ static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int n
   s.P = sp->inner_prec;
   s.R = (nc == 1) ? 18 : (sp->inner_recon == 12 ? 12 : 18);
   s.mixed = s.P != F64;
+  s.fused = s.mixed;
   s.PY = s.mixed ? F32 : F64;
   s.cb64 = col_bytes_of(c, F64);
   s.cbP = col_bytes_of(c, s.P);
@@ -675,33 +731,72 @@ static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int n
   HIPCHK(hipMemcpyAsync(c->cols.p, hc, sizeof(ColState) * ncol, hipMemcpyHostToDevice, st));
   CHK(s.reliable_update());
 
+  // Iteration blocks of `poll`, 4 and 1 iterations (as HIP graphs when
+  // requested); after each block the host reads the per-column state and
+  // picks the next block from the predicted number of remaining iterations
+  // (the observed residual decay rate), so little work is queued past
+  // convergence.
   const int poll = sp->poll_every > 0 ? sp->poll_every : 8;
-  hipGraphExec_t gexec = nullptr;
-  hipGraph_t graph = nullptr;
+  std::vector<int> sizes;
+  for (int n : {poll, 4, 1})
+    if (n <= poll && (sizes.empty() || n < sizes.back())) sizes.push_back(n);
+  std::vector<hipGraph_t> graphs(sizes.size(), nullptr);
+  std::vector<hipGraphExec_t> execs(sizes.size(), nullptr);
+  auto cleanup = [&]() {
+    for (auto& e : execs)
+      if (e) (void)hipGraphExecDestroy(e);
+    for (auto& g : graphs)
+      if (g) (void)hipGraphDestroy(g);
+  };
   if (sp->use_graph) {
-    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    int rc = WL_OK;
-    for (int i = 0; i < poll && rc == WL_OK; ++i) rc = s.iteration();
-    hipError_t e = hipStreamEndCapture(st, &graph);
-    if (rc != WL_OK) return rc;
-    if (e != hipSuccess) return fail(WL_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
-    e = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
-    if (e != hipSuccess) {
-      (void)hipGraphDestroy(graph);
-      return fail(WL_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+    for (size_t k = 0; k < sizes.size(); ++k) {
+      hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+      if (e != hipSuccess) {
+        cleanup();
+        return fail(WL_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+      }
+      int rc = WL_OK;
+      for (int i = 0; i < sizes[k] && rc == WL_OK; ++i) rc = s.iteration();
+      e = hipStreamEndCapture(st, &graphs[k]);
+      if (rc != WL_OK) {
+        cleanup();
+        return rc;
+      }
+      if (e == hipSuccess) e = hipGraphInstantiate(&execs[k], graphs[k], nullptr, nullptr, 0);
+      if (e != hipSuccess) {
+        cleanup();
+        return fail(WL_ERR_HIP, "graph capture/instantiate: %s", hipGetErrorString(e));
+      }
     }
   }
+  std::vector<double> prev_rr(ncol, 0.0);
+  std::vector<int> prev_it(ncol, 0);
   int result = WL_OK;
   long long guard = 0;
-  const long long guard_max = (long long)sp->maxiter / poll + 64LL + 8LL * sp->maxiter;
+  const long long guard_max = 64LL + 8LL * sp->maxiter;
   while (true) {
     HIPCHK(hipMemcpyAsync(hc, c->cols.p, sizeof(ColState) * ncol, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     bool all_done = true, any_update = false, any_active = false;
+    double remaining = 0.0;
+    bool have_rate = false;
     for (int i = 0; i < ncol; ++i) {
-      if (!hc[i].done) all_done = false;
-      if (hc[i].need_update && !hc[i].done) any_update = true;
-      if (hc[i].active) any_active = true;
+      const ColState& cs = hc[i];
+      if (!cs.done) all_done = false;
+      if (cs.need_update && !cs.done) any_update = true;
+      if (!cs.active) continue;
+      any_active = true;
+      if (cs.iter > prev_it[i] && prev_rr[i] > cs.rr && cs.rr > 0.0 && cs.stop2 > 0.0) {
+        double rate = std::log(prev_rr[i] / cs.rr) / (cs.iter - prev_it[i]);
+        double left = cs.rr > cs.stop2 ? std::log(cs.rr / cs.stop2) / rate : 1.0;
+        left = std::min(left, (double)(cs.maxiter - cs.iter));
+        remaining = std::max(remaining, left);
+        have_rate = true;
+      } else {
+        remaining = std::max(remaining, (double)poll);
+      }
+      prev_rr[i] = cs.rr;
+      prev_it[i] = cs.iter;
     }
     if (all_done) break;
     if (++guard > guard_max) {
@@ -717,19 +812,22 @@ static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int n
       result = fail(WL_ERR_HIP, "solver state inconsistent: no active column");
       break;
     }
-    if (gexec) {
-      hipError_t e = hipGraphLaunch(gexec, st);
+    size_t k = 0;
+    if (have_rate) {
+      while (k + 1 < sizes.size() && sizes[k] > remaining + 1.0) ++k;
+    }
+    if (execs[k]) {
+      hipError_t e = hipGraphLaunch(execs[k], st);
       if (e != hipSuccess) {
         result = fail(WL_ERR_HIP, "graph launch: %s", hipGetErrorString(e));
         break;
       }
     } else {
-      for (int i = 0; i < poll && result == WL_OK; ++i) result = s.iteration();
+      for (int i = 0; i < sizes[k] && result == WL_OK; ++i) result = s.iteration();
       if (result != WL_OK) break;
     }
   }
-  if (gexec) (void)hipGraphExecDestroy(gexec);
-  if (graph) (void)hipGraphDestroy(graph);
+  cleanup();
   if (result != WL_OK) return result;
   if (s.mixed) {
     // fold any remaining inner correction (columns stopped by NaN/breakdown)
@@ -789,6 +887,263 @@ int wl_cg_normal(wl_ctx* c, wl_gauge* g, const void* b, void* x, const void* x0,
   CHK(cg_fields(c, g, c->ws[7].p, c->ws[8].p, x0 != nullptr, nrhs, sp, stats, st));
   e = unpack_spinor(F64, nc, c->ws[8].p, x, dtype, c->V, c->Vp, ncol, cb, st);
   if (e != hipSuccess) return fail(WL_ERR_HIP, "unpack: %s", hipGetErrorString(e));
+  return WL_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Continuous-batching solve engine: a persistent set of `capacity` columns on
+// one resident gauge configuration.  Right-hand sides are admitted into free
+// columns and retired when converged while the others keep iterating; one
+// captured iteration graph serves every occupancy because all kernels gate
+// per column on the device state (free columns are "done").
+// ---------------------------------------------------------------------------
+struct wl_batch {
+  wl_ctx* c = nullptr;
+  wl_gauge* g = nullptr;
+  hipStream_t st = nullptr;
+  int cap = 0;
+  int poll = 8;
+  double tol2 = 0.0;
+  wl_solve_params sp{};
+  Solver s{};
+  DevBuf bufs[11];
+  DevBuf partials, cols;
+  ColState* host = nullptr;
+  std::vector<char> busy;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::mutex mu;
+};
+
+static size_t batch_bytes_per_col(const wl_ctx* c, int prec) {
+  const bool mixed = prec != F64;
+  size_t b = 4 * col_bytes_of(c, F64) + 3 * col_bytes_of(c, prec);
+  if (mixed) b += col_bytes_of(c, prec) + col_bytes_of(c, F32);
+  return b + sizeof(double) * max_tiles(c) + sizeof(ColState);
+}
+
+static void batch_free(wl_batch* bt) {
+  if (bt->exec) (void)hipGraphExecDestroy(bt->exec);
+  if (bt->graph) (void)hipGraphDestroy(bt->graph);
+  for (auto& b : bt->bufs) b.release();
+  bt->partials.release();
+  bt->cols.release();
+  if (bt->host) (void)hipHostFree(bt->host);
+  delete bt;
+}
+
+extern "C" {
+
+size_t wl_batch_bytes(wl_ctx* c, int capacity, int inner_prec) {
+  if (!c || capacity < 1 || inner_prec < F64 || inner_prec > F16) return 0;
+  return (size_t)capacity * batch_bytes_per_col(c, inner_prec);
+}
+
+int wl_batch_create(wl_ctx* c, wl_gauge* g, int capacity, const wl_solve_params* sp, void* stream, wl_batch** out) {
+  CHK(check_ctx(c));
+  if (!g || g->ctx != c || !sp || !out) return fail(WL_ERR_BAD_ARG, "null or foreign handle");
+  if (c->geom.batch != 1) return fail(WL_ERR_BAD_ARG, "a solve batch needs a context with one gauge configuration");
+  if (capacity < 1) return fail(WL_ERR_BAD_ARG, "capacity must be >= 1");
+  if (sp->inner_prec < F64 || sp->inner_prec > F16) return fail(WL_ERR_BAD_ARG, "bad inner precision %d", sp->inner_prec);
+  if (!(sp->tol >= 0.0) || sp->maxiter < 0) return fail(WL_ERR_BAD_ARG, "bad tol / maxiter");
+  if (stream == nullptr && sp->use_graph) return fail(WL_ERR_BAD_ARG, "graph capture needs a non-default stream");
+  std::lock_guard<std::mutex> lk(c->mu);
+  wl_batch* bt = new wl_batch();
+  bt->c = c;
+  bt->g = g;
+  bt->st = (hipStream_t)stream;
+  bt->cap = capacity;
+  bt->sp = *sp;
+  bt->sp.mass = nullptr;
+  bt->poll = sp->poll_every > 0 ? sp->poll_every : 8;
+  bt->tol2 = sp->tol * sp->tol;
+  bt->busy.assign(capacity, 0);
+  Solver& s = bt->s;
+  const int nc = c->geom.nc;
+  s.c = c;
+  s.g = g;
+  s.st = bt->st;
+  s.nc = nc;
+  s.ncol = capacity;
+  s.nrhs = capacity;
+  s.ntiles = (int)((c->V + NT - 1) / NT);
+  s.P = sp->inner_prec;
+  s.R = (nc == 1) ? 18 : (sp->inner_recon == 12 ? 12 : 18);
+  s.mixed = s.P != F64;
+  s.fused = true;
+  s.PY = s.mixed ? F32 : F64;
+  s.cb64 = col_bytes_of(c, F64);
+  s.cbP = col_bytes_of(c, s.P);
+  s.cbY = col_bytes_of(c, s.PY);
+  int rc = WL_OK;
+  auto alloc = [&](int i, size_t bytes) -> void* {
+    if (rc == WL_OK) rc = bt->bufs[i].ensure(bytes);
+    return bt->bufs[i].p;
+  };
+  s.b = alloc(0, s.cb64 * capacity);
+  s.x = alloc(1, s.cb64 * capacity);
+  s.rhi = alloc(2, s.cb64 * capacity);
+  s.thi = alloc(3, s.cb64 * capacity);
+  s.p = alloc(4, s.cbP * capacity);
+  s.Ap = alloc(5, s.cbP * capacity);
+  s.t = alloc(6, s.cbP * capacity);
+  if (s.mixed) {
+    s.r = alloc(7, s.cbP * capacity);
+    s.y = alloc(8, s.cbY * capacity);
+  } else {
+    s.r = s.rhi;
+    s.y = s.x;
+  }
+  if (rc == WL_OK) rc = bt->partials.ensure(sizeof(double) * (size_t)capacity * max_tiles(c));
+  if (rc == WL_OK) rc = bt->cols.ensure(sizeof(ColState) * capacity);
+  if (rc == WL_OK) rc = gauge_encoding(g, F64, 18, bt->st, &s.U64);
+  if (rc == WL_OK) rc = gauge_encoding(g, s.P, s.R, bt->st, &s.UP);
+  if (rc != WL_OK) {
+    batch_free(bt);
+    return rc;
+  }
+  if (hipHostMalloc((void**)&bt->host, sizeof(ColState) * capacity, hipHostMallocDefault) != hipSuccess) {
+    batch_free(bt);
+    return fail(WL_ERR_OOM, "pinned host state");
+  }
+  s.red.partials = (double*)bt->partials.p;
+  s.red.cols = (ColState*)bt->cols.p;
+  s.red.action = RA_NONE;
+  s.red.only_col = -1;
+  for (int i = 0; i < capacity; ++i) {
+    memset(&bt->host[i], 0, sizeof(ColState));
+    bt->host[i].done = 1;
+    bt->host[i].status = WL_STATUS_RUNNING;
+  }
+  hipError_t e = hipMemcpyAsync(bt->cols.p, bt->host, sizeof(ColState) * capacity, hipMemcpyHostToDevice, bt->st);
+  for (int i = 0; i < 9 && e == hipSuccess; ++i)
+    if (bt->bufs[i].p) e = hipMemsetAsync(bt->bufs[i].p, 0, bt->bufs[i].n, bt->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(bt->st);
+  if (e != hipSuccess) {
+    batch_free(bt);
+    return fail(WL_ERR_HIP, "batch init: %s", hipGetErrorString(e));
+  }
+  if (sp->use_graph) {
+    e = hipStreamBeginCapture(bt->st, hipStreamCaptureModeThreadLocal);
+    int r2 = WL_OK;
+    for (int i = 0; i < bt->poll && r2 == WL_OK && e == hipSuccess; ++i) r2 = s.iteration();
+    hipError_t e2 = hipStreamEndCapture(bt->st, &bt->graph);
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess) e = hipGraphInstantiate(&bt->exec, bt->graph, nullptr, nullptr, 0);
+    if (r2 != WL_OK || e != hipSuccess) {
+      batch_free(bt);
+      return r2 != WL_OK ? r2 : fail(WL_ERR_HIP, "batch graph: %s", hipGetErrorString(e));
+    }
+  }
+  *out = bt;
+  return WL_OK;
+}
+
+void wl_batch_destroy(wl_batch* bt) {
+  if (!bt) return;
+  (void)hipStreamSynchronize(bt->st);
+  batch_free(bt);
+}
+
+int wl_batch_admit(wl_batch* bt, int slot, const void* b, int dtype, double mass) {
+  if (!bt || !b) return fail(WL_ERR_BAD_ARG, "null argument");
+  if (slot < 0 || slot >= bt->cap) return fail(WL_ERR_BAD_ARG, "slot %d out of range", slot);
+  if (dtype != WL_C128 && dtype != WL_C64) return fail(WL_ERR_BAD_ARG, "bad dtype %d", dtype);
+  std::lock_guard<std::mutex> lk(bt->mu);
+  if (bt->busy[slot]) return fail(WL_ERR_BAD_ARG, "slot %d is occupied", slot);
+  wl_ctx* c = bt->c;
+  Solver& s = bt->s;
+  const int nc = c->geom.nc;
+  hipStream_t st = bt->st;
+  hipError_t e = pack_spinor(F64, nc, b, dtype, (char*)s.b + slot * s.cb64, c->V, c->Vp, 1, s.cb64, st);
+  if (e == hipSuccess) e = hipMemsetAsync((char*)s.x + slot * s.cb64, 0, s.cb64, st);
+  if (e == hipSuccess) e = hipMemsetAsync((char*)s.p + slot * s.cbP, 0, s.cbP, st);
+  if (e == hipSuccess && s.mixed) e = hipMemsetAsync((char*)s.y + slot * s.cbY, 0, s.cbY, st);
+  if (e != hipSuccess) return fail(WL_ERR_HIP, "admit: %s", hipGetErrorString(e));
+  ColState& cs = bt->host[slot];
+  memset(&cs, 0, sizeof(cs));
+  cs.m0 = mass + 4.0 * c->params.wilson_r;
+  cs.maxiter = bt->sp.maxiter;
+  cs.mode = bt->sp.stop_rule == WL_STOP_ITERATED ? 1 : 0;
+  cs.delta = s.mixed ? bt->sp.delta : 0.0;
+  cs.status = WL_STATUS_RUNNING;
+  e = hipMemcpyAsync((ColState*)bt->cols.p + slot, &cs, sizeof(ColState), hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) return fail(WL_ERR_HIP, "admit: %s", hipGetErrorString(e));
+  BlasArgs a = s.bargs(3);
+  a.red.only_col = slot;
+  a.red.action = RA_B2;
+  a.a = s.b;
+  a.col_bytes_a = s.cb64;
+  e = launch_norm2(F64, nc, a, st);
+  if (e == hipSuccess) e = launch_admit_finish((ColState*)bt->cols.p, slot, bt->tol2, st);
+  if (e != hipSuccess) return fail(WL_ERR_HIP, "admit: %s", hipGetErrorString(e));
+  bt->busy[slot] = 1;
+  return WL_OK;
+}
+
+int wl_batch_step(wl_batch* bt) {
+  if (!bt) return fail(WL_ERR_BAD_ARG, "null batch");
+  std::lock_guard<std::mutex> lk(bt->mu);
+  if (bt->exec) {
+    hipError_t e = hipGraphLaunch(bt->exec, bt->st);
+    if (e != hipSuccess) return fail(WL_ERR_HIP, "graph launch: %s", hipGetErrorString(e));
+    return WL_OK;
+  }
+  for (int i = 0; i < bt->poll; ++i) CHK(bt->s.iteration());
+  return WL_OK;
+}
+
+int wl_batch_query(wl_batch* bt, wl_solve_stats* stats, int* state) {
+  if (!bt) return fail(WL_ERR_BAD_ARG, "null batch");
+  std::lock_guard<std::mutex> lk(bt->mu);
+  hipError_t e = hipMemcpyAsync(bt->host, bt->cols.p, sizeof(ColState) * bt->cap, hipMemcpyDeviceToHost, bt->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(bt->st);
+  if (e != hipSuccess) return fail(WL_ERR_HIP, "query: %s", hipGetErrorString(e));
+  for (int i = 0; i < bt->cap; ++i) {
+    const ColState& cs = bt->host[i];
+    if (state) state[i] = !bt->busy[i] ? 0 : (cs.done ? 2 : 1);
+    if (stats) {
+      double nb = cs.b2 > 0 ? std::sqrt(cs.b2) : 1.0;
+      stats[i].iters = cs.iter;
+      stats[i].status = cs.status;
+      stats[i].reliable_updates = cs.nupdates;
+      stats[i].pad = 0;
+      stats[i].true_resid = std::sqrt(cs.rtrue2) / nb;
+      stats[i].iter_resid = std::sqrt(cs.rr) / nb;
+    }
+  }
+  return WL_OK;
+}
+
+int wl_batch_retire(wl_batch* bt, int slot, void* x, int dtype) {
+  if (!bt || !x) return fail(WL_ERR_BAD_ARG, "null argument");
+  if (slot < 0 || slot >= bt->cap) return fail(WL_ERR_BAD_ARG, "slot %d out of range", slot);
+  if (dtype != WL_C128 && dtype != WL_C64) return fail(WL_ERR_BAD_ARG, "bad dtype %d", dtype);
+  std::lock_guard<std::mutex> lk(bt->mu);
+  if (!bt->busy[slot]) return fail(WL_ERR_BAD_ARG, "slot %d is free", slot);
+  wl_ctx* c = bt->c;
+  Solver& s = bt->s;
+  // a column stopped early (NaN, breakdown) may hold an unfolded correction
+  if (s.mixed) {
+    BlasArgs a = s.bargs(3);
+    a.red.only_col = slot;
+    a.a = s.x; a.col_bytes_a = s.cb64;
+    a.b = s.y; a.col_bytes_b = s.cbY;
+    hipError_t e = launch_accumulate(F64, s.PY, c->geom.nc, a, bt->st);
+    if (e != hipSuccess) return fail(WL_ERR_HIP, "retire: %s", hipGetErrorString(e));
+  }
+  hipError_t e = unpack_spinor(F64, c->geom.nc, (char*)s.x + slot * s.cb64, x, dtype, c->V, c->Vp, 1, s.cb64, bt->st);
+  if (e == hipSuccess) {
+    ColState& cs = bt->host[slot];
+    memset(&cs, 0, sizeof(cs));
+    cs.done = 1;
+    cs.status = WL_STATUS_RUNNING;
+    e = hipMemcpyAsync((ColState*)bt->cols.p + slot, &cs, sizeof(ColState), hipMemcpyHostToDevice, bt->st);
+  }
+  if (e != hipSuccess) return fail(WL_ERR_HIP, "retire: %s", hipGetErrorString(e));
+  bt->busy[slot] = 0;
   return WL_OK;
 }
 

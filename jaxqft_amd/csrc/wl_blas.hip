@@ -29,7 +29,7 @@ __device__ __forceinline__ void store_as(void* base, size_t V, size_t site, cons
 #define WL_BLAS_PROLOGUE                                            \
   const int col = blockIdx.x % a.ncol;                              \
   const int tile = blockIdx.x / a.ncol;                             \
-  if (gate_skip(a.gate, a.red.cols[col])) return;                   \
+  if (gate_skip_col(a.gate, a.red, col)) return;                    \
   const size_t V = (size_t)a.Vp;                                    \
   const size_t site = (size_t)tile * NT + threadIdx.x;              \
   const bool valid = site < (size_t)a.V;                            \
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(NT) void k_convert(BlasArgs a) {
 
 __global__ __launch_bounds__(NT) void k_reduce_finish(Reduction red, int nblk, int gate) {
   const int col = blockIdx.x;
-  if (gate_skip(gate, red.cols[col])) return;
+  if (gate_skip_col(gate, red, col)) return;
   __shared__ double lds[NT / 64];
   const double* parts = red.partials + (size_t)col * nblk;
   double s = 0.0;
@@ -138,6 +138,22 @@ __global__ __launch_bounds__(NT) void k_reduce_finish(Reduction red, int nblk, i
 
 hipError_t launch_reduce_finish(const Reduction& red, int ncol, int nblk, int gate, hipStream_t stream) {
   hipLaunchKernelGGL(k_reduce_finish, dim3(ncol), dim3(NT), 0, stream, red, nblk, gate);
+  return hipGetLastError();
+}
+
+// A newly admitted column: targets from |b|^2, then the next reliable update
+// computes its initial residual.
+__global__ void k_admit_finish(ColState* cols, int col, double tol2) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    ColState& s = cols[col];
+    s.tol2b2 = tol2 * s.b2;
+    s.stop2 = tol2 * s.b2;
+    s.need_update = 1;
+  }
+}
+
+hipError_t launch_admit_finish(ColState* cols, int col, double tol2, hipStream_t stream) {
+  hipLaunchKernelGGL(k_admit_finish, dim3(1), dim3(64), 0, stream, cols, col, tol2);
   return hipGetLastError();
 }
 

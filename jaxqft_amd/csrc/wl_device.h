@@ -322,7 +322,8 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
 // Map the hardware workgroup id to a logical tile so that each XCD (workgroups
 // are dispatched round-robin over the 8 XCDs) walks one contiguous range of the
 // lattice; neighbour planes then stay in that XCD's L2.  Bijective for any n.
-__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned n) {
+__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned n, int remap = 1) {
+  if (!remap) return b;
   const unsigned X = 8;
   unsigned per = n / X, rem = n % X;
   unsigned xcd = b % X, idx = b / X;

@@ -1,0 +1,392 @@
+// Wilson operator kernels: D, D^dag, gamma5 D, the hopping term K and K^dag,
+// with optional fused reductions for CG (Re<p, A p> and |b - A x|^2).
+//
+// The 1D grid is ntiles x ncol workgroups ordered spinor-column-fastest and
+// remapped so that each XCD walks one contiguous range of tiles: the k
+// right-hand sides of one tile run back to back on the same XCD and share
+// its link lines in L2.
+#pragma once
+#include "wl_kernels.h"
+#include "wl_reduce.h"
+#include "wl_stencil.h"
+
+namespace wl {
+
+// gamma5 = sigma1 x sigma3 in this basis: (g5 psi)_s = (psi2, -psi3, psi0, -psi1)
+template <class T, int NC>
+__device__ __forceinline__ void apply_g5(cx<T> (&v)[4 * NC]) {
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    cx<T> a0 = v[0 * NC + c], a1 = v[1 * NC + c], a2 = v[2 * NC + c], a3 = v[3 * NC + c];
+    v[0 * NC + c] = a2;
+    v[1 * NC + c] = {-a3.re, -a3.im};
+    v[2 * NC + c] = a0;
+    v[3 * NC + c] = {-a1.re, -a1.im};
+  }
+}
+
+template <int P, int NC>
+__device__ __forceinline__ void store_nt(void* base, size_t V, size_t site, const cx<typename PT<P>::acc> (&v)[4 * NC]) {
+  SpinorIO<P, NC>::store(base, V, site, v);
+}
+
+// One site per thread (march = 1): every neighbour comes from memory.
+// EPI: 0 store, 1 store + Re<w, out>, 2 out = w - result and |out|^2
+template <int P, int NC, int RECON, bool DAG, bool DIAG, bool G5, int EPI>
+__global__ __launch_bounds__(NT) void k_dslash_site(DslashArgs a) {
+  using T = typename PT<P>::acc;
+  const unsigned logical = xcd_tile(blockIdx.x, gridDim.x, a.xcd_remap);
+  const int col = logical % a.ncol;
+  const int tile = logical / a.ncol;
+  const ColState* st = a.red.cols;
+  if (a.gate && gate_skip_col(a.gate, a.red, col)) return;
+
+  const size_t V = (size_t)a.g.Vp;  // plane stride
+  const size_t site = (size_t)tile * NT + threadIdx.x;
+  const bool valid = site < (size_t)a.g.V;
+  const size_t s = valid ? site : 0;
+
+  const int cfg = col / a.cols_per_cfg;
+  const char* Ucfg = reinterpret_cast<const char*>(a.U) + (size_t)cfg * a.cfg_bytes;
+  const char* inb = reinterpret_cast<const char*>(a.in) + (size_t)col * a.col_bytes;
+  char* outb = reinterpret_cast<char*>(a.out) + (size_t)col * a.col_bytes;
+
+  // coordinates (last axis fastest)
+  int x[4];
+  {
+    size_t rem = s;
+    x[3] = rem % a.g.L[3];
+    rem /= a.g.L[3];
+    x[2] = rem % a.g.L[2];
+    rem /= a.g.L[2];
+    x[1] = rem % a.g.L[1];
+    x[0] = rem / a.g.L[1];
+  }
+
+  cx<T> acc[4 * NC];
+#pragma unroll
+  for (int c = 0; c < 4 * NC; ++c) acc[c] = {T(0), T(0)};
+
+  constexpr int SF = DAG ? +1 : -1;
+#define WL_HOP(MU)                                                                                        \
+  {                                                                                                       \
+    const int L = a.g.L[MU];                                                                              \
+    const size_t str = (size_t)a.g.stride[MU];                                                            \
+    const bool top = x[MU] == L - 1, bot = x[MU] == 0;                                                    \
+    const size_t sf = top ? s - (size_t)(L - 1) * str : s + str;                                          \
+    const size_t sb = bot ? s + (size_t)(L - 1) * str : s - str;                                          \
+    cx<T> ph = {T(a.bc_re[MU]), T(a.bc_im[MU])};                                                          \
+    hop_dir<P, NC, RECON, MU, SF>(Ucfg, a.dir_bytes, inb, V, s, sf, sb, top, bot, ph, acc);               \
+  }
+  WL_HOP(0)
+  WL_HOP(1)
+  WL_HOP(2)
+  WL_HOP(3)
+#undef WL_HOP
+
+  cx<T> res[4 * NC];
+  if (DIAG) {
+    const T m0 = a.m0_from_state ? T(st[col].m0) : T(a.m0);
+    cx<T> p[4 * NC];
+    SpinorIO<P, NC>::load(inb, V, s, p);
+#pragma unroll
+    for (int c = 0; c < 4 * NC; ++c) res[c] = {m0 * p[c].re - T(0.5) * acc[c].re, m0 * p[c].im - T(0.5) * acc[c].im};
+  } else {
+#pragma unroll
+    for (int c = 0; c < 4 * NC; ++c) res[c] = {T(-0.5) * acc[c].re, T(-0.5) * acc[c].im};
+  }
+  if (G5) apply_g5<T, NC>(res);
+
+  double part = 0.0;
+  if (EPI == 2) {
+    const char* wb = reinterpret_cast<const char*>(a.w) + (size_t)col * a.col_bytes;
+    cx<T> w[4 * NC];
+    SpinorIO<P, NC>::load(wb, V, s, w);
+#pragma unroll
+    for (int c = 0; c < 4 * NC; ++c) {
+      res[c] = csub(w[c], res[c]);
+      part += (double)res[c].re * (double)res[c].re + (double)res[c].im * (double)res[c].im;
+    }
+  } else if (EPI == 1) {
+    const char* wb = reinterpret_cast<const char*>(a.w) + (size_t)col * a.col_bytes;
+    cx<T> w[4 * NC];
+    SpinorIO<P, NC>::load(wb, V, s, w);
+#pragma unroll
+    for (int c = 0; c < 4 * NC; ++c) part += (double)w[c].re * (double)res[c].re + (double)w[c].im * (double)res[c].im;
+  }
+  if (valid) store_nt<P, NC>(outb, V, s, res);
+  if (EPI != 0) {
+    if (!valid) part = 0.0;
+    write_partial(a.red, col, tile, a.ntiles, part);
+  }
+}
+
+// Multi-RHS: one site per thread and K right-hand sides of the same gauge
+// configuration per thread; each link is loaded once and applied to K
+// spinors.  Columns col0 .. col0+K-1 share one workgroup; a column whose
+// solve is inactive is skipped (its loads and stores are predicated off).
+// EPI as for k_dslash_site, one reduction partial per column.
+template <int P, int NC, int RECON, bool DAG, bool DIAG, bool G5, int EPI, int K>
+__global__ __launch_bounds__(NT) void k_dslash_mrhs(DslashArgs a) {
+  using T = typename PT<P>::acc;
+  const int ngrp = a.ncol / K;
+  const unsigned logical = xcd_tile(blockIdx.x, gridDim.x, a.xcd_remap);
+  const int grp = logical % ngrp;
+  const int tile = logical / ngrp;
+  const int col0 = grp * K;
+  const ColState* st = a.red.cols;
+  bool on[K];
+  bool any = false;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    on[k] = !(a.gate && gate_skip_col(a.gate, a.red, col0 + k));
+    any |= on[k];
+  }
+  if (!any) return;
+
+  const size_t V = (size_t)a.g.Vp;  // plane stride
+  const size_t site = (size_t)tile * NT + threadIdx.x;
+  const bool valid = site < (size_t)a.g.V;
+  const size_t s = valid ? site : 0;
+  const int cfg = col0 / a.cols_per_cfg;
+  const char* Ucfg = reinterpret_cast<const char*>(a.U) + (size_t)cfg * a.cfg_bytes;
+
+  int x[4];
+  {
+    size_t rem = s;
+    x[3] = rem % a.g.L[3];
+    rem /= a.g.L[3];
+    x[2] = rem % a.g.L[2];
+    rem /= a.g.L[2];
+    x[1] = rem % a.g.L[1];
+    x[0] = rem / a.g.L[1];
+  }
+  cx<T> acc[K][4 * NC];
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+#pragma unroll
+    for (int q = 0; q < 4 * NC; ++q) acc[k][q] = {T(0), T(0)};
+
+  constexpr int SF = DAG ? +1 : -1;
+#define WL_MHOP(MU)                                                                                         \
+  {                                                                                                         \
+    const int L = a.g.L[MU];                                                                                \
+    const size_t str = (size_t)a.g.stride[MU];                                                              \
+    const bool top = x[MU] == L - 1, bot = x[MU] == 0;                                                      \
+    const size_t sf = top ? s - (size_t)(L - 1) * str : s + str;                                            \
+    const size_t sb = bot ? s + (size_t)(L - 1) * str : s - str;                                            \
+    const cx<T> ph = {T(a.bc_re[MU]), T(a.bc_im[MU])};                                                      \
+    const char* Umu = Ucfg + MU * a.dir_bytes;                                                              \
+    {                                                                                                       \
+      cx<T> u[NC][NC];                                                                                      \
+      LinkIO<P, NC, RECON>::load(Umu, V, s, u);                                                             \
+      _Pragma("unroll") for (int k = 0; k < K; ++k) {                                                       \
+        if (!on[k]) continue;                                                                               \
+        cx<T> p[4 * NC], h[2][NC], o[2][NC];                                                                \
+        SpinorIO<P, NC>::load(reinterpret_cast<const char*>(a.in) + (size_t)(col0 + k) * a.col_bytes, V, sf, p); \
+        spin_project<MU, SF>(p, h);                                                                         \
+        color_mul<false>(u, h, o);                                                                          \
+        if (top) half_phase(o, ph);                                                                         \
+        spin_reconstruct_add<MU, SF>(acc[k], o);                                                            \
+      }                                                                                                     \
+    }                                                                                                       \
+    {                                                                                                       \
+      cx<T> u[NC][NC];                                                                                      \
+      LinkIO<P, NC, RECON>::load(Umu, V, sb, u);                                                            \
+      _Pragma("unroll") for (int k = 0; k < K; ++k) {                                                       \
+        if (!on[k]) continue;                                                                               \
+        cx<T> p[4 * NC], h[2][NC], o[2][NC];                                                                \
+        SpinorIO<P, NC>::load(reinterpret_cast<const char*>(a.in) + (size_t)(col0 + k) * a.col_bytes, V, sb, p); \
+        spin_project<MU, -SF>(p, h);                                                                        \
+        color_mul<true>(u, h, o);                                                                           \
+        if (bot) half_phase(o, cconj(ph));                                                                  \
+        spin_reconstruct_add<MU, -SF>(acc[k], o);                                                           \
+      }                                                                                                     \
+    }                                                                                                       \
+  }
+  WL_MHOP(0)
+  WL_MHOP(1)
+  WL_MHOP(2)
+  WL_MHOP(3)
+#undef WL_MHOP
+
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int col = col0 + k;
+    double part = 0.0;
+    if (on[k]) {
+      const char* inb = reinterpret_cast<const char*>(a.in) + (size_t)col * a.col_bytes;
+      char* outb = reinterpret_cast<char*>(a.out) + (size_t)col * a.col_bytes;
+      cx<T> res[4 * NC];
+      if (DIAG) {
+        const T m0 = a.m0_from_state ? T(st[col].m0) : T(a.m0);
+        cx<T> p[4 * NC];
+        SpinorIO<P, NC>::load(inb, V, s, p);
+#pragma unroll
+        for (int q = 0; q < 4 * NC; ++q)
+          res[q] = {m0 * p[q].re - T(0.5) * acc[k][q].re, m0 * p[q].im - T(0.5) * acc[k][q].im};
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4 * NC; ++q) res[q] = {T(-0.5) * acc[k][q].re, T(-0.5) * acc[k][q].im};
+      }
+      if (G5) apply_g5<T, NC>(res);
+      if (EPI != 0) {
+        const char* wb = reinterpret_cast<const char*>(a.w) + (size_t)col * a.col_bytes;
+        cx<T> w[4 * NC];
+        SpinorIO<P, NC>::load(wb, V, s, w);
+#pragma unroll
+        for (int q = 0; q < 4 * NC; ++q) {
+          if (EPI == 2) {
+            res[q] = csub(w[q], res[q]);
+            part += (double)res[q].re * (double)res[q].re + (double)res[q].im * (double)res[q].im;
+          } else {
+            part += (double)w[q].re * (double)res[q].re + (double)w[q].im * (double)res[q].im;
+          }
+        }
+      }
+      if (valid) store_nt<P, NC>(outb, V, s, res);
+    }
+    if (EPI != 0 && on[k]) write_partial(a.red, col, tile, a.ntiles, valid ? part : 0.0);
+  }
+}
+
+// One thread owns one (x1, x2, x3) column and marches over M consecutive x0
+// sites (x0 is the largest-stride axis).  The x0 neighbours come from
+// registers: psi(x+0) loaded for the forward hop is the next site's centre,
+// and U_0(x)^dag P psi(x), formed with the link already loaded for the
+// forward hop, is the next site's backward hop.  Directions 1..3 load their
+// neighbours from memory (L1/L2).  Workgroup tiles are ordered
+// segment-fastest, so the segment that reads a boundary plane and the one
+// that owns it run back to back on the same XCD.
+//
+// EPI: 0 store, 1 store + Re<w, out> partials, 2 out = w - result and |out|^2 partials
+template <int P, int NC, int RECON, bool DAG, bool DIAG, bool G5, int EPI>
+__global__ __launch_bounds__(NT) void k_dslash(DslashArgs a) {
+  using T = typename PT<P>::acc;
+  const unsigned logical = xcd_tile(blockIdx.x, gridDim.x, a.xcd_remap);
+  const int col = logical % a.ncol;
+  const int tile = logical / a.ncol;
+  const ColState* st = a.red.cols;
+  if (a.gate && gate_skip_col(a.gate, a.red, col)) return;
+
+  const size_t V = (size_t)a.g.Vp;  // plane stride
+  const size_t S0 = (size_t)a.g.stride[0];  // sites per x0 slice
+  const int G = a.nseg;
+  const int chunk = tile / G, seg = tile % G;
+  const size_t c = (size_t)chunk * NT + threadIdx.x;
+  const bool valid = c < S0;
+  const size_t cc = valid ? c : 0;
+  const int L0 = a.g.L[0];
+  const int M = a.march;
+  const int x0s = seg * M;
+
+  const int cfg = col / a.cols_per_cfg;
+  const char* Ucfg = reinterpret_cast<const char*>(a.U) + (size_t)cfg * a.cfg_bytes;
+  const char* inb = reinterpret_cast<const char*>(a.in) + (size_t)col * a.col_bytes;
+  char* outb = reinterpret_cast<char*>(a.out) + (size_t)col * a.col_bytes;
+  const char* wb = reinterpret_cast<const char*>(a.w) + (size_t)col * a.col_bytes;
+
+  // coordinates of the column (last axis fastest)
+  int x[4];
+  {
+    size_t rem = cc;
+    x[3] = rem % a.g.L[3];
+    rem /= a.g.L[3];
+    x[2] = rem % a.g.L[2];
+    x[1] = rem / a.g.L[2];
+  }
+  // signed neighbour offsets of directions 1..3 (independent of x0)
+  long long off_f[4], off_b[4];
+  bool top[4], bot[4];
+#pragma unroll
+  for (int mu = 1; mu < 4; ++mu) {
+    const int L = a.g.L[mu];
+    const long long str = a.g.stride[mu];
+    top[mu] = x[mu] == L - 1;
+    bot[mu] = x[mu] == 0;
+    off_f[mu] = top[mu] ? -(long long)(L - 1) * str : str;
+    off_b[mu] = bot[mu] ? (long long)(L - 1) * str : -str;
+  }
+
+  constexpr int SF = DAG ? +1 : -1;
+  const cx<T> ph0 = {T(a.bc_re[0]), T(a.bc_im[0])};
+  const T m0 = DIAG ? (a.m0_from_state ? T(st[col].m0) : T(a.m0)) : T(0);
+
+  cx<T> cur[4 * NC];
+  SpinorIO<P, NC>::load(inb, V, (size_t)x0s * S0 + cc, cur);
+  cx<T> carry[2][NC];
+  {
+    const int xm = (x0s == 0) ? L0 - 1 : x0s - 1;
+    const size_t sm = (size_t)xm * S0 + cc;
+    cx<T> p[4 * NC], u[NC][NC], h[2][NC];
+    SpinorIO<P, NC>::load(inb, V, sm, p);
+    LinkIO<P, NC, RECON>::load(Ucfg, V, sm, u);
+    spin_project<0, -SF>(p, h);
+    color_mul<true>(u, h, carry);
+    if (xm == L0 - 1) half_phase(carry, cconj(ph0));
+  }
+
+  double part = 0.0;
+  for (int j = 0; j < M; ++j) {
+    const int x0 = x0s + j;
+    const size_t s = (size_t)x0 * S0 + cc;
+    const bool top0 = x0 == L0 - 1;
+    cx<T> acc[4 * NC];
+#pragma unroll
+    for (int q = 0; q < 4 * NC; ++q) acc[q] = {T(0), T(0)};
+    // direction 0, backward: carried from the previous site
+    spin_reconstruct_add<0, -SF>(acc, carry);
+    // direction 0, forward
+    cx<T> nxt[4 * NC];
+    SpinorIO<P, NC>::load(inb, V, top0 ? cc : s + S0, nxt);
+    {
+      cx<T> u[NC][NC], h[2][NC], o[2][NC];
+      LinkIO<P, NC, RECON>::load(Ucfg, V, s, u);
+      spin_project<0, SF>(nxt, h);
+      color_mul<false>(u, h, o);
+      if (top0) half_phase(o, ph0);
+      spin_reconstruct_add<0, SF>(acc, o);
+      // backward hop of the next site: U_0(x)^dag P psi(x)
+      spin_project<0, -SF>(cur, h);
+      color_mul<true>(u, h, carry);
+      if (top0) half_phase(carry, cconj(ph0));
+    }
+#define WL_HOP(MU)                                                                                          \
+    {                                                                                                       \
+      const cx<T> ph = {T(a.bc_re[MU]), T(a.bc_im[MU])};                                                    \
+      hop_dir<P, NC, RECON, MU, SF>(Ucfg, a.dir_bytes, inb, V, s, (size_t)((long long)s + off_f[MU]),       \
+                                    (size_t)((long long)s + off_b[MU]), top[MU], bot[MU], ph, acc);         \
+    }
+    WL_HOP(1)
+    WL_HOP(2)
+    WL_HOP(3)
+#undef WL_HOP
+
+    cx<T> res[4 * NC];
+#pragma unroll
+    for (int q = 0; q < 4 * NC; ++q)
+      res[q] = DIAG ? cx<T>{m0 * cur[q].re - T(0.5) * acc[q].re, m0 * cur[q].im - T(0.5) * acc[q].im}
+                    : cx<T>{T(-0.5) * acc[q].re, T(-0.5) * acc[q].im};
+    if (G5) apply_g5<T, NC>(res);
+    if (EPI == 2) {
+      cx<T> w[4 * NC];
+      SpinorIO<P, NC>::load(wb, V, s, w);
+#pragma unroll
+      for (int q = 0; q < 4 * NC; ++q) {
+        res[q] = csub(w[q], res[q]);
+        part += (double)res[q].re * (double)res[q].re + (double)res[q].im * (double)res[q].im;
+      }
+    } else if (EPI == 1) {
+      cx<T> w[4 * NC];
+      SpinorIO<P, NC>::load(wb, V, s, w);
+#pragma unroll
+      for (int q = 0; q < 4 * NC; ++q) part += (double)w[q].re * (double)res[q].re + (double)w[q].im * (double)res[q].im;
+    }
+    if (valid) store_nt<P, NC>(outb, V, s, res);
+#pragma unroll
+    for (int q = 0; q < 4 * NC; ++q) cur[q] = nxt[q];
+  }
+  if (EPI != 0) write_partial(a.red, col, tile, a.ntiles, valid ? part : 0.0);
+}
+
+}  // namespace wl

@@ -56,6 +56,7 @@ struct Reduction {
   double* partials;      // [ncol][nblk_per_col]
   ColState* cols;        // [ncol]
   int action;
+  int only_col;          // column of gate 3
 };
 
 struct DslashArgs {
@@ -72,11 +73,13 @@ struct DslashArgs {
   int ntiles;             // nchunk * nseg
   int march;              // x0 sites per thread (divides L0)
   int nseg;               // L0 / march
+  int xcd_remap;          // map workgroups so that each XCD walks a contiguous tile range
+  int mrhs;               // right-hand sides per thread (1, 2 or 4; > 1 implies one site per thread)
   double bc_re[4], bc_im[4];
   double m0;              // diagonal coefficient (used when m0_from_state == 0)
   double r;               // Wilson r
   int m0_from_state;      // take m0 per column from red.cols[col].m0
-  int gate;               // 0 all columns, 1 active columns, 2 columns awaiting a reliable update
+  int gate;               // 0 all, 1 active columns, 2 columns awaiting a reliable update, 3 column red.only_col
   Reduction red;
 };
 
@@ -85,7 +88,7 @@ struct BlasArgs {
   long long Vp;
   int ncol;
   int ntiles;
-  int gate;  // 0 none, 1 active columns, 2 columns awaiting a reliable update
+  int gate;  // 0 none, 1 active columns, 2 columns awaiting a reliable update, 3 column red.only_col
   size_t col_bytes_a, col_bytes_b, col_bytes_c, col_bytes_d;
   void* a;
   void* b;

@@ -25,6 +25,12 @@ __device__ __forceinline__ bool gate_skip(int gate, const ColState& s) {
   return false;
 }
 
+// gate 3 selects one column (admission of a new right-hand side)
+__device__ __forceinline__ bool gate_skip_col(int gate, const Reduction& red, int col) {
+  if (gate == 3) return col != red.only_col;
+  return gate_skip(gate, red.cols[col]);
+}
+
 __device__ __forceinline__ void col_finish(ColState& s, int status) {
   s.status = status;
   s.done = 1;

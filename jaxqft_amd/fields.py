@@ -79,3 +79,28 @@ def point_sources(lattice_shape: Sequence[int], site: Sequence[int], ns: int = 4
         for c in range(nc):
             out[(s * nc + c, *tuple(int(v) for v in site), s, c)] = 1.0
     return out
+
+
+def su3_hot_start_device(seed: int, batch: int, lattice_shape: Sequence[int], eps: float, device="cuda",
+                         chunk: int = 1 << 20):
+    """Same distribution as su3_hot_start, drawn on the device with a seeded
+    torch generator (a different random stream from the NumPy one): the
+    exponential of the projected algebra element via torch.linalg.matrix_exp.
+    Returns a complex128 tensor (B, 4, *L, 3, 3) on `device`."""
+    import torch
+
+    L = tuple(int(v) for v in lattice_shape)
+    shape = (int(batch), len(L), *L, 3, 3)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed))
+    out = torch.empty(shape, dtype=torch.complex128, device=device)
+    flat = out.view(-1, 3, 3)
+    eye = torch.eye(3, dtype=torch.complex128, device=device)
+    for s in range(0, flat.shape[0], chunk):
+        n = min(chunk, flat.shape[0] - s)
+        a = torch.complex(torch.randn((n, 3, 3), generator=gen, device=device, dtype=torch.float64),
+                          torch.randn((n, 3, 3), generator=gen, device=device, dtype=torch.float64)) * float(eps)
+        y = 0.5 * (a - a.conj().transpose(-1, -2))
+        y = y - (torch.diagonal(y, dim1=-2, dim2=-1).sum(-1) / 3.0)[:, None, None] * eye
+        flat[s:s + n] = torch.linalg.matrix_exp(y)
+    return out

@@ -207,3 +207,40 @@ def test_low_precision_stencils(prec, recon, tol, march):
         nat.check(ctx._lib.wl_dslash(ctx.h, g.h, op, recon, fi.h, fo.h, 1, nat.c_void_p(nat.current_stream())))
         fo.export(out)
         assert rel(out.cpu().numpy(), ref) < tol, (op, rel(out.cpu().numpy(), ref))
+
+
+@pytest.mark.parametrize("prec,recon,tol", [("f64", 18, 1e-13), ("f32", 12, 2e-6), ("f16", 12, 3e-3)])
+@pytest.mark.parametrize("per_thread", [1, 2, 4])
+def test_multi_rhs_stencil(prec, recon, tol, per_thread):
+    """k right-hand sides sharing each link load (1, 2 or 4 per thread), two
+    gauge configurations, against the oracle column by column."""
+    print(f"[gpu] multi-rhs stencil {prec} k/thread={per_thread}", flush=True)
+    import torch
+
+    from jaxqft_amd import _native as nat
+    from jaxqft_amd import fields
+    from oracle.wilson_oracle import WilsonOracle
+
+    L = (4, 8, 4, 6)
+    bc = (1, -1, 1, -1)
+    k, B = 4, 2
+    rng = fields.rng_for(1100)
+    U = fields.su3_hot_start(rng, B, L, 0.5)
+    psi = fields.random_fermion(rng, (B * k, *L, 4, 3))
+    ctx = nat.Context(L, 3, B, 0.04, 1.0, bc)
+    pc = nat.PREC[prec]
+    ctx.set_mrhs(pc, per_thread)
+    assert ctx.get_mrhs(pc, k) == per_thread
+    g = nat.Gauge(ctx, torch.from_numpy(U).cuda(), nat.WL_LAYOUT_BMXYIJ)
+    fi, fo = nat.Field(ctx, pc, B * k), nat.Field(ctx, pc, B * k)
+    fi.import_(torch.from_numpy(psi).cuda())
+    out = torch.empty((B * k, *L, 4, 3), dtype=torch.complex128, device="cuda")
+    ora = WilsonOracle(L, 3, 0.04, fermion_bc=bc)
+    for op, fn in ((nat.WL_OP_D, ora.apply_D), (nat.WL_OP_DDAG, ora.apply_Ddag), (nat.WL_OP_G5D, ora.apply_g5D)):
+        nat.check(ctx._lib.wl_dslash(ctx.h, g.h, op, recon, fi.h, fo.h, k, nat.c_void_p(nat.current_stream())))
+        fo.export(out)
+        got = out.cpu().numpy()
+        for c in range(B * k):
+            cfg = c // k
+            ref = fn(U[cfg:cfg + 1], psi[c:c + 1])
+            assert rel(got[c:c + 1], ref) < tol, (op, c)
