@@ -56,6 +56,7 @@ def parse():
     ap.add_argument("--inner", type=str, default="f16", help="inner precision of the mixed solve")
     ap.add_argument("--recon", type=int, default=12)
     ap.add_argument("--delta", type=float, default=0.01, help="reliable-update trigger of the headline solve")
+    ap.add_argument("--accum", type=str, default="f16", help="inner solution accumulator of the headline solve: f16 | auto (f32) | f64")
     ap.add_argument("--poll", type=int, default=16)
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--dslash-reps", type=int, default=50)
@@ -91,7 +92,8 @@ class Bench:
         nat = self.nat
         a = self.args
         sp, keep = nat.solve_params(a.tol, 20000, nat.WL_STOP_TRUE_RESIDUAL, nat.PREC[prec], recon,
-                                    a.delta if delta is None else delta, a.poll, bool(a.graph), masses)
+                                    a.delta if delta is None else delta, a.poll, bool(a.graph), masses,
+                                    accum=a.accum if delta is None else "auto")
         st = (nat.WlSolveStats * b.ncol)()
         nat.check(self.lib.wl_cg_normal_field(ctx.h, gauge.h, b.h, x.h, 0, nrhs, byref(sp), st, self.h))
         del keep
@@ -120,14 +122,15 @@ def summarize(stats):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU path is a design "
-              "(see the domain-decomposition section of DESIGN.md) and is not run.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}): the Wilson solver here runs on a "
+              "single GPU, and sharding the lattice over GPUs (a halo exchange per stencil application) "
+              "is not built.", file=sys.stderr)
         sys.exit(2)
     import torch
 
     from jaxqft_amd import _native as nat
     from jaxqft_amd import fields
-    from jaxqft_amd.autotune import time_dslash, tune_march
+    from jaxqft_amd.autotune import time_dslash, tune, tune_march
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
@@ -158,19 +161,28 @@ def main():
             pc = nat.PREC[prec]
             fi, fo = nat.Field(ctx, pc, 1), nat.Field(ctx, pc, 1)
             fi.import_(b1_t)
-            tune = tune_march(ctx, gauge, pc, recon, fi, fo, reps=max(3, args.dslash_reps // 5), stream=stream)
+            t1 = tune_march(ctx, gauge, pc, recon, fi, fo, reps=max(3, args.dslash_reps // 5), stream=stream)
             us = time_dslash(ctx, gauge, pc, recon, fi, fo, 1, args.dslash_reps, stream)
             bps = dslash_bytes_per_site(prec, recon)
             gbs = bps * V / (us * 1e-6) / 1e9
             dslash[f"{prec}_r{recon}"] = {
                 "us": round(us, 2), "GB/s": round(gbs, 1), "frac_hbm_peak": round(gbs / PEAK_HBM_GBS, 4),
                 "GFLOP/s": round(FLOP_PER_SITE * V / (us * 1e-6) / 1e9, 1), "bytes_per_site": bps,
-                "march": tune["best"], "xcd_remap": tune["xcd_remap"], "tune_us": {m: round(t, 2) for m, t in tune["us"].items()},
+                "march": t1["best"], "xcd_remap": t1["xcd_remap"], "tune_us": {m: round(t, 2) for m, t in t1["us"].items()},
             }
-            log(f"dslash {prec} r{recon}: {us:.1f} us {gbs:.0f} GB/s march {tune['best']}")
+            log(f"dslash {prec} r{recon}: {us:.1f} us {gbs:.0f} GB/s march {t1['best']}")
             del fi, fo
 
         if args.profile_only:
+            # the headline's inner stencil: k right-hand sides, FP16 12-real
+            pc = nat.PREC[args.inner]
+            fi, fo = nat.Field(ctx, pc, k), nat.Field(ctx, pc, k)
+            fi.import_(bk_t)
+            gauge.prepare(pc, args.recon)
+            ctx.set_tuning(pc, True, 1, 2, True)
+            us_k = time_dslash(ctx, gauge, pc, args.recon, fi, fo, k, args.dslash_reps, stream)
+            dslash[f"{args.inner}_r{args.recon}_k{k}"] = {"us": round(us_k, 2)}
+            del fi, fo
             src = torch.empty(32 << 20, dtype=torch.float64, device="cuda")
             dst = torch.empty_like(src)
             for _ in range(5):
@@ -186,19 +198,15 @@ def main():
             fi, fo = nat.Field(ctx, pc, k), nat.Field(ctx, pc, k)
             fi.import_(bk_t)
             gauge.prepare(pc, recon)
-            best = None
-            tried = {}
-            for kk in (1, 2, 4):
-                ctx.set_mrhs(pc, kk)
-                us = time_dslash(ctx, gauge, pc, recon, fi, fo, k, max(5, args.dslash_reps // 5), stream)
-                tried[str(kk)] = round(us, 2)
-                if best is None or us < best[1]:
-                    best = (kk, us)
-            ctx.set_mrhs(pc, best[0])
+            tk = tune(ctx, gauge, pc, recon, fi, fo, nrhs=k, reps=max(3, args.dslash_reps // 10), stream=stream)
+            us = time_dslash(ctx, gauge, pc, recon, fi, fo, k, max(5, args.dslash_reps // 5), stream)
+            best = (tk["rhs_per_thread"], us)
+            tried = {m: round(t, 2) for m, t in tk["us"].items()}
             bps = dslash_bytes_per_site(prec, recon, k)
             gbs = bps * V / (best[1] * 1e-6) / 1e9
             dslash_k[f"{prec}_r{recon}"] = {
-                "nrhs": k, "rhs_per_thread": best[0], "us": round(best[1], 2), "us_per_rhs": round(best[1] / k, 2),
+                "nrhs": k, "rhs_per_thread": best[0], "march": tk["march"], "xcd_remap": tk["xcd_remap"],
+                "us": round(best[1], 2), "us_per_rhs": round(best[1] / k, 2),
                 "GB/s": round(gbs, 1), "frac_hbm_peak": round(gbs / PEAK_HBM_GBS, 4), "bytes_per_site": bps,
                 "tune_us": tried,
             }
@@ -259,7 +267,7 @@ def main():
         dom_us = time_dslash(ctx, gauge, pc, args.recon, fi, fo, k, args.dslash_reps, stream)
         dom_bytes = dslash_bytes_per_site(args.inner, args.recon, k) * V
         dom_gbs = dom_bytes / (dom_us * 1e-6) / 1e9
-        rhs_per_thread = ctx.get_mrhs(pc, k)
+        dom_tuning = ctx.get_tuning(pc, k)
         del fi, fo, bk, xk, b1, x1, x1m
         torch.cuda.synchronize()
 
@@ -280,12 +288,12 @@ def main():
             "workload": "SU(3) Wilson D^dag D CG, %s, m=%g, bc=(1,1,1,-1), %d RHS per solve, FP64 true residual %g"
                         % ("x".join(map(str, L)), args.mass, k, args.tol),
             "lattice": list(L), "nrhs": k, "inner": args.inner, "recon": args.recon, "delta": args.delta,
+            "accumulator": args.accum,
             "graph": bool(args.graph),
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": f"k_dslash_mrhs {args.inner} r{args.recon}, {k} RHS, {rhs_per_thread} per thread "
-                      "(links counted once)",
+            "kernel": f"stencil {args.inner} r{args.recon}, {k} RHS, tuning {dom_tuning} (links counted once)",
             "achieved": round(dom_gbs, 1),
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",

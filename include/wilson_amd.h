@@ -107,7 +107,13 @@ typedef struct {
   int poll_every;   /* iterations queued between host polls of the convergence flags */
   int use_graph;    /* capture the poll_every-iteration body as a HIP graph */
   const double* mass; /* host array of per-column masses, or NULL for the context mass */
+  int accum_prec;     /* WL_ACCUM_*: storage of the inner solution accumulator between reliable updates */
 } wl_solve_params;
+
+#define WL_ACCUM_AUTO 0 /* FP32 for a mixed-precision solve */
+#define WL_ACCUM_F64 1
+#define WL_ACCUM_F32 2
+#define WL_ACCUM_F16 3
 
 typedef struct {
   int iters;            /* CG iterations performed */
@@ -124,23 +130,19 @@ const char* wl_last_error(void);
 int wl_ctx_create(const wl_geom* geom, const wl_params* params, wl_ctx** out);
 void wl_ctx_destroy(wl_ctx* ctx);
 int wl_ctx_set_mass(wl_ctx* ctx, double mass);
-/* Tuning knob (no counterpart in the reference): number of x0 sites each GPU
- * thread marches over in the stencil kernels of precision prec (0 =
- * automatic; otherwise it must divide L0).  wl_ctx_get_march returns the
- * value in effect. */
-int wl_ctx_set_march(wl_ctx* ctx, int prec, int march);
-int wl_ctx_get_march(wl_ctx* ctx, int prec);
-/* Tuning knob (no counterpart in the reference): right-hand sides handled per
- * GPU thread by the multi-RHS stencil (each link load is shared by them):
- * 1, 2 or 4, or 0 = automatic.  It applies when it divides the number of
- * right-hand sides per configuration; wl_ctx_get_mrhs returns the value used
- * for nrhs right-hand sides. */
-int wl_ctx_set_mrhs(wl_ctx* ctx, int prec, int k);
-/* Tuning knob (no counterpart in the reference): 1 (default) maps the stencil
- * workgroups of precision prec so that each XCD walks one contiguous range of
- * lattice tiles; 0 keeps the hardware dispatch order. */
-int wl_ctx_set_xcd_remap(wl_ctx* ctx, int prec, int on);
-int wl_ctx_get_mrhs(wl_ctx* ctx, int prec, int nrhs);
+/* Stencil launch tuning (no counterpart in the reference), per storage
+ * precision and separately for single-RHS (multi_rhs = 0) and multi-RHS
+ * (multi_rhs = 1) launches:
+ *   march           x0 sites each GPU thread marches over (0 = automatic; must
+ *                   divide L0; 1 = one site per thread),
+ *   rhs_per_thread  right-hand sides sharing each link load in a thread (multi-RHS
+ *                   only; 0 = automatic, 1, 2 or 4; used when it divides the
+ *                   right-hand sides per configuration),
+ *   xcd_remap       1: each XCD walks one contiguous range of lattice tiles,
+ *                   0: hardware workgroup order.
+ * wl_ctx_get_tuning reports the values in effect for nrhs right-hand sides. */
+int wl_ctx_set_tuning(wl_ctx* ctx, int prec, int multi_rhs, int march, int rhs_per_thread, int xcd_remap);
+int wl_ctx_get_tuning(wl_ctx* ctx, int prec, int nrhs, int* march, int* rhs_per_thread, int* xcd_remap);
 
 /* where: 0 = device pointer, 1 = host pointer */
 int wl_gauge_upload(wl_ctx* ctx, const void* U, int layout, int dtype, int where, wl_gauge** out);

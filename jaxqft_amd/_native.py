@@ -24,6 +24,8 @@ WL_C128, WL_C64 = 0, 1
 WL_LAYOUT_BMXYIJ, WL_LAYOUT_BXYMIJ = 0, 1
 WL_OP_D, WL_OP_DDAG, WL_OP_G5D, WL_OP_K, WL_OP_KDAG, WL_OP_DDAGD = 0, 1, 2, 3, 4, 5
 WL_STOP_TRUE_RESIDUAL, WL_STOP_ITERATED = 0, 1
+WL_ACCUM_AUTO, WL_ACCUM_F64, WL_ACCUM_F32, WL_ACCUM_F16 = 0, 1, 2, 3
+ACCUM = {None: WL_ACCUM_AUTO, "auto": WL_ACCUM_AUTO, "f64": WL_ACCUM_F64, "f32": WL_ACCUM_F32, "f16": WL_ACCUM_F16}
 STATUS_NAMES = {-1: "running", 0: "converged", 1: "maxiter", 2: "nan", 3: "breakdown"}
 PREC = {"f64": WL_F64, "f32": WL_F32, "f16": WL_F16}
 
@@ -47,6 +49,7 @@ class WlSolveParams(ctypes.Structure):
         ("poll_every", c_int),
         ("use_graph", c_int),
         ("mass", POINTER(c_double)),
+        ("accum_prec", c_int),
     ]
 
 
@@ -94,11 +97,8 @@ def load() -> ctypes.CDLL:
         "wl_ctx_create": (c_int, [POINTER(WlGeom), POINTER(WlParams), POINTER(vp)]),
         "wl_ctx_destroy": (None, [vp]),
         "wl_ctx_set_mass": (c_int, [vp, c_double]),
-        "wl_ctx_set_march": (c_int, [vp, c_int, c_int]),
-        "wl_ctx_get_march": (c_int, [vp, c_int]),
-        "wl_ctx_set_mrhs": (c_int, [vp, c_int, c_int]),
-        "wl_ctx_set_xcd_remap": (c_int, [vp, c_int, c_int]),
-        "wl_ctx_get_mrhs": (c_int, [vp, c_int, c_int]),
+        "wl_ctx_set_tuning": (c_int, [vp, c_int, c_int, c_int, c_int, c_int]),
+        "wl_ctx_get_tuning": (c_int, [vp, c_int, c_int, POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
         "wl_gauge_upload": (c_int, [vp, vp, c_int, c_int, c_int, POINTER(vp)]),
         "wl_gauge_release": (None, [vp]),
         "wl_gauge_prepare": (c_int, [vp, c_int, c_int, vp]),
@@ -168,20 +168,23 @@ class Context:
         self.volume = L[0] * L[1] * L[2] * L[3]
         self._lib = lib
 
+    def set_tuning(self, prec: int, multi_rhs: bool, march: int = 0, rhs_per_thread: int = 0,
+                   xcd_remap: bool = True) -> None:
+        check(self._lib.wl_ctx_set_tuning(self.h, int(prec), 1 if multi_rhs else 0, int(march), int(rhs_per_thread),
+                                          1 if xcd_remap else 0))
+
+    def get_tuning(self, prec: int, nrhs: int = 1):
+        m, k, x = c_int(), c_int(), c_int()
+        check(self._lib.wl_ctx_get_tuning(self.h, int(prec), int(nrhs), byref(m), byref(k), byref(x)))
+        return {"march": m.value, "rhs_per_thread": k.value, "xcd_remap": bool(x.value)}
+
+    # single-RHS shortcuts
     def set_march(self, prec: int, march: int) -> None:
-        check(self._lib.wl_ctx_set_march(self.h, int(prec), int(march)))
+        cur = self.get_tuning(prec, 1)
+        self.set_tuning(prec, False, march, 0, cur["xcd_remap"])
 
     def get_march(self, prec: int) -> int:
-        return int(self._lib.wl_ctx_get_march(self.h, int(prec)))
-
-    def set_mrhs(self, prec: int, k: int) -> None:
-        check(self._lib.wl_ctx_set_mrhs(self.h, int(prec), int(k)))
-
-    def get_mrhs(self, prec: int, nrhs: int) -> int:
-        return int(self._lib.wl_ctx_get_mrhs(self.h, int(prec), int(nrhs)))
-
-    def set_xcd_remap(self, prec: int, on: bool) -> None:
-        check(self._lib.wl_ctx_set_xcd_remap(self.h, int(prec), 1 if on else 0))
+        return self.get_tuning(prec, 1)["march"]
 
     def set_mass(self, mass: float) -> None:
         check(self._lib.wl_ctx_set_mass(self.h, float(mass)))
@@ -259,8 +262,9 @@ class Field:
 
 def solve_params(tol: float, maxiter: int, stop_rule: int = WL_STOP_TRUE_RESIDUAL, inner_prec: int = WL_F64,
                  inner_recon: int = 18, delta: float = 0.1, poll_every: int = 8, use_graph: bool = False,
-                 masses=None):
+                 masses=None, accum: str = "auto"):
     sp = WlSolveParams()
+    sp.accum_prec = ACCUM[accum]
     sp.tol = float(tol)
     sp.maxiter = int(maxiter)
     sp.stop_rule = int(stop_rule)

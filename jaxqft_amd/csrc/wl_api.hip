@@ -105,9 +105,10 @@ struct wl_ctx {
   DevBuf ws[10];
   ColState* host_cols = nullptr;  // pinned mirror
   size_t host_cols_n = 0;
-  int march[3] = {0, 0, 0};       // x0 sites per thread per precision (0 = automatic)
-  int mrhs[3] = {0, 0, 0};        // right-hand sides per thread per precision (0 = automatic)
-  int xcd_remap[3] = {1, 1, 1};   // XCD-contiguous workgroup order per precision
+  // stencil launch tuning per [single-RHS / multi-RHS launch][precision]
+  int march[2][3] = {{0, 0, 0}, {0, 0, 0}};      // x0 sites per thread (0 = automatic)
+  int mrhs[3] = {0, 0, 0};                       // right-hand sides per thread (0 = automatic)
+  int xcd_remap[2][3] = {{1, 1, 1}, {1, 1, 1}};  // XCD-contiguous workgroup order
 };
 
 struct wl_gauge {
@@ -139,8 +140,8 @@ static inline int nchunk_of(const wl_ctx* c) { return (int)((c->g.stride[0] + NT
 
 // x0 sites marched per thread: the largest of 4, 2, 1 dividing L0 that still
 // leaves at least 512 workgroups per spinor column.
-static int march_for(const wl_ctx* c, int prec) {
-  int m = c->march[prec];
+static int march_for(const wl_ctx* c, int prec, int nrhs) {
+  int m = c->march[nrhs > 1 ? 1 : 0][prec];
   if (m > 0) return m;
   const int L0 = c->geom.L[0];
   for (int cand : {4, 2}) {
@@ -225,41 +226,29 @@ int wl_ctx_set_mass(wl_ctx* c, double mass) {
   return WL_OK;
 }
 
-int wl_ctx_set_march(wl_ctx* c, int prec, int march) {
+int wl_ctx_set_tuning(wl_ctx* c, int prec, int multi_rhs, int march, int rhs_per_thread, int xcd_remap) {
   CHK(check_ctx(c));
   if (prec < F64 || prec > F16) return fail(WL_ERR_BAD_ARG, "bad precision %d", prec);
   if (march < 0 || (march > 0 && c->geom.L[0] % march != 0))
     return fail(WL_ERR_BAD_ARG, "march %d must be 0 (automatic) or divide L0=%d", march, c->geom.L[0]);
+  if (rhs_per_thread != 0 && rhs_per_thread != 1 && rhs_per_thread != 2 && rhs_per_thread != 4)
+    return fail(WL_ERR_BAD_ARG, "rhs_per_thread must be 0 (automatic), 1, 2 or 4");
   std::lock_guard<std::mutex> lk(c->mu);
-  c->march[prec] = march;
+  const int m = multi_rhs ? 1 : 0;
+  c->march[m][prec] = march;
+  c->xcd_remap[m][prec] = xcd_remap ? 1 : 0;
+  if (multi_rhs) c->mrhs[prec] = rhs_per_thread;
   return WL_OK;
 }
 
-int wl_ctx_set_mrhs(wl_ctx* c, int prec, int k) {
+int wl_ctx_get_tuning(wl_ctx* c, int prec, int nrhs, int* march, int* rhs_per_thread, int* xcd_remap) {
   CHK(check_ctx(c));
-  if (prec < F64 || prec > F16) return fail(WL_ERR_BAD_ARG, "bad precision %d", prec);
-  if (k != 0 && k != 1 && k != 2 && k != 4) return fail(WL_ERR_BAD_ARG, "mrhs must be 0 (automatic), 1, 2 or 4");
-  std::lock_guard<std::mutex> lk(c->mu);
-  c->mrhs[prec] = k;
+  if (prec < F64 || prec > F16 || nrhs < 1) return fail(WL_ERR_BAD_ARG, "bad precision or nrhs");
+  const int k = mrhs_for(c, prec, nrhs);
+  if (march) *march = k > 1 ? 1 : march_for(c, prec, nrhs);
+  if (rhs_per_thread) *rhs_per_thread = k;
+  if (xcd_remap) *xcd_remap = c->xcd_remap[nrhs > 1 ? 1 : 0][prec];
   return WL_OK;
-}
-
-int wl_ctx_get_mrhs(wl_ctx* c, int prec, int nrhs) {
-  if (!c || prec < F64 || prec > F16 || nrhs < 1) return 0;
-  return mrhs_for(c, prec, nrhs);
-}
-
-int wl_ctx_set_xcd_remap(wl_ctx* c, int prec, int on) {
-  CHK(check_ctx(c));
-  if (prec < F64 || prec > F16) return fail(WL_ERR_BAD_ARG, "bad precision %d", prec);
-  std::lock_guard<std::mutex> lk(c->mu);
-  c->xcd_remap[prec] = on ? 1 : 0;
-  return WL_OK;
-}
-
-int wl_ctx_get_march(wl_ctx* c, int prec) {
-  if (!c || prec < F64 || prec > F16) return 0;
-  return march_for(c, prec);
 }
 
 int wl_gauge_upload(wl_ctx* c, const void* U, int layout, int dtype, int where, wl_gauge** out) {
@@ -351,7 +340,8 @@ static int gauge_encoding(wl_gauge* g, int prec, int recon, hipStream_t st, cons
 // launch geometry once ncol and cols_per_cfg are set
 static void set_launch_shape(const wl_ctx* c, DslashArgs& a, int prec) {
   a.mrhs = mrhs_for(c, prec, a.cols_per_cfg);
-  if (a.mrhs > 1) a.march = 1;
+  a.march = a.mrhs > 1 ? 1 : march_for(c, prec, a.cols_per_cfg);
+  a.xcd_remap = c->xcd_remap[a.cols_per_cfg > 1 ? 1 : 0][prec];
   a.nseg = c->geom.L[0] / a.march;
   a.ntiles = a.march == 1 ? (int)((c->V + NT - 1) / NT) : nchunk_of(c) * a.nseg;
 }
@@ -365,11 +355,9 @@ static void fill_dslash_common(const wl_ctx* c, DslashArgs& a, int prec) {
   }
   a.m0 = c->params.mass + 4.0 * c->params.wilson_r;
   a.r = c->params.wilson_r;
-  a.march = march_for(c, prec);
-  a.xcd_remap = c->xcd_remap[prec];
-  a.nseg = c->geom.L[0] / a.march;
-  // march 1 runs the one-site-per-thread kernel over flat tiles of NT sites
-  a.ntiles = a.march == 1 ? (int)((c->V + NT - 1) / NT) : nchunk_of(c) * a.nseg;
+  a.march = 1;  // set_launch_shape fills the launch shape once ncol is known
+  a.nseg = c->geom.L[0];
+  a.ntiles = (int)((c->V + NT - 1) / NT);
 }
 
 // op on internal fields of precision prec (no gating, no reduction)
@@ -634,6 +622,16 @@ struct Solver {
 
 }  // namespace
 
+// storage of the inner solution accumulator y (x itself for a plain FP64 solve)
+static int accum_prec_of(const wl_solve_params* sp, int inner) {
+  if (inner == F64) return F64;
+  switch (sp->accum_prec) {
+    case WL_ACCUM_F64: return F64;
+    case WL_ACCUM_F16: return F16;
+    default: return F32;
+  }
+}
+
 static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int nrhs, const wl_solve_params* sp,
                      wl_solve_stats* stats, hipStream_t st) {
   const int nc = c->geom.nc;
@@ -654,7 +652,7 @@ static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int n
   s.R = (nc == 1) ? 18 : (sp->inner_recon == 12 ? 12 : 18);
   s.mixed = s.P != F64;
   s.fused = s.mixed;
-  s.PY = s.mixed ? F32 : F64;
+  s.PY = accum_prec_of(sp, s.P);
   s.cb64 = col_bytes_of(c, F64);
   s.cbP = col_bytes_of(c, s.P);
   s.cbY = col_bytes_of(c, s.PY);
@@ -920,7 +918,7 @@ struct wl_batch {
 static size_t batch_bytes_per_col(const wl_ctx* c, int prec) {
   const bool mixed = prec != F64;
   size_t b = 4 * col_bytes_of(c, F64) + 3 * col_bytes_of(c, prec);
-  if (mixed) b += col_bytes_of(c, prec) + col_bytes_of(c, F32);
+  if (mixed) b += col_bytes_of(c, prec) + col_bytes_of(c, F64);  // r and an accumulator of at most FP64
   return b + sizeof(double) * max_tiles(c) + sizeof(ColState);
 }
 
@@ -973,7 +971,7 @@ int wl_batch_create(wl_ctx* c, wl_gauge* g, int capacity, const wl_solve_params*
   s.R = (nc == 1) ? 18 : (sp->inner_recon == 12 ? 12 : 18);
   s.mixed = s.P != F64;
   s.fused = true;
-  s.PY = s.mixed ? F32 : F64;
+  s.PY = accum_prec_of(sp, s.P);
   s.cb64 = col_bytes_of(c, F64);
   s.cbP = col_bytes_of(c, s.P);
   s.cbY = col_bytes_of(c, s.PY);

@@ -202,6 +202,8 @@ static hipError_t cg_update_main(int py, int p, int nc, BlasArgs a, hipStream_t 
   if (py == F32 && p == F16) return WL_DISPATCH_NC(k_cg_update, F32, F16);
   if (py == F64 && p == F32) return WL_DISPATCH_NC(k_cg_update, F64, F32);
   if (py == F64 && p == F16) return WL_DISPATCH_NC(k_cg_update, F64, F16);
+  if (py == F16 && p == F16) return WL_DISPATCH_NC(k_cg_update, F16, F16);
+  if (py == F16 && p == F32) return WL_DISPATCH_NC(k_cg_update, F16, F32);
   return hipErrorInvalidValue;
 }
 
@@ -222,6 +224,7 @@ hipError_t launch_accumulate(int px, int py, int nc, BlasArgs a, hipStream_t str
   dim3 grid = blas_grid(a), block(NT);
   if (px == F64 && py == F32) return WL_DISPATCH_NC(k_accumulate, F64, F32);
   if (px == F64 && py == F64) return WL_DISPATCH_NC(k_accumulate, F64, F64);
+  if (px == F64 && py == F16) return WL_DISPATCH_NC(k_accumulate, F64, F16);
   return hipErrorInvalidValue;
 }
 

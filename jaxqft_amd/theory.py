@@ -88,6 +88,7 @@ class WilsonNf2Theory:
     solver_stop: str = "true"       # "true": FP64 true residual <= cg_tol; "iterated": reference rule
     poll_every: int = 8
     use_graph: bool = False
+    solver_accum: str = "auto"      # inner solution accumulator storage: auto (f32) | f64 | f32 | f16
     Nc: int = field(default=3, init=False)
     last_solve_stats: List[Dict[str, Any]] = field(default_factory=list, init=False, repr=False)
 
@@ -122,6 +123,8 @@ class WilsonNf2Theory:
             raise ValueError("solver_recon must be 18 or 12")
         if self.solver_stop not in ("true", "iterated"):
             raise ValueError("solver_stop must be 'true' or 'iterated'")
+        if self.solver_accum not in nat.ACCUM:
+            raise ValueError("solver_accum must be one of: auto, f64, f32, f16")
         self.fermion_bc = parse_fermion_bc(self.fermion_bc, self.Nd, self._default_antiperiodic_t)
         self.dirac = WilsonDiracOperator(ndim=self.Nd, mass=self.mass, wilson_r=self.wilson_r, dtype=self.dtype)
         self.gamma = self.dirac.gamma
@@ -224,6 +227,7 @@ class WilsonNf2Theory:
             poll_every=self.poll_every,
             use_graph=self.use_graph,
             masses=masses,
+            accum=self.solver_accum,
         )
         pol.update(overrides)
         return nat.solve_params(**pol)

@@ -229,8 +229,8 @@ def test_multi_rhs_stencil(prec, recon, tol, per_thread):
     psi = fields.random_fermion(rng, (B * k, *L, 4, 3))
     ctx = nat.Context(L, 3, B, 0.04, 1.0, bc)
     pc = nat.PREC[prec]
-    ctx.set_mrhs(pc, per_thread)
-    assert ctx.get_mrhs(pc, k) == per_thread
+    ctx.set_tuning(pc, True, 1, per_thread, True)
+    assert ctx.get_tuning(pc, k)["rhs_per_thread"] == per_thread
     g = nat.Gauge(ctx, torch.from_numpy(U).cuda(), nat.WL_LAYOUT_BMXYIJ)
     fi, fo = nat.Field(ctx, pc, B * k), nat.Field(ctx, pc, B * k)
     fi.import_(torch.from_numpy(psi).cuda())
