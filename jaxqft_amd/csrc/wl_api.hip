@@ -19,7 +19,6 @@
 namespace wl {
 hipError_t launch_dslash(int prec, int nc, int recon, int op, int epi, const DslashArgs& a, hipStream_t stream);
 hipError_t launch_norm2(int p, int nc, BlasArgs a, hipStream_t stream);
-hipError_t launch_cg_update(int py, int p, int nc, BlasArgs a, hipStream_t stream);
 hipError_t launch_xpay(int p, int nc, BlasArgs a, hipStream_t stream);
 hipError_t launch_accumulate(int px, int py, int nc, BlasArgs a, hipStream_t stream);
 hipError_t launch_convert(int po, int pi, int nc, BlasArgs a, hipStream_t stream);
@@ -518,7 +517,7 @@ struct Solver {
   const void* UP;
   void *b, *x;    // F64 caller-side internal fields
   void *rhi, *thi;
-  void *r, *p, *Ap, *t, *y;
+  void *r, *p, *t, *y;
   Reduction red;
 
   void dargs(DslashArgs& a, int prec, const void* U, int recon) {
@@ -558,11 +557,11 @@ struct Solver {
     dargs(d, F64, U64, 18);
     d.gate = 2;
     d.in = x; d.out = thi; d.col_bytes = cb64;
-    e = launch_dslash(F64, nc, 18, WL_OP_D, 0, d, st);
+    e = launch_dslash(F64, nc, 18, WL_OP_D, EPI_STORE, d, st);
     if (e != hipSuccess) return fail(WL_ERR_HIP, "dslash: %s", hipGetErrorString(e));
     d.in = thi; d.out = rhi; d.w = b;
     d.red.action = RA_TRUERES;
-    e = launch_dslash(F64, nc, 18, WL_OP_DDAG, 2, d, st);
+    e = launch_dslash(F64, nc, 18, WL_OP_DDAG, EPI_RESID, d, st);
     if (e != hipSuccess) return fail(WL_ERR_HIP, "dslash: %s", hipGetErrorString(e));
     if (mixed) {
       BlasArgs a = bargs(2);
@@ -583,28 +582,24 @@ struct Solver {
     return WL_OK;
   }
 
+  // One CG iteration on A = D^dag D, two stencil launches with fused vector
+  // work: t = D p with |t|^2 = <p, A p> (alpha), then A p = D^dag t consumed
+  // in the epilogue by y += alpha p, r -= alpha A p and |r|^2 (beta); then
+  // p = r + beta p.
   int iteration() {
     hipError_t e;
     DslashArgs d;
     dargs(d, P, UP, R);
     d.gate = 1;
     d.in = p; d.out = t; d.col_bytes = cbP;
-    e = launch_dslash(P, nc, nc == 1 ? 18 : R, WL_OP_D, 0, d, st);
-    if (e != hipSuccess) return fail(WL_ERR_HIP, "dslash: %s", hipGetErrorString(e));
-    d.in = t; d.out = Ap; d.w = p;
     d.red.action = RA_PAP;
-    e = launch_dslash(P, nc, nc == 1 ? 18 : R, WL_OP_DDAG, 1, d, st);
+    e = launch_dslash(P, nc, nc == 1 ? 18 : R, WL_OP_D, EPI_NORM, d, st);
     if (e != hipSuccess) return fail(WL_ERR_HIP, "dslash: %s", hipGetErrorString(e));
-    {
-      BlasArgs a = bargs(1);
-      a.red.action = RA_ITER;
-      a.a = y; a.col_bytes_a = cbY;
-      a.b = p; a.col_bytes_b = cbP;
-      a.c = r; a.col_bytes_c = cbP;
-      a.d = Ap; a.col_bytes_d = cbP;
-      e = launch_cg_update(PY, P, nc, a, st);
-      if (e != hipSuccess) return fail(WL_ERR_HIP, "cg update: %s", hipGetErrorString(e));
-    }
+    d.in = t; d.out = nullptr; d.w = p; d.res = r; d.y = y; d.col_bytes_y = cbY;
+    d.red.action = RA_ITER;
+    const int epi = PY == F64 ? EPI_CG64 : (PY == F32 ? EPI_CG32 : EPI_CG16);
+    e = launch_dslash(P, nc, nc == 1 ? 18 : R, WL_OP_DDAG, epi, d, st);
+    if (e != hipSuccess) return fail(WL_ERR_HIP, "dslash: %s", hipGetErrorString(e));
     {
       BlasArgs a = bargs(1);
       a.a = p; a.col_bytes_a = cbP;
@@ -664,12 +659,10 @@ static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int n
   CHK(c->ws[0].ensure(s.cb64 * ncol));  // r_hi
   CHK(c->ws[1].ensure(s.cb64 * ncol));  // t_hi
   CHK(c->ws[2].ensure(s.cbP * ncol));   // p
-  CHK(c->ws[3].ensure(s.cbP * ncol));   // Ap
   CHK(c->ws[4].ensure(s.cbP * ncol));   // t
   s.rhi = c->ws[0].p;
   s.thi = c->ws[1].p;
   s.p = c->ws[2].p;
-  s.Ap = c->ws[3].p;
   s.t = c->ws[4].p;
   if (s.mixed) {
     CHK(c->ws[5].ensure(s.cbP * ncol));  // r
@@ -917,7 +910,7 @@ struct wl_batch {
 
 static size_t batch_bytes_per_col(const wl_ctx* c, int prec) {
   const bool mixed = prec != F64;
-  size_t b = 4 * col_bytes_of(c, F64) + 3 * col_bytes_of(c, prec);
+  size_t b = 4 * col_bytes_of(c, F64) + 2 * col_bytes_of(c, prec);  // b x r_hi t_hi; p t
   if (mixed) b += col_bytes_of(c, prec) + col_bytes_of(c, F64);  // r and an accumulator of at most FP64
   return b + sizeof(double) * max_tiles(c) + sizeof(ColState);
 }
@@ -985,7 +978,6 @@ int wl_batch_create(wl_ctx* c, wl_gauge* g, int capacity, const wl_solve_params*
   s.rhi = alloc(2, s.cb64 * capacity);
   s.thi = alloc(3, s.cb64 * capacity);
   s.p = alloc(4, s.cbP * capacity);
-  s.Ap = alloc(5, s.cbP * capacity);
   s.t = alloc(6, s.cbP * capacity);
   if (s.mixed) {
     s.r = alloc(7, s.cbP * capacity);

@@ -48,37 +48,6 @@ __global__ __launch_bounds__(NT) void k_norm2(BlasArgs a) {
   write_partial(a.red, col, tile, a.ntiles, valid ? part : 0.0);
 }
 
-// y += alpha p ; r -= alpha Ap ; |r|^2 -> RA_ITER.   a=y (PY), b=p, c=r, d=Ap (P)
-template <int PY, int P, int NC>
-__global__ __launch_bounds__(NT) void k_cg_update(BlasArgs a) {
-  WL_BLAS_PROLOGUE
-  using T = typename PT<P>::acc;
-  using TY = typename PT<PY>::acc;
-  const double alpha = a.red.cols[col].alpha;
-  cx<T> p[4 * NC], r[4 * NC], ap[4 * NC];
-  cx<TY> y[4 * NC];
-  load_as<PY, NC>((char*)a.a + col * a.col_bytes_a, V, s, y);
-  load_as<P, NC>((char*)a.b + col * a.col_bytes_b, V, s, p);
-  load_as<P, NC>((char*)a.c + col * a.col_bytes_c, V, s, r);
-  load_as<P, NC>((char*)a.d + col * a.col_bytes_d, V, s, ap);
-  const TY ay = (TY)alpha;
-  const T ar = (T)alpha;
-  double part = 0.0;
-#pragma unroll
-  for (int c = 0; c < 4 * NC; ++c) {
-    y[c].re += ay * (TY)p[c].re;
-    y[c].im += ay * (TY)p[c].im;
-    r[c].re -= ar * ap[c].re;
-    r[c].im -= ar * ap[c].im;
-    part += (double)r[c].re * r[c].re + (double)r[c].im * r[c].im;
-  }
-  if (valid) {
-    store_as<PY, NC>((char*)a.a + col * a.col_bytes_a, V, s, y);
-    store_as<P, NC>((char*)a.c + col * a.col_bytes_c, V, s, r);
-  }
-  write_partial(a.red, col, tile, a.ntiles, valid ? part : 0.0);
-}
-
 // p = r + beta p.   a=p, b=r
 template <int P, int NC>
 __global__ __launch_bounds__(NT) void k_xpay(BlasArgs a) {
@@ -191,24 +160,6 @@ static hipError_t norm2_main(int p, int nc, BlasArgs a, hipStream_t stream) {
 
 hipError_t launch_norm2(int p, int nc, BlasArgs a, hipStream_t stream) {
   hipError_t e = norm2_main(p, nc, a, stream);
-  if (e != hipSuccess) return e;
-  return launch_reduce_finish(a.red, a.ncol, a.ntiles, a.gate, stream);
-}
-
-static hipError_t cg_update_main(int py, int p, int nc, BlasArgs a, hipStream_t stream) {
-  dim3 grid = blas_grid(a), block(NT);
-  if (py == F64 && p == F64) return WL_DISPATCH_NC(k_cg_update, F64, F64);
-  if (py == F32 && p == F32) return WL_DISPATCH_NC(k_cg_update, F32, F32);
-  if (py == F32 && p == F16) return WL_DISPATCH_NC(k_cg_update, F32, F16);
-  if (py == F64 && p == F32) return WL_DISPATCH_NC(k_cg_update, F64, F32);
-  if (py == F64 && p == F16) return WL_DISPATCH_NC(k_cg_update, F64, F16);
-  if (py == F16 && p == F16) return WL_DISPATCH_NC(k_cg_update, F16, F16);
-  if (py == F16 && p == F32) return WL_DISPATCH_NC(k_cg_update, F16, F32);
-  return hipErrorInvalidValue;
-}
-
-hipError_t launch_cg_update(int py, int p, int nc, BlasArgs a, hipStream_t stream) {
-  hipError_t e = cg_update_main(py, p, nc, a, stream);
   if (e != hipSuccess) return e;
   return launch_reduce_finish(a.red, a.ncol, a.ntiles, a.gate, stream);
 }

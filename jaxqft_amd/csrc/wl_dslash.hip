@@ -1,7 +1,8 @@
 // Wilson operator dispatch: D, D^dag, gamma5 D, the hopping term K and K^dag,
-// with optional fused reductions for CG (Re<p, A p> and |b - A x|^2).  The
-// kernels are in wl_dslash_kernels.h, instantiated per storage precision in
-// wl_dslash_{f64,f32,f16}.hip.
+// with the epilogues of the fused CG iteration (Epilogue in wl_kernels.h).
+// The kernels are in wl_dslash_kernels.h, instantiated per storage precision
+// in wl_dslash_{f64,f32,f16}.hip; a reducing epilogue is followed by the
+// per-column finish kernel.
 #include "wl_device.h"
 #include "wl_kernels.h"
 
@@ -18,7 +19,7 @@ hipError_t launch_dslash(int prec, int nc, int recon, int op, int epi, const Dsl
   else if (prec == F32) e = launch_dslash_f32(nc, recon, op, epi, a, stream);
   else if (prec == F16) e = launch_dslash_f16(nc, recon, op, epi, a, stream);
   else return hipErrorInvalidValue;
-  if (e != hipSuccess || epi == 0) return e;
+  if (e != hipSuccess || epi == EPI_STORE) return e;
   return launch_reduce_finish(a.red, a.ncol, a.ntiles, a.gate, stream);
 }
 

@@ -1,5 +1,9 @@
 // Wilson operator kernels: D, D^dag, gamma5 D, the hopping term K and K^dag,
-// with optional fused reductions for CG (Re<p, A p> and |b - A x|^2).
+// with epilogues that fuse the CG vector work into the stencil (Epilogue in
+// wl_kernels.h).  One CG iteration on A = D^dag D is two stencil launches:
+//   t = D p         with |t|^2 = <p, A p>          -> alpha = rr / |t|^2
+//   A p = D^dag t   with y += alpha p, r -= alpha A p, |r|^2 -> beta
+// so A p is never written to memory.
 //
 // The 1D grid is ntiles x ncol workgroups ordered spinor-column-fastest and
 // remapped so that each XCD walks one contiguous range of tiles: the k
@@ -30,8 +34,68 @@ __device__ __forceinline__ void store_nt(void* base, size_t V, size_t site, cons
   SpinorIO<P, NC>::store(base, V, site, v);
 }
 
+template <int EPI>
+struct EpiAccum {
+  static constexpr int prec = EPI == EPI_CG64 ? F64 : (EPI == EPI_CG32 ? F32 : F16);
+};
+
+// Epilogue at site s of column col: res holds the operator result (modified
+// in place for EPI_RESID).  Stores what the epilogue stores (only when
+// `valid`) and returns the site's contribution to the column's reduction
+// partial (0 for an invalid thread).
+template <int P, int NC, int EPI>
+__device__ __forceinline__ double site_epilogue(const DslashArgs& a, int col, size_t V, size_t s, bool valid,
+                                                cx<typename PT<P>::acc> (&res)[4 * NC]) {
+  using T = typename PT<P>::acc;
+  char* outb = reinterpret_cast<char*>(a.out) + (size_t)col * a.col_bytes;
+  double part = 0.0;
+  if (EPI == EPI_STORE || EPI == EPI_NORM) {
+    if (EPI == EPI_NORM) {
+#pragma unroll
+      for (int q = 0; q < 4 * NC; ++q) part += (double)res[q].re * (double)res[q].re + (double)res[q].im * (double)res[q].im;
+    }
+    if (valid) store_nt<P, NC>(outb, V, s, res);
+  } else if (EPI == EPI_RESID) {
+    const char* wb = reinterpret_cast<const char*>(a.w) + (size_t)col * a.col_bytes;
+    cx<T> w[4 * NC];
+    SpinorIO<P, NC>::load(wb, V, s, w);
+#pragma unroll
+    for (int q = 0; q < 4 * NC; ++q) {
+      res[q] = csub(w[q], res[q]);
+      part += (double)res[q].re * (double)res[q].re + (double)res[q].im * (double)res[q].im;
+    }
+    if (valid) store_nt<P, NC>(outb, V, s, res);
+  } else {
+    constexpr int PY = EpiAccum<EPI>::prec;
+    using TY = typename PT<PY>::acc;
+    const double alpha = a.red.cols[col].alpha;
+    const char* pb = reinterpret_cast<const char*>(a.w) + (size_t)col * a.col_bytes;
+    char* rb = reinterpret_cast<char*>(a.res) + (size_t)col * a.col_bytes;
+    char* yb = reinterpret_cast<char*>(a.y) + (size_t)col * a.col_bytes_y;
+    cx<T> p[4 * NC], r[4 * NC];
+    cx<TY> y[4 * NC];
+    SpinorIO<P, NC>::load(pb, V, s, p);
+    SpinorIO<P, NC>::load(rb, V, s, r);
+    SpinorIO<PY, NC>::load(yb, V, s, y);
+    const T ar = (T)alpha;
+    const TY ay = (TY)alpha;
+#pragma unroll
+    for (int q = 0; q < 4 * NC; ++q) {
+      y[q].re += ay * (TY)p[q].re;
+      y[q].im += ay * (TY)p[q].im;
+      r[q].re -= ar * res[q].re;
+      r[q].im -= ar * res[q].im;
+      part += (double)r[q].re * (double)r[q].re + (double)r[q].im * (double)r[q].im;
+    }
+    if (valid) {
+      SpinorIO<PY, NC>::store(yb, V, s, y);
+      SpinorIO<P, NC>::store(rb, V, s, r);
+    }
+  }
+  return valid ? part : 0.0;
+}
+
 // One site per thread (march = 1): every neighbour comes from memory.
-// EPI: 0 store, 1 store + Re<w, out>, 2 out = w - result and |out|^2
 template <int P, int NC, int RECON, bool DAG, bool DIAG, bool G5, int EPI>
 __global__ __launch_bounds__(NT) void k_dslash_site(DslashArgs a) {
   using T = typename PT<P>::acc;
@@ -49,7 +113,6 @@ __global__ __launch_bounds__(NT) void k_dslash_site(DslashArgs a) {
   const int cfg = col / a.cols_per_cfg;
   const char* Ucfg = reinterpret_cast<const char*>(a.U) + (size_t)cfg * a.cfg_bytes;
   const char* inb = reinterpret_cast<const char*>(a.in) + (size_t)col * a.col_bytes;
-  char* outb = reinterpret_cast<char*>(a.out) + (size_t)col * a.col_bytes;
 
   // coordinates (last axis fastest)
   int x[4];
@@ -97,35 +160,15 @@ __global__ __launch_bounds__(NT) void k_dslash_site(DslashArgs a) {
   }
   if (G5) apply_g5<T, NC>(res);
 
-  double part = 0.0;
-  if (EPI == 2) {
-    const char* wb = reinterpret_cast<const char*>(a.w) + (size_t)col * a.col_bytes;
-    cx<T> w[4 * NC];
-    SpinorIO<P, NC>::load(wb, V, s, w);
-#pragma unroll
-    for (int c = 0; c < 4 * NC; ++c) {
-      res[c] = csub(w[c], res[c]);
-      part += (double)res[c].re * (double)res[c].re + (double)res[c].im * (double)res[c].im;
-    }
-  } else if (EPI == 1) {
-    const char* wb = reinterpret_cast<const char*>(a.w) + (size_t)col * a.col_bytes;
-    cx<T> w[4 * NC];
-    SpinorIO<P, NC>::load(wb, V, s, w);
-#pragma unroll
-    for (int c = 0; c < 4 * NC; ++c) part += (double)w[c].re * (double)res[c].re + (double)w[c].im * (double)res[c].im;
-  }
-  if (valid) store_nt<P, NC>(outb, V, s, res);
-  if (EPI != 0) {
-    if (!valid) part = 0.0;
-    write_partial(a.red, col, tile, a.ntiles, part);
-  }
+  const double part = site_epilogue<P, NC, EPI>(a, col, V, s, valid, res);
+  if (EPI != EPI_STORE) write_partial(a.red, col, tile, a.ntiles, part);
 }
 
 // Multi-RHS: one site per thread and K right-hand sides of the same gauge
 // configuration per thread; each link is loaded once and applied to K
 // spinors.  Columns col0 .. col0+K-1 share one workgroup; a column whose
 // solve is inactive is skipped (its loads and stores are predicated off).
-// EPI as for k_dslash_site, one reduction partial per column.
+// One reduction partial per column.
 template <int P, int NC, int RECON, bool DAG, bool DIAG, bool G5, int EPI, int K>
 __global__ __launch_bounds__(NT) void k_dslash_mrhs(DslashArgs a) {
   using T = typename PT<P>::acc;
@@ -216,7 +259,6 @@ __global__ __launch_bounds__(NT) void k_dslash_mrhs(DslashArgs a) {
     double part = 0.0;
     if (on[k]) {
       const char* inb = reinterpret_cast<const char*>(a.in) + (size_t)col * a.col_bytes;
-      char* outb = reinterpret_cast<char*>(a.out) + (size_t)col * a.col_bytes;
       cx<T> res[4 * NC];
       if (DIAG) {
         const T m0 = a.m0_from_state ? T(st[col].m0) : T(a.m0);
@@ -230,23 +272,9 @@ __global__ __launch_bounds__(NT) void k_dslash_mrhs(DslashArgs a) {
         for (int q = 0; q < 4 * NC; ++q) res[q] = {T(-0.5) * acc[k][q].re, T(-0.5) * acc[k][q].im};
       }
       if (G5) apply_g5<T, NC>(res);
-      if (EPI != 0) {
-        const char* wb = reinterpret_cast<const char*>(a.w) + (size_t)col * a.col_bytes;
-        cx<T> w[4 * NC];
-        SpinorIO<P, NC>::load(wb, V, s, w);
-#pragma unroll
-        for (int q = 0; q < 4 * NC; ++q) {
-          if (EPI == 2) {
-            res[q] = csub(w[q], res[q]);
-            part += (double)res[q].re * (double)res[q].re + (double)res[q].im * (double)res[q].im;
-          } else {
-            part += (double)w[q].re * (double)res[q].re + (double)w[q].im * (double)res[q].im;
-          }
-        }
-      }
-      if (valid) store_nt<P, NC>(outb, V, s, res);
+      part = site_epilogue<P, NC, EPI>(a, col, V, s, valid, res);
     }
-    if (EPI != 0 && on[k]) write_partial(a.red, col, tile, a.ntiles, valid ? part : 0.0);
+    if (EPI != EPI_STORE && on[k]) write_partial(a.red, col, tile, a.ntiles, part);
   }
 }
 
@@ -258,8 +286,6 @@ __global__ __launch_bounds__(NT) void k_dslash_mrhs(DslashArgs a) {
 // neighbours from memory (L1/L2).  Workgroup tiles are ordered
 // segment-fastest, so the segment that reads a boundary plane and the one
 // that owns it run back to back on the same XCD.
-//
-// EPI: 0 store, 1 store + Re<w, out> partials, 2 out = w - result and |out|^2 partials
 template <int P, int NC, int RECON, bool DAG, bool DIAG, bool G5, int EPI>
 __global__ __launch_bounds__(NT) void k_dslash(DslashArgs a) {
   using T = typename PT<P>::acc;
@@ -283,8 +309,6 @@ __global__ __launch_bounds__(NT) void k_dslash(DslashArgs a) {
   const int cfg = col / a.cols_per_cfg;
   const char* Ucfg = reinterpret_cast<const char*>(a.U) + (size_t)cfg * a.cfg_bytes;
   const char* inb = reinterpret_cast<const char*>(a.in) + (size_t)col * a.col_bytes;
-  char* outb = reinterpret_cast<char*>(a.out) + (size_t)col * a.col_bytes;
-  const char* wb = reinterpret_cast<const char*>(a.w) + (size_t)col * a.col_bytes;
 
   // coordinates of the column (last axis fastest)
   int x[4];
@@ -368,25 +392,11 @@ __global__ __launch_bounds__(NT) void k_dslash(DslashArgs a) {
       res[q] = DIAG ? cx<T>{m0 * cur[q].re - T(0.5) * acc[q].re, m0 * cur[q].im - T(0.5) * acc[q].im}
                     : cx<T>{T(-0.5) * acc[q].re, T(-0.5) * acc[q].im};
     if (G5) apply_g5<T, NC>(res);
-    if (EPI == 2) {
-      cx<T> w[4 * NC];
-      SpinorIO<P, NC>::load(wb, V, s, w);
-#pragma unroll
-      for (int q = 0; q < 4 * NC; ++q) {
-        res[q] = csub(w[q], res[q]);
-        part += (double)res[q].re * (double)res[q].re + (double)res[q].im * (double)res[q].im;
-      }
-    } else if (EPI == 1) {
-      cx<T> w[4 * NC];
-      SpinorIO<P, NC>::load(wb, V, s, w);
-#pragma unroll
-      for (int q = 0; q < 4 * NC; ++q) part += (double)w[q].re * (double)res[q].re + (double)w[q].im * (double)res[q].im;
-    }
-    if (valid) store_nt<P, NC>(outb, V, s, res);
+    part += site_epilogue<P, NC, EPI>(a, col, V, s, valid, res);
 #pragma unroll
     for (int q = 0; q < 4 * NC; ++q) cur[q] = nxt[q];
   }
-  if (EPI != 0) write_partial(a.red, col, tile, a.ntiles, valid ? part : 0.0);
+  if (EPI != EPI_STORE) write_partial(a.red, col, tile, a.ntiles, part);
 }
 
 }  // namespace wl

@@ -1,0 +1,64 @@
+// Kernel selection for one storage precision; included by the per-precision
+// translation units wl_dslash_{f64,f32,f16}.hip so they compile in parallel.
+#pragma once
+#include "wl_dslash_kernels.h"
+
+namespace wl {
+
+template <int P, int NC, int RECON, bool DAG, bool DIAG, bool G5, int EPI>
+static void launch_k(const DslashArgs& a, dim3 grid, hipStream_t stream) {
+  if (NC == 3 && a.mrhs == 4)
+    hipLaunchKernelGGL((k_dslash_mrhs<P, NC, RECON, DAG, DIAG, G5, EPI, 4>), dim3(a.ntiles * (a.ncol / 4)), dim3(NT), 0,
+                       stream, a);
+  else if (NC == 3 && a.mrhs == 2)
+    hipLaunchKernelGGL((k_dslash_mrhs<P, NC, RECON, DAG, DIAG, G5, EPI, 2>), dim3(a.ntiles * (a.ncol / 2)), dim3(NT), 0,
+                       stream, a);
+  else if (a.march == 1)
+    hipLaunchKernelGGL((k_dslash_site<P, NC, RECON, DAG, DIAG, G5, EPI>), grid, dim3(NT), 0, stream, a);
+  else
+    hipLaunchKernelGGL((k_dslash<P, NC, RECON, DAG, DIAG, G5, EPI>), grid, dim3(NT), 0, stream, a);
+}
+
+// op: 0 D, 1 D^dag, 2 gamma5 D, 3 K, 4 K^dag.  The reducing epilogues exist for
+// the operator the solver uses them with: EPI_NORM on D, EPI_RESID (FP64 only)
+// and the CG steps on D^dag (an FP64 solve accumulates into FP64).
+template <int P, int NC, int RECON>
+static hipError_t launch_p(int op, int epi, const DslashArgs& a, hipStream_t stream) {
+  dim3 grid((unsigned)(a.ntiles * a.ncol));
+  if (epi == EPI_STORE) {
+    switch (op) {
+      case 0: launch_k<P, NC, RECON, false, true, false, EPI_STORE>(a, grid, stream); break;
+      case 1: launch_k<P, NC, RECON, true, true, false, EPI_STORE>(a, grid, stream); break;
+      case 2: launch_k<P, NC, RECON, false, true, true, EPI_STORE>(a, grid, stream); break;
+      case 3: launch_k<P, NC, RECON, false, false, false, EPI_STORE>(a, grid, stream); break;
+      case 4: launch_k<P, NC, RECON, true, false, false, EPI_STORE>(a, grid, stream); break;
+      default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+  }
+  if (epi == EPI_NORM && op == 0) {
+    launch_k<P, NC, RECON, false, true, false, EPI_NORM>(a, grid, stream);
+    return hipGetLastError();
+  }
+  if (op != 1) return hipErrorInvalidValue;
+  if constexpr (P == F64) {
+    if (epi == EPI_RESID) launch_k<P, NC, RECON, true, true, false, EPI_RESID>(a, grid, stream);
+    else if (epi == EPI_CG64) launch_k<P, NC, RECON, true, true, false, EPI_CG64>(a, grid, stream);
+    else return hipErrorInvalidValue;
+  } else {
+    if (epi == EPI_CG64) launch_k<P, NC, RECON, true, true, false, EPI_CG64>(a, grid, stream);
+    else if (epi == EPI_CG32) launch_k<P, NC, RECON, true, true, false, EPI_CG32>(a, grid, stream);
+    else if (epi == EPI_CG16) launch_k<P, NC, RECON, true, true, false, EPI_CG16>(a, grid, stream);
+    else return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int P>
+static hipError_t launch_dslash_prec(int nc, int recon, int op, int epi, const DslashArgs& a, hipStream_t stream) {
+  if (nc == 3) return recon == 12 ? launch_p<P, 3, 12>(op, epi, a, stream) : launch_p<P, 3, 18>(op, epi, a, stream);
+  if (nc == 1) return launch_p<P, 1, 18>(op, epi, a, stream);
+  return hipErrorInvalidValue;
+}
+
+}  // namespace wl

@@ -24,7 +24,7 @@ constexpr long long PLANE_PAD = 288;
 struct ColState {
   double rr;        // |r|^2 of the iterated residual
   double rr_old;    // |r|^2 at the start of the current iteration
-  double pAp;       // Re <p, A p>
+  double pAp;       // <p, A p> = |D p|^2
   double alpha, beta;
   double b2;        // |b|^2
   double stop2;     // inner stopping threshold on rr
@@ -46,7 +46,7 @@ struct ColState {
 // Which per-column action the last workgroup of a reduction performs.
 enum RedAction {
   RA_NONE = 0,
-  RA_PAP = 1,     // pAp = sum; alpha = rr / pAp
+  RA_PAP = 1,     // pAp = sum (= |D p|^2 = <p, D^dag D p>); alpha = rr / pAp
   RA_ITER = 2,    // rr update, beta, iteration count, stopping / update triggers
   RA_B2 = 3,      // b2 = sum
   RA_TRUERES = 4, // rtrue2 = sum: convergence decision after a reliable update
@@ -59,6 +59,17 @@ struct Reduction {
   int only_col;          // column of gate 3
 };
 
+// Stencil epilogues.  EPI_NORM and the CG steps feed the fused CG iteration
+// (see wl_dslash_kernels.h); EPI_RESID is the FP64 true residual.
+enum Epilogue {
+  EPI_STORE = 0,  // out = result
+  EPI_NORM = 1,   // out = result; partial |result|^2
+  EPI_RESID = 2,  // out = w - result; partial |out|^2
+  EPI_CG64 = 3,   // CG step with A p = result (not stored): y += alpha w, r -= alpha A p; partial |r|^2; y in F64
+  EPI_CG32 = 4,   //   ... y in F32
+  EPI_CG16 = 5,   //   ... y in F16
+};
+
 struct DslashArgs {
   Geom g;
   const void* U;          // gauge configurations, internal layout
@@ -66,8 +77,11 @@ struct DslashArgs {
   size_t dir_bytes;       // bytes of one direction
   const void* in;
   void* out;
-  const void* w;          // epilogue operand (dot / residual), same precision as out
-  size_t col_bytes;       // bytes of one spinor column (in, out and w share precision)
+  const void* w;          // epilogue operand: b of EPI_RESID, p of the CG steps (precision of out)
+  void* res;              // CG steps: iterated residual r (precision of out)
+  void* y;                // CG steps: solution accumulator (precision given by the epilogue)
+  size_t col_bytes;       // bytes of one spinor column (in, out, w and r share precision)
+  size_t col_bytes_y;     // bytes of one column of y
   int ncol;
   int cols_per_cfg;
   int ntiles;             // nchunk * nseg
