@@ -168,9 +168,10 @@ def main():
             dslash[f"{prec}_r{recon}"] = {
                 "us": round(us, 2), "GB/s": round(gbs, 1), "frac_hbm_peak": round(gbs / PEAK_HBM_GBS, 4),
                 "GFLOP/s": round(FLOP_PER_SITE * V / (us * 1e-6) / 1e9, 1), "bytes_per_site": bps,
-                "march": t1["best"], "xcd_remap": t1["xcd_remap"], "tune_us": {m: round(t, 2) for m, t in t1["us"].items()},
+                "march": t1["best"], "xcd_remap": t1["xcd_remap"], "slab": t1["slab"],
+                "tune_us": {m: round(t, 2) for m, t in t1["us"].items()},
             }
-            log(f"dslash {prec} r{recon}: {us:.1f} us {gbs:.0f} GB/s march {t1['best']}")
+            log(f"dslash {prec} r{recon}: {us:.1f} us {gbs:.0f} GB/s march {t1['best']} slab {t1['slab']}")
             del fi, fo
 
         if args.profile_only:
@@ -206,6 +207,7 @@ def main():
             gbs = bps * V / (best[1] * 1e-6) / 1e9
             dslash_k[f"{prec}_r{recon}"] = {
                 "nrhs": k, "rhs_per_thread": best[0], "march": tk["march"], "xcd_remap": tk["xcd_remap"],
+                "slab": tk["slab"],
                 "us": round(best[1], 2), "us_per_rhs": round(best[1] / k, 2),
                 "GB/s": round(gbs, 1), "frac_hbm_peak": round(gbs / PEAK_HBM_GBS, 4), "bytes_per_site": bps,
                 "tune_us": tried,

@@ -132,17 +132,25 @@ void wl_ctx_destroy(wl_ctx* ctx);
 int wl_ctx_set_mass(wl_ctx* ctx, double mass);
 /* Stencil launch tuning (no counterpart in the reference), per storage
  * precision and separately for single-RHS (multi_rhs = 0) and multi-RHS
- * (multi_rhs = 1) launches:
- *   march           x0 sites each GPU thread marches over (0 = automatic; must
- *                   divide L0; 1 = one site per thread),
- *   rhs_per_thread  right-hand sides sharing each link load in a thread (multi-RHS
- *                   only; 0 = automatic, 1, 2 or 4; used when it divides the
- *                   right-hand sides per configuration),
- *   xcd_remap       1: each XCD walks one contiguous range of lattice tiles,
- *                   0: hardware workgroup order.
- * wl_ctx_get_tuning reports the values in effect for nrhs right-hand sides. */
-int wl_ctx_set_tuning(wl_ctx* ctx, int prec, int multi_rhs, int march, int rhs_per_thread, int xcd_remap);
-int wl_ctx_get_tuning(wl_ctx* ctx, int prec, int nrhs, int* march, int* rhs_per_thread, int* xcd_remap);
+ * (multi_rhs = 1) launches. */
+typedef struct wl_tuning {
+  int march;          /* x0 sites each GPU thread marches over (0 = automatic; must
+                         divide L0; 1 = one site per thread) */
+  int rhs_per_thread; /* right-hand sides sharing each link load in a thread (multi-RHS
+                         only; 0 = automatic, 1, 2 or 4; used when it divides the
+                         right-hand sides per configuration) */
+  int xcd_remap;      /* 1: each XCD walks one contiguous range of tiles; 0: hardware
+                         workgroup order */
+  int slab;           /* one-site-per-thread kernels: 0 = flat site order; s > 0 (must
+                         divide L1) = tiles walk slabs of s consecutive x1 values,
+                         x0 fastest, so that x0 neighbours are reused from L2 */
+  int cols_outer;     /* several spinor columns: 0 = the columns of one tile run back to
+                         back (links shared in L2); 1 = one column group sweeps the
+                         lattice before the next (spinor neighbours shared in L2) */
+} wl_tuning;
+/* wl_ctx_get_tuning reports the values in effect for nrhs right-hand sides. */
+int wl_ctx_set_tuning(wl_ctx* ctx, int prec, int multi_rhs, const wl_tuning* t);
+int wl_ctx_get_tuning(wl_ctx* ctx, int prec, int nrhs, wl_tuning* t);
 
 /* where: 0 = device pointer, 1 = host pointer */
 int wl_gauge_upload(wl_ctx* ctx, const void* U, int layout, int dtype, int where, wl_gauge** out);

@@ -64,6 +64,11 @@ class WlSolveStats(ctypes.Structure):
     ]
 
 
+class WlTuning(ctypes.Structure):
+    _fields_ = [("march", c_int), ("rhs_per_thread", c_int), ("xcd_remap", c_int), ("slab", c_int),
+                ("cols_outer", c_int)]
+
+
 class NativeError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"libwilson_amd error {code}: {msg}")
@@ -97,8 +102,8 @@ def load() -> ctypes.CDLL:
         "wl_ctx_create": (c_int, [POINTER(WlGeom), POINTER(WlParams), POINTER(vp)]),
         "wl_ctx_destroy": (None, [vp]),
         "wl_ctx_set_mass": (c_int, [vp, c_double]),
-        "wl_ctx_set_tuning": (c_int, [vp, c_int, c_int, c_int, c_int, c_int]),
-        "wl_ctx_get_tuning": (c_int, [vp, c_int, c_int, POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
+        "wl_ctx_set_tuning": (c_int, [vp, c_int, c_int, POINTER(WlTuning)]),
+        "wl_ctx_get_tuning": (c_int, [vp, c_int, c_int, POINTER(WlTuning)]),
         "wl_gauge_upload": (c_int, [vp, vp, c_int, c_int, c_int, POINTER(vp)]),
         "wl_gauge_release": (None, [vp]),
         "wl_gauge_prepare": (c_int, [vp, c_int, c_int, vp]),
@@ -169,19 +174,20 @@ class Context:
         self._lib = lib
 
     def set_tuning(self, prec: int, multi_rhs: bool, march: int = 0, rhs_per_thread: int = 0,
-                   xcd_remap: bool = True) -> None:
-        check(self._lib.wl_ctx_set_tuning(self.h, int(prec), 1 if multi_rhs else 0, int(march), int(rhs_per_thread),
-                                          1 if xcd_remap else 0))
+                   xcd_remap: bool = True, slab: int = 0, cols_outer: bool = False) -> None:
+        t = WlTuning(int(march), int(rhs_per_thread), 1 if xcd_remap else 0, int(slab), 1 if cols_outer else 0)
+        check(self._lib.wl_ctx_set_tuning(self.h, int(prec), 1 if multi_rhs else 0, byref(t)))
 
     def get_tuning(self, prec: int, nrhs: int = 1):
-        m, k, x = c_int(), c_int(), c_int()
-        check(self._lib.wl_ctx_get_tuning(self.h, int(prec), int(nrhs), byref(m), byref(k), byref(x)))
-        return {"march": m.value, "rhs_per_thread": k.value, "xcd_remap": bool(x.value)}
+        t = WlTuning()
+        check(self._lib.wl_ctx_get_tuning(self.h, int(prec), int(nrhs), byref(t)))
+        return {"march": t.march, "rhs_per_thread": t.rhs_per_thread, "xcd_remap": bool(t.xcd_remap),
+                "slab": t.slab, "cols_outer": bool(t.cols_outer)}
 
     # single-RHS shortcuts
     def set_march(self, prec: int, march: int) -> None:
         cur = self.get_tuning(prec, 1)
-        self.set_tuning(prec, False, march, 0, cur["xcd_remap"])
+        self.set_tuning(prec, False, march, 0, cur["xcd_remap"], cur["slab"] if march == 1 else 0, cur["cols_outer"])
 
     def get_march(self, prec: int) -> int:
         return self.get_tuning(prec, 1)["march"]

@@ -5,6 +5,7 @@ field with HIP events on the caller's stream and installs the fastest in the
 context.  The tunable is the number of x0 sites each thread marches over
 (``march``: 1 selects the one-site-per-thread kernel), for multi-RHS
 launches the number of right-hand sides sharing each link load in a thread,
+the site order of the one-site kernels (flat or x1 slabs walked x0-fastest)
 and the workgroup order (XCD-contiguous or hardware order).
 """
 
@@ -37,32 +38,45 @@ def time_dslash(ctx: nat.Context, gauge: nat.Gauge, prec: int, recon: int, fin: 
 
 def tune(ctx: nat.Context, gauge: nat.Gauge, prec: int, recon: int, fin: nat.Field, fout: nat.Field,
          nrhs: int = 1, marches: Iterable[int] = (1, 2, 4, 8), reps: int = 20,
-         stream: Optional[torch.cuda.Stream] = None) -> Dict[str, object]:
+         stream: Optional[torch.cuda.Stream] = None, slabs: Iterable[int] = (0, 2, 4, 8),
+         cols_outer: Iterable[bool] = (False,)) -> Dict[str, object]:
     """Pick the fastest launch shape for (prec, recon) at nrhs right-hand sides
     per configuration and install it.  Candidates: every march length dividing
     L0 with one right-hand side per thread, plus 2 and 4 right-hand sides per
-    thread when nrhs > 1; each in XCD-contiguous and hardware workgroup order.
-    Returns {"march", "rhs_per_thread", "xcd_remap", "us": {label: us}}."""
+    thread when nrhs > 1; the one-site-per-thread shapes in flat and x1-slab
+    site order (slab extents dividing L1); each in XCD-contiguous and hardware
+    workgroup order, and for nrhs > 1 in each column order of `cols_outer`
+    (column-fastest only by default: sweeping the lattice once per column
+    group re-reads the links and measured slower at 32^4).
+    Returns {"march", "rhs_per_thread", "xcd_remap", "slab", "cols_outer", "us": {label: us}}."""
     stream = stream or torch.cuda.current_stream()
-    L0 = ctx.lattice_shape[0]
+    L0, L1 = ctx.lattice_shape[0], ctx.lattice_shape[1]
     multi = nrhs > 1
     gauge.prepare(prec, recon)
-    shapes = [(m, 1) for m in marches if L0 % m == 0]
+    slab_c = [s for s in slabs if s == 0 or (s < L1 and L1 % s == 0)]
+    shapes = []
+    for m in marches:
+        if L0 % m:
+            continue
+        shapes += [(m, 1, sl) for sl in (slab_c if m == 1 else [0])]
     if multi:
-        shapes += [(1, k) for k in (2, 4) if nrhs % k == 0]
+        shapes += [(1, k, sl) for k in (2, 4) if nrhs % k == 0 for sl in slab_c]
+    orders = [bool(c) for c in cols_outer] if multi else [False]
     times = {}
-    for r in (1, 0):
-        for m, k in shapes:
-            ctx.set_tuning(prec, multi, m, k if multi else 0, bool(r))
-            times[(m, k, r)] = time_dslash(ctx, gauge, prec, recon, fin, fout, nrhs, reps, stream)
+    for co in orders:
+        for r in (1, 0):
+            for m, k, sl in shapes:
+                ctx.set_tuning(prec, multi, m, k if multi else 0, bool(r), sl, co)
+                times[(m, k, r, sl, co)] = time_dslash(ctx, gauge, prec, recon, fin, fout, nrhs, reps, stream)
     best = min(times, key=times.get)
-    ctx.set_tuning(prec, multi, best[0], best[1] if multi else 0, bool(best[2]))
-    return {"march": best[0], "rhs_per_thread": best[1], "xcd_remap": best[2],
-            "us": {f"m{m}k{k}x{r}": t for (m, k, r), t in times.items()}}
+    ctx.set_tuning(prec, multi, best[0], best[1] if multi else 0, bool(best[2]), best[3], best[4])
+    return {"march": best[0], "rhs_per_thread": best[1], "xcd_remap": best[2], "slab": best[3],
+            "cols_outer": best[4],
+            "us": {f"m{m}k{k}x{r}s{sl}" + ("c" if co else ""): t for (m, k, r, sl, co), t in times.items()}}
 
 
 def tune_march(ctx, gauge, prec, recon, fin, fout, nrhs: int = 1, candidates=(1, 2, 4, 8), reps: int = 20,
                stream=None) -> Dict[str, object]:
     """Single-RHS form of `tune`: {"best": march, "xcd_remap": r, "us": ...}."""
     t = tune(ctx, gauge, prec, recon, fin, fout, nrhs, candidates, reps, stream)
-    return {"best": t["march"], "xcd_remap": t["xcd_remap"], "us": t["us"]}
+    return {"best": t["march"], "xcd_remap": t["xcd_remap"], "slab": t["slab"], "us": t["us"]}

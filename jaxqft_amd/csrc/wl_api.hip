@@ -108,6 +108,8 @@ struct wl_ctx {
   int march[2][3] = {{0, 0, 0}, {0, 0, 0}};      // x0 sites per thread (0 = automatic)
   int mrhs[3] = {0, 0, 0};                       // right-hand sides per thread (0 = automatic)
   int xcd_remap[2][3] = {{1, 1, 1}, {1, 1, 1}};  // XCD-contiguous workgroup order
+  int slab[2][3] = {{0, 0, 0}, {0, 0, 0}};       // x1 extent of a slab (0 = flat site order)
+  int cols_outer[2][3] = {{0, 0, 0}, {0, 0, 0}}; // column group is the slowest workgroup index
 };
 
 struct wl_gauge {
@@ -162,10 +164,19 @@ static int mrhs_for(const wl_ctx* c, int prec, int nrhs) {
   return (k > 1 && nrhs % k == 0) ? k : 1;
 }
 
-// upper bound of reduction partials per column over every kernel
+// tiles of a one-site-per-thread launch: flat, or L0 * (L1 / slab) slabs of
+// ceil(slab * L2 * L3 / NT) tiles
+static inline int site_tiles(const wl_ctx* c, int slab) {
+  if (slab <= 0) return (int)((c->V + NT - 1) / NT);
+  const long long plane = (long long)c->geom.L[2] * c->geom.L[3];
+  return c->geom.L[0] * (c->geom.L[1] / slab) * (int)((slab * plane + NT - 1) / NT);
+}
+
+// upper bound of reduction partials per column over every kernel (a slab
+// order adds at most one partial tile per slab, i.e. at most L0 * L1)
 static inline size_t max_tiles(const wl_ctx* c) {
   size_t a = (size_t)nchunk_of(c) * c->geom.L[0];
-  size_t b = (size_t)((c->V + NT - 1) / NT);
+  size_t b = (size_t)((c->V + NT - 1) / NT) + (size_t)c->geom.L[0] * c->geom.L[1];
   return a > b ? a : b;
 }
 
@@ -225,28 +236,37 @@ int wl_ctx_set_mass(wl_ctx* c, double mass) {
   return WL_OK;
 }
 
-int wl_ctx_set_tuning(wl_ctx* c, int prec, int multi_rhs, int march, int rhs_per_thread, int xcd_remap) {
+int wl_ctx_set_tuning(wl_ctx* c, int prec, int multi_rhs, const wl_tuning* t) {
   CHK(check_ctx(c));
+  if (!t) return fail(WL_ERR_BAD_ARG, "null tuning");
   if (prec < F64 || prec > F16) return fail(WL_ERR_BAD_ARG, "bad precision %d", prec);
-  if (march < 0 || (march > 0 && c->geom.L[0] % march != 0))
-    return fail(WL_ERR_BAD_ARG, "march %d must be 0 (automatic) or divide L0=%d", march, c->geom.L[0]);
-  if (rhs_per_thread != 0 && rhs_per_thread != 1 && rhs_per_thread != 2 && rhs_per_thread != 4)
+  if (t->march < 0 || (t->march > 0 && c->geom.L[0] % t->march != 0))
+    return fail(WL_ERR_BAD_ARG, "march %d must be 0 (automatic) or divide L0=%d", t->march, c->geom.L[0]);
+  if (t->rhs_per_thread != 0 && t->rhs_per_thread != 1 && t->rhs_per_thread != 2 && t->rhs_per_thread != 4)
     return fail(WL_ERR_BAD_ARG, "rhs_per_thread must be 0 (automatic), 1, 2 or 4");
+  if (t->slab < 0 || (t->slab > 0 && c->geom.L[1] % t->slab != 0))
+    return fail(WL_ERR_BAD_ARG, "slab %d must be 0 (flat) or divide L1=%d", t->slab, c->geom.L[1]);
   std::lock_guard<std::mutex> lk(c->mu);
   const int m = multi_rhs ? 1 : 0;
-  c->march[m][prec] = march;
-  c->xcd_remap[m][prec] = xcd_remap ? 1 : 0;
-  if (multi_rhs) c->mrhs[prec] = rhs_per_thread;
+  c->march[m][prec] = t->march;
+  c->xcd_remap[m][prec] = t->xcd_remap ? 1 : 0;
+  c->slab[m][prec] = t->slab;
+  c->cols_outer[m][prec] = t->cols_outer ? 1 : 0;
+  if (multi_rhs) c->mrhs[prec] = t->rhs_per_thread;
   return WL_OK;
 }
 
-int wl_ctx_get_tuning(wl_ctx* c, int prec, int nrhs, int* march, int* rhs_per_thread, int* xcd_remap) {
+int wl_ctx_get_tuning(wl_ctx* c, int prec, int nrhs, wl_tuning* t) {
   CHK(check_ctx(c));
+  if (!t) return fail(WL_ERR_BAD_ARG, "null tuning");
   if (prec < F64 || prec > F16 || nrhs < 1) return fail(WL_ERR_BAD_ARG, "bad precision or nrhs");
   const int k = mrhs_for(c, prec, nrhs);
-  if (march) *march = k > 1 ? 1 : march_for(c, prec, nrhs);
-  if (rhs_per_thread) *rhs_per_thread = k;
-  if (xcd_remap) *xcd_remap = c->xcd_remap[nrhs > 1 ? 1 : 0][prec];
+  const int m = nrhs > 1 ? 1 : 0;
+  t->march = k > 1 ? 1 : march_for(c, prec, nrhs);
+  t->rhs_per_thread = k;
+  t->xcd_remap = c->xcd_remap[m][prec];
+  t->slab = t->march == 1 ? c->slab[m][prec] : 0;
+  t->cols_outer = c->cols_outer[m][prec];
   return WL_OK;
 }
 
@@ -341,8 +361,11 @@ static void set_launch_shape(const wl_ctx* c, DslashArgs& a, int prec) {
   a.mrhs = mrhs_for(c, prec, a.cols_per_cfg);
   a.march = a.mrhs > 1 ? 1 : march_for(c, prec, a.cols_per_cfg);
   a.xcd_remap = c->xcd_remap[a.cols_per_cfg > 1 ? 1 : 0][prec];
+  a.slab = a.march == 1 ? c->slab[a.cols_per_cfg > 1 ? 1 : 0][prec] : 0;
+  a.cols_outer = c->cols_outer[a.cols_per_cfg > 1 ? 1 : 0][prec];
+  a.slab_tiles = a.slab ? (int)(((long long)a.slab * c->geom.L[2] * c->geom.L[3] + NT - 1) / NT) : 0;
   a.nseg = c->geom.L[0] / a.march;
-  a.ntiles = a.march == 1 ? (int)((c->V + NT - 1) / NT) : nchunk_of(c) * a.nseg;
+  a.ntiles = a.march == 1 ? site_tiles(c, a.slab) : nchunk_of(c) * a.nseg;
 }
 
 static void fill_dslash_common(const wl_ctx* c, DslashArgs& a, int prec) {

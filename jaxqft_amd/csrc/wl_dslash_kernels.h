@@ -5,10 +5,11 @@
 //   A p = D^dag t   with y += alpha p, r -= alpha A p, |r|^2 -> beta
 // so A p is never written to memory.
 //
-// The 1D grid is ntiles x ncol workgroups ordered spinor-column-fastest and
-// remapped so that each XCD walks one contiguous range of tiles: the k
-// right-hand sides of one tile run back to back on the same XCD and share
-// its link lines in L2.
+// The 1D grid is ntiles x ncol workgroups, ordered spinor-column-fastest (the
+// columns of one tile run back to back and share its link lines in L2) or
+// column-slowest (one column sweeps the lattice before the next, so its
+// neighbour spinors stay in L2), and remapped so that each XCD walks one
+// contiguous range of that order.
 #pragma once
 #include "wl_kernels.h"
 #include "wl_reduce.h"
@@ -32,6 +33,31 @@ __device__ __forceinline__ void apply_g5(cx<T> (&v)[4 * NC]) {
 template <int P, int NC>
 __device__ __forceinline__ void store_nt(void* base, size_t V, size_t site, const cx<typename PT<P>::acc> (&v)[4 * NC]) {
   SpinorIO<P, NC>::store(base, V, site, v);
+}
+
+// Site of this thread in `tile` for the one-site-per-thread kernels.  Flat
+// order: tile t covers sites [t NT, (t+1) NT).  Slab order: a slab is the
+// sites with one x0 and `slab` consecutive x1 values (slab * L2 * L3 sites,
+// ceil(./NT) tiles); slabs are numbered x0-fastest, so a tile's x0
+// neighbours are read by the tiles of the adjacent slabs moments before or
+// after it, and each XCD (contiguous tile range) sweeps x0 over its own x1
+// range.  `valid` is false for the padding threads of a slab's last tile.
+__device__ __forceinline__ size_t site_of_tile(const DslashArgs& a, int tile, bool& valid) {
+  if (a.slab == 0) {
+    const size_t site = (size_t)tile * NT + threadIdx.x;
+    valid = site < (size_t)a.g.V;
+    return valid ? site : 0;
+  }
+  const int sl = tile / a.slab_tiles;
+  const int sub = tile - sl * a.slab_tiles;
+  const int x0 = sl % a.g.L[0];
+  const int x1b = sl / a.g.L[0];
+  const unsigned plane = (unsigned)a.g.stride[1];  // L2 * L3
+  const unsigned i = (unsigned)sub * NT + threadIdx.x;
+  valid = i < (unsigned)a.slab * plane;
+  const unsigned ii = valid ? i : 0u;
+  const unsigned q = ii / plane;
+  return (size_t)x0 * (size_t)a.g.stride[0] + (size_t)(x1b * a.slab + (int)q) * plane + (ii - q * plane);
 }
 
 template <int EPI>
@@ -100,15 +126,14 @@ template <int P, int NC, int RECON, bool DAG, bool DIAG, bool G5, int EPI>
 __global__ __launch_bounds__(NT) void k_dslash_site(DslashArgs a) {
   using T = typename PT<P>::acc;
   const unsigned logical = xcd_tile(blockIdx.x, gridDim.x, a.xcd_remap);
-  const int col = logical % a.ncol;
-  const int tile = logical / a.ncol;
+  const int col = a.cols_outer ? (int)(logical / a.ntiles) : (int)(logical % a.ncol);
+  const int tile = a.cols_outer ? (int)(logical % a.ntiles) : (int)(logical / a.ncol);
   const ColState* st = a.red.cols;
   if (a.gate && gate_skip_col(a.gate, a.red, col)) return;
 
   const size_t V = (size_t)a.g.Vp;  // plane stride
-  const size_t site = (size_t)tile * NT + threadIdx.x;
-  const bool valid = site < (size_t)a.g.V;
-  const size_t s = valid ? site : 0;
+  bool valid;
+  const size_t s = site_of_tile(a, tile, valid);
 
   const int cfg = col / a.cols_per_cfg;
   const char* Ucfg = reinterpret_cast<const char*>(a.U) + (size_t)cfg * a.cfg_bytes;
@@ -131,6 +156,7 @@ __global__ __launch_bounds__(NT) void k_dslash_site(DslashArgs a) {
   for (int c = 0; c < 4 * NC; ++c) acc[c] = {T(0), T(0)};
 
   constexpr int SF = DAG ? +1 : -1;
+  constexpr bool FOLD = P == F16;
 #define WL_HOP(MU)                                                                                        \
   {                                                                                                       \
     const int L = a.g.L[MU];                                                                              \
@@ -139,7 +165,7 @@ __global__ __launch_bounds__(NT) void k_dslash_site(DslashArgs a) {
     const size_t sf = top ? s - (size_t)(L - 1) * str : s + str;                                          \
     const size_t sb = bot ? s + (size_t)(L - 1) * str : s - str;                                          \
     cx<T> ph = {T(a.bc_re[MU]), T(a.bc_im[MU])};                                                          \
-    hop_dir<P, NC, RECON, MU, SF>(Ucfg, a.dir_bytes, inb, V, s, sf, sb, top, bot, ph, acc);               \
+    hop_dir<P, NC, RECON, MU, SF, FOLD>(Ucfg, a.dir_bytes, inb, V, s, sf, sb, top, bot, ph, acc);         \
   }
   WL_HOP(0)
   WL_HOP(1)
@@ -148,16 +174,8 @@ __global__ __launch_bounds__(NT) void k_dslash_site(DslashArgs a) {
 #undef WL_HOP
 
   cx<T> res[4 * NC];
-  if (DIAG) {
-    const T m0 = a.m0_from_state ? T(st[col].m0) : T(a.m0);
-    cx<T> p[4 * NC];
-    SpinorIO<P, NC>::load(inb, V, s, p);
-#pragma unroll
-    for (int c = 0; c < 4 * NC; ++c) res[c] = {m0 * p[c].re - T(0.5) * acc[c].re, m0 * p[c].im - T(0.5) * acc[c].im};
-  } else {
-#pragma unroll
-    for (int c = 0; c < 4 * NC; ++c) res[c] = {T(-0.5) * acc[c].re, T(-0.5) * acc[c].im};
-  }
+  const T m0 = DIAG ? (a.m0_from_state ? T(st[col].m0) : T(a.m0)) : T(0);
+  site_result<P, NC, DIAG, FOLD>(inb, V, s, m0, acc, res);
   if (G5) apply_g5<T, NC>(res);
 
   const double part = site_epilogue<P, NC, EPI>(a, col, V, s, valid, res);
@@ -174,8 +192,8 @@ __global__ __launch_bounds__(NT) void k_dslash_mrhs(DslashArgs a) {
   using T = typename PT<P>::acc;
   const int ngrp = a.ncol / K;
   const unsigned logical = xcd_tile(blockIdx.x, gridDim.x, a.xcd_remap);
-  const int grp = logical % ngrp;
-  const int tile = logical / ngrp;
+  const int grp = a.cols_outer ? (int)(logical / a.ntiles) : (int)(logical % ngrp);
+  const int tile = a.cols_outer ? (int)(logical % a.ntiles) : (int)(logical / ngrp);
   const int col0 = grp * K;
   const ColState* st = a.red.cols;
   bool on[K];
@@ -188,9 +206,8 @@ __global__ __launch_bounds__(NT) void k_dslash_mrhs(DslashArgs a) {
   if (!any) return;
 
   const size_t V = (size_t)a.g.Vp;  // plane stride
-  const size_t site = (size_t)tile * NT + threadIdx.x;
-  const bool valid = site < (size_t)a.g.V;
-  const size_t s = valid ? site : 0;
+  bool valid;
+  const size_t s = site_of_tile(a, tile, valid);
   const int cfg = col0 / a.cols_per_cfg;
   const char* Ucfg = reinterpret_cast<const char*>(a.U) + (size_t)cfg * a.cfg_bytes;
 
@@ -211,6 +228,7 @@ __global__ __launch_bounds__(NT) void k_dslash_mrhs(DslashArgs a) {
     for (int q = 0; q < 4 * NC; ++q) acc[k][q] = {T(0), T(0)};
 
   constexpr int SF = DAG ? +1 : -1;
+  constexpr bool FOLD = P == F16;
 #define WL_MHOP(MU)                                                                                         \
   {                                                                                                         \
     const int L = a.g.L[MU];                                                                                \
@@ -221,29 +239,21 @@ __global__ __launch_bounds__(NT) void k_dslash_mrhs(DslashArgs a) {
     const cx<T> ph = {T(a.bc_re[MU]), T(a.bc_im[MU])};                                                      \
     const char* Umu = Ucfg + MU * a.dir_bytes;                                                              \
     {                                                                                                       \
-      cx<T> u[NC][NC];                                                                                      \
-      LinkIO<P, NC, RECON>::load(Umu, V, s, u);                                                             \
+      typename LinkRegs<P, NC>::type u;                                                                     \
+      load_link<P, NC, RECON>(Umu, V, s, u);                                                                \
       _Pragma("unroll") for (int k = 0; k < K; ++k) {                                                       \
         if (!on[k]) continue;                                                                               \
-        cx<T> p[4 * NC], h[2][NC], o[2][NC];                                                                \
-        SpinorIO<P, NC>::load(reinterpret_cast<const char*>(a.in) + (size_t)(col0 + k) * a.col_bytes, V, sf, p); \
-        spin_project<MU, SF>(p, h);                                                                         \
-        color_mul<false>(u, h, o);                                                                          \
-        if (top) half_phase(o, ph);                                                                         \
-        spin_reconstruct_add<MU, SF>(acc[k], o);                                                            \
+        hop_one<P, NC, MU, SF, false, FOLD>(u, reinterpret_cast<const char*>(a.in) + (size_t)(col0 + k) * a.col_bytes, \
+                                            V, sf, top, ph, acc[k]);                                        \
       }                                                                                                     \
     }                                                                                                       \
     {                                                                                                       \
-      cx<T> u[NC][NC];                                                                                      \
-      LinkIO<P, NC, RECON>::load(Umu, V, sb, u);                                                            \
+      typename LinkRegs<P, NC>::type u;                                                                     \
+      load_link<P, NC, RECON>(Umu, V, sb, u);                                                               \
       _Pragma("unroll") for (int k = 0; k < K; ++k) {                                                       \
         if (!on[k]) continue;                                                                               \
-        cx<T> p[4 * NC], h[2][NC], o[2][NC];                                                                \
-        SpinorIO<P, NC>::load(reinterpret_cast<const char*>(a.in) + (size_t)(col0 + k) * a.col_bytes, V, sb, p); \
-        spin_project<MU, -SF>(p, h);                                                                        \
-        color_mul<true>(u, h, o);                                                                           \
-        if (bot) half_phase(o, cconj(ph));                                                                  \
-        spin_reconstruct_add<MU, -SF>(acc[k], o);                                                           \
+        hop_one<P, NC, MU, -SF, true, FOLD>(u, reinterpret_cast<const char*>(a.in) + (size_t)(col0 + k) * a.col_bytes, \
+                                            V, sb, bot, ph, acc[k]);                                        \
       }                                                                                                     \
     }                                                                                                       \
   }
@@ -260,17 +270,8 @@ __global__ __launch_bounds__(NT) void k_dslash_mrhs(DslashArgs a) {
     if (on[k]) {
       const char* inb = reinterpret_cast<const char*>(a.in) + (size_t)col * a.col_bytes;
       cx<T> res[4 * NC];
-      if (DIAG) {
-        const T m0 = a.m0_from_state ? T(st[col].m0) : T(a.m0);
-        cx<T> p[4 * NC];
-        SpinorIO<P, NC>::load(inb, V, s, p);
-#pragma unroll
-        for (int q = 0; q < 4 * NC; ++q)
-          res[q] = {m0 * p[q].re - T(0.5) * acc[k][q].re, m0 * p[q].im - T(0.5) * acc[k][q].im};
-      } else {
-#pragma unroll
-        for (int q = 0; q < 4 * NC; ++q) res[q] = {T(-0.5) * acc[k][q].re, T(-0.5) * acc[k][q].im};
-      }
+      const T m0 = DIAG ? (a.m0_from_state ? T(st[col].m0) : T(a.m0)) : T(0);
+      site_result<P, NC, DIAG, FOLD>(inb, V, s, m0, acc[k], res);
       if (G5) apply_g5<T, NC>(res);
       part = site_epilogue<P, NC, EPI>(a, col, V, s, valid, res);
     }
@@ -290,8 +291,8 @@ template <int P, int NC, int RECON, bool DAG, bool DIAG, bool G5, int EPI>
 __global__ __launch_bounds__(NT) void k_dslash(DslashArgs a) {
   using T = typename PT<P>::acc;
   const unsigned logical = xcd_tile(blockIdx.x, gridDim.x, a.xcd_remap);
-  const int col = logical % a.ncol;
-  const int tile = logical / a.ncol;
+  const int col = a.cols_outer ? (int)(logical / a.ntiles) : (int)(logical % a.ncol);
+  const int tile = a.cols_outer ? (int)(logical % a.ntiles) : (int)(logical / a.ncol);
   const ColState* st = a.red.cols;
   if (a.gate && gate_skip_col(a.gate, a.red, col)) return;
 

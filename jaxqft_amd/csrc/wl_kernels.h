@@ -89,6 +89,9 @@ struct DslashArgs {
   int nseg;               // L0 / march
   int xcd_remap;          // map workgroups so that each XCD walks a contiguous tile range
   int mrhs;               // right-hand sides per thread (1, 2 or 4; > 1 implies one site per thread)
+  int slab;               // one site per thread: 0 flat order, else x1 extent of a slab (x0-fastest slabs)
+  int slab_tiles;         // tiles per slab
+  int cols_outer;         // workgroup index = column (group) * ntiles + tile instead of tile * ncol + column
   double bc_re[4], bc_im[4];
   double m0;              // diagonal coefficient (used when m0_from_state == 0)
   double r;               // Wilson r

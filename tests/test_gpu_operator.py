@@ -150,11 +150,13 @@ def test_properties_at_benchmark_sizes(L):
     assert abs(float(q.imag)) < 1e-10 * float(q.real) and float(q.real) > 0
 
 
-@pytest.mark.parametrize("march", [1, 2, 4, 8])
+@pytest.mark.parametrize("march,slab", [(1, 0), (1, 1), (1, 2), (2, 0), (4, 0), (8, 0)])
 @pytest.mark.parametrize("nc", [3, 1])
-def test_march_variants_match_oracle(march, nc):
-    """Every stencil launch shape (x0 sites per thread) gives the oracle's D and D^dag."""
-    print(f"[gpu] march={march} nc={nc}", flush=True)
+def test_march_variants_match_oracle(march, slab, nc):
+    """Every stencil launch shape (x0 sites per thread; flat or x1-slab site
+    order, whose slabs of 24 and 48 sites leave most of each tile padding)
+    gives the oracle's D and D^dag."""
+    print(f"[gpu] march={march} slab={slab} nc={nc}", flush=True)
     import torch
 
     from jaxqft_amd import _native as nat
@@ -165,8 +167,9 @@ def test_march_variants_match_oracle(march, nc):
     bc = (-1, np.exp(0.3j), 1, -1)
     rng = fields.rng_for(800 + march)
     th = theory_for(L, nc, 0.07, bc, batch_size=2)
-    th._context().set_march(nat.WL_F64, march)
-    assert th._context().get_march(nat.WL_F64) == march
+    th._context().set_tuning(nat.WL_F64, False, march, 0, True, slab)
+    assert th._context().get_tuning(nat.WL_F64)["march"] == march
+    assert th._context().get_tuning(nat.WL_F64)["slab"] == slab
     U = fields.su3_hot_start(rng, 2, L, 0.6) if nc == 3 else fields.u1_hot_start(rng, 2, L, 0.6)
     psi = fields.random_fermion(rng, th.fermion_shape())
     ora = WilsonOracle(L, nc, 0.07, fermion_bc=bc)
@@ -209,12 +212,51 @@ def test_low_precision_stencils(prec, recon, tol, march):
         assert rel(out.cpu().numpy(), ref) < tol, (op, rel(out.cpu().numpy(), ref))
 
 
+@pytest.mark.parametrize("nc,recon", [(3, 12), (3, 18), (1, 18)])
+@pytest.mark.parametrize("prec,tol", [("f32", 2e-6), ("f16", 3e-3)])
+@pytest.mark.parametrize("nrhs,per_thread", [(1, 0), (2, 2)])
+def test_low_precision_complex_bc(nc, recon, prec, tol, nrhs, per_thread):
+    """Twisted (complex) boundary phases in directions 0 and 3 and an
+    antiperiodic direction 2 through the reduced-precision stencils (the F16
+    path applies real phases as a factor and complex ones in FP32), SU(3) and
+    U(1), one and two right-hand sides per thread, against the FP64 oracle."""
+    print(f"[gpu] complex bc {prec} nc={nc} r{recon} nrhs={nrhs}", flush=True)
+    import torch
+
+    from jaxqft_amd import _native as nat
+    from jaxqft_amd import fields
+    from oracle.wilson_oracle import WilsonOracle
+
+    L = (4, 6, 4, 8)
+    bc = (np.exp(0.4j), 1, -1, np.exp(-0.7j))
+    rng = fields.rng_for(1300 + nc)
+    U = fields.su3_hot_start(rng, 1, L, 0.5) if nc == 3 else fields.u1_hot_start(rng, 1, L, 0.5)
+    psi = fields.random_fermion(rng, (nrhs, *L, 4, nc))
+    ctx = nat.Context(L, nc, 1, 0.06, 1.0, bc)
+    pc = nat.PREC[prec]
+    ctx.set_tuning(pc, nrhs > 1, 1, per_thread, True, 0)
+    g = nat.Gauge(ctx, torch.from_numpy(U).cuda(), nat.WL_LAYOUT_BMXYIJ)
+    fi, fo = nat.Field(ctx, pc, nrhs), nat.Field(ctx, pc, nrhs)
+    fi.import_(torch.from_numpy(psi).cuda())
+    ora = WilsonOracle(L, nc, 0.06, fermion_bc=bc)
+    out = torch.empty((nrhs, *L, 4, nc), dtype=torch.complex128, device="cuda")
+    for op, fn in ((nat.WL_OP_D, ora.apply_D), (nat.WL_OP_DDAG, ora.apply_Ddag)):
+        nat.check(ctx._lib.wl_dslash(ctx.h, g.h, op, recon, fi.h, fo.h, nrhs, nat.c_void_p(nat.current_stream())))
+        fo.export(out)
+        got = out.cpu().numpy()
+        for j in range(nrhs):
+            ref = fn(U, psi[j:j + 1])
+            assert rel(got[j:j + 1], ref) < tol, (op, j, rel(got[j:j + 1], ref))
+
+
 @pytest.mark.parametrize("prec,recon,tol", [("f64", 18, 1e-13), ("f32", 12, 2e-6), ("f16", 12, 3e-3)])
-@pytest.mark.parametrize("per_thread", [1, 2, 4])
-def test_multi_rhs_stencil(prec, recon, tol, per_thread):
-    """k right-hand sides sharing each link load (1, 2 or 4 per thread), two
+@pytest.mark.parametrize("per_thread,slab,cols_outer", [(1, 0, False), (2, 0, False), (4, 0, False), (2, 2, False),
+                                                      (4, 4, False), (1, 2, True), (2, 0, True)])
+def test_multi_rhs_stencil(prec, recon, tol, per_thread, slab, cols_outer):
+    """k right-hand sides sharing each link load (1, 2 or 4 per thread; flat or
+    x1-slab site order; column-fastest or column-slowest workgroup order), two
     gauge configurations, against the oracle column by column."""
-    print(f"[gpu] multi-rhs stencil {prec} k/thread={per_thread}", flush=True)
+    print(f"[gpu] multi-rhs stencil {prec} k/thread={per_thread} slab={slab} cols_outer={cols_outer}", flush=True)
     import torch
 
     from jaxqft_amd import _native as nat
@@ -229,8 +271,10 @@ def test_multi_rhs_stencil(prec, recon, tol, per_thread):
     psi = fields.random_fermion(rng, (B * k, *L, 4, 3))
     ctx = nat.Context(L, 3, B, 0.04, 1.0, bc)
     pc = nat.PREC[prec]
-    ctx.set_tuning(pc, True, 1, per_thread, True)
+    ctx.set_tuning(pc, True, 1, per_thread, True, slab, cols_outer)
     assert ctx.get_tuning(pc, k)["rhs_per_thread"] == per_thread
+    assert ctx.get_tuning(pc, k)["slab"] == slab
+    assert ctx.get_tuning(pc, k)["cols_outer"] == cols_outer
     g = nat.Gauge(ctx, torch.from_numpy(U).cuda(), nat.WL_LAYOUT_BMXYIJ)
     fi, fo = nat.Field(ctx, pc, B * k), nat.Field(ctx, pc, B * k)
     fi.import_(torch.from_numpy(psi).cuda())
