@@ -168,10 +168,11 @@ def main():
             dslash[f"{prec}_r{recon}"] = {
                 "us": round(us, 2), "GB/s": round(gbs, 1), "frac_hbm_peak": round(gbs / PEAK_HBM_GBS, 4),
                 "GFLOP/s": round(FLOP_PER_SITE * V / (us * 1e-6) / 1e9, 1), "bytes_per_site": bps,
-                "march": t1["best"], "xcd_remap": t1["xcd_remap"], "slab": t1["slab"],
+                "march": t1["best"], "xcd_remap": t1["xcd_remap"], "slab": t1["slab"], "block_shape": t1["block_shape"],
                 "tune_us": {m: round(t, 2) for m, t in t1["us"].items()},
             }
-            log(f"dslash {prec} r{recon}: {us:.1f} us {gbs:.0f} GB/s march {t1['best']} slab {t1['slab']}")
+            log(f"dslash {prec} r{recon}: {us:.1f} us {gbs:.0f} GB/s march {t1['best']} slab {t1['slab']} "
+                f"box {t1['block_shape']}")
             del fi, fo
 
         if args.profile_only:
@@ -207,7 +208,7 @@ def main():
             gbs = bps * V / (best[1] * 1e-6) / 1e9
             dslash_k[f"{prec}_r{recon}"] = {
                 "nrhs": k, "rhs_per_thread": best[0], "march": tk["march"], "xcd_remap": tk["xcd_remap"],
-                "slab": tk["slab"],
+                "slab": tk["slab"], "block_shape": tk["block_shape"],
                 "us": round(best[1], 2), "us_per_rhs": round(best[1] / k, 2),
                 "GB/s": round(gbs, 1), "frac_hbm_peak": round(gbs / PEAK_HBM_GBS, 4), "bytes_per_site": bps,
                 "tune_us": tried,

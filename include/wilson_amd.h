@@ -147,6 +147,17 @@ typedef struct wl_tuning {
   int cols_outer;     /* several spinor columns: 0 = the columns of one tile run back to
                          back (links shared in L2); 1 = one column group sweeps the
                          lattice before the next (spinor neighbours shared in L2) */
+  int block_shape;    /* one-site-per-thread kernels: 0 = flat / slab order; k = 1..8 =
+                         each workgroup covers a compact 4D box of 256 sites (preset k,
+                         x3 extent >= 8 so loads stay whole cache lines), which keeps
+                         about half the neighbour loads inside the workgroup; boxes are
+                         ordered x3-fastest.  Presets (b0,b1,b2,b3): 1 (8,4,1,8),
+                         2 (4,4,2,8), 3 (2,4,4,8), 4 (4,2,4,8), 5 (8,2,2,8),
+                         6 (16,2,1,8), 7 (4,4,1,16), 8 (8,2,1,16); each b_mu must
+                         divide L_mu.  Takes precedence over slab. */
+  int blocks_per_cu;  /* 0 = as many resident workgroups per CU as registers allow;
+                         n > 0 = at most n (by reserving LDS), which narrows the window
+                         of lattice sites in flight on each XCD to what its L2 holds */
 } wl_tuning;
 /* wl_ctx_get_tuning reports the values in effect for nrhs right-hand sides. */
 int wl_ctx_set_tuning(wl_ctx* ctx, int prec, int multi_rhs, const wl_tuning* t);

@@ -5,8 +5,9 @@ field with HIP events on the caller's stream and installs the fastest in the
 context.  The tunable is the number of x0 sites each thread marches over
 (``march``: 1 selects the one-site-per-thread kernel), for multi-RHS
 launches the number of right-hand sides sharing each link load in a thread,
-the site order of the one-site kernels (flat or x1 slabs walked x0-fastest)
-and the workgroup order (XCD-contiguous or hardware order).
+the site order of the one-site kernels (flat, x1 slabs walked x0-fastest, or
+compact 4D workgroup boxes) and the workgroup order (XCD-contiguous or
+hardware order).
 """
 
 from __future__ import annotations
@@ -39,7 +40,7 @@ def time_dslash(ctx: nat.Context, gauge: nat.Gauge, prec: int, recon: int, fin: 
 def tune(ctx: nat.Context, gauge: nat.Gauge, prec: int, recon: int, fin: nat.Field, fout: nat.Field,
          nrhs: int = 1, marches: Iterable[int] = (1, 2, 4, 8), reps: int = 20,
          stream: Optional[torch.cuda.Stream] = None, slabs: Iterable[int] = (0, 2, 4, 8),
-         cols_outer: Iterable[bool] = (False,)) -> Dict[str, object]:
+         cols_outer: Iterable[bool] = (False,), boxes: Iterable[int] = (1, 2, 7)) -> Dict[str, object]:
     """Pick the fastest launch shape for (prec, recon) at nrhs right-hand sides
     per configuration and install it.  Candidates: every march length dividing
     L0 with one right-hand side per thread, plus 2 and 4 right-hand sides per
@@ -53,30 +54,49 @@ def tune(ctx: nat.Context, gauge: nat.Gauge, prec: int, recon: int, fin: nat.Fie
     L0, L1 = ctx.lattice_shape[0], ctx.lattice_shape[1]
     multi = nrhs > 1
     gauge.prepare(prec, recon)
-    slab_c = [s for s in slabs if s == 0 or (s < L1 and L1 % s == 0)]
+    # site orders of the one-site kernels: ("s", slab) or ("b", box preset)
+    orders1 = [("s", s) for s in slabs if s == 0 or (s < L1 and L1 % s == 0)]
+    orders1 += [("b", b) for b in boxes if _box_fits(ctx.lattice_shape, b)]
     shapes = []
     for m in marches:
         if L0 % m:
             continue
-        shapes += [(m, 1, sl) for sl in (slab_c if m == 1 else [0])]
+        shapes += [(m, 1, o) for o in (orders1 if m == 1 else [("s", 0)])]
     if multi:
-        shapes += [(1, k, sl) for k in (2, 4) if nrhs % k == 0 for sl in slab_c]
-    orders = [bool(c) for c in cols_outer] if multi else [False]
+        shapes += [(1, k, o) for k in (2, 4) if nrhs % k == 0 for o in orders1]
+    corders = [bool(c) for c in cols_outer] if multi else [False]
+
+    def install(m, k, r, o, co):
+        ctx.set_tuning(prec, multi, m, k if multi else 0, bool(r), o[1] if o[0] == "s" else 0, co, 0,
+                       o[1] if o[0] == "b" else 0)
+
     times = {}
-    for co in orders:
+    for co in corders:
         for r in (1, 0):
-            for m, k, sl in shapes:
-                ctx.set_tuning(prec, multi, m, k if multi else 0, bool(r), sl, co)
-                times[(m, k, r, sl, co)] = time_dslash(ctx, gauge, prec, recon, fin, fout, nrhs, reps, stream)
+            for m, k, o in shapes:
+                install(m, k, r, o, co)
+                times[(m, k, r, o, co)] = time_dslash(ctx, gauge, prec, recon, fin, fout, nrhs, reps, stream)
     best = min(times, key=times.get)
-    ctx.set_tuning(prec, multi, best[0], best[1] if multi else 0, bool(best[2]), best[3], best[4])
-    return {"march": best[0], "rhs_per_thread": best[1], "xcd_remap": best[2], "slab": best[3],
-            "cols_outer": best[4],
-            "us": {f"m{m}k{k}x{r}s{sl}" + ("c" if co else ""): t for (m, k, r, sl, co), t in times.items()}}
+    install(*best)
+    m, k, r, o, co = best
+    return {"march": m, "rhs_per_thread": k, "xcd_remap": r, "slab": o[1] if o[0] == "s" else 0,
+            "block_shape": o[1] if o[0] == "b" else 0, "cols_outer": co,
+            "us": {f"m{m}k{k}x{r}{o[0]}{o[1]}" + ("c" if co else ""): t for (m, k, r, o, co), t in times.items()}}
+
+
+# 4D workgroup box presets (log2 extents per direction), as in the library
+BOX_SHAPES = {1: (3, 2, 0, 3), 2: (2, 2, 1, 3), 3: (1, 2, 2, 3), 4: (2, 1, 2, 3), 5: (3, 1, 1, 3),
+              6: (4, 1, 0, 3), 7: (2, 2, 0, 4), 8: (3, 1, 0, 4)}
+
+
+def _box_fits(L, b: int) -> bool:
+    sh = BOX_SHAPES.get(int(b))
+    return sh is not None and all(int(L[mu]) % (1 << sh[mu]) == 0 for mu in range(4))
 
 
 def tune_march(ctx, gauge, prec, recon, fin, fout, nrhs: int = 1, candidates=(1, 2, 4, 8), reps: int = 20,
                stream=None) -> Dict[str, object]:
     """Single-RHS form of `tune`: {"best": march, "xcd_remap": r, "us": ...}."""
     t = tune(ctx, gauge, prec, recon, fin, fout, nrhs, candidates, reps, stream)
-    return {"best": t["march"], "xcd_remap": t["xcd_remap"], "slab": t["slab"], "us": t["us"]}
+    return {"best": t["march"], "xcd_remap": t["xcd_remap"], "slab": t["slab"], "block_shape": t["block_shape"],
+            "us": t["us"]}

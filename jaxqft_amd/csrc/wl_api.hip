@@ -110,6 +110,10 @@ struct wl_ctx {
   int xcd_remap[2][3] = {{1, 1, 1}, {1, 1, 1}};  // XCD-contiguous workgroup order
   int slab[2][3] = {{0, 0, 0}, {0, 0, 0}};       // x1 extent of a slab (0 = flat site order)
   int cols_outer[2][3] = {{0, 0, 0}, {0, 0, 0}}; // column group is the slowest workgroup index
+  int blocks_per_cu[2][3] = {{0, 0, 0}, {0, 0, 0}};  // resident workgroup cap (0 = none)
+  int block_shape[2][3] = {{0, 0, 0}, {0, 0, 0}};    // 4D box preset of one-site kernels (0 = none)
+  int lds_per_cu = 0;     // bytes of LDS per CU
+  int lds_per_block = 0;  // largest LDS allocation of one workgroup
 };
 
 struct wl_gauge {
@@ -212,7 +216,10 @@ int wl_ctx_create(const wl_geom* geom, const wl_params* params, wl_ctx** out) {
   c->g.stride[0] = geom->L[3] * geom->L[2] * geom->L[1];
   c->g.V = V;
   c->g.Vp = c->Vp;
-  if (hipGetDevice(&c->device) != hipSuccess) {
+  if (hipGetDevice(&c->device) != hipSuccess ||
+      hipDeviceGetAttribute(&c->lds_per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, c->device) !=
+          hipSuccess ||
+      hipDeviceGetAttribute(&c->lds_per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) != hipSuccess) {
     delete c;
     return fail(WL_ERR_HIP, "no HIP device");
   }
@@ -236,6 +243,18 @@ int wl_ctx_set_mass(wl_ctx* c, double mass) {
   return WL_OK;
 }
 
+// 4D workgroup boxes (b0, b1, b2, b3), product NT = 256, as log2 extents
+static const int kBoxShapes[9][4] = {
+    {0, 0, 0, 0}, {3, 2, 0, 3}, {2, 2, 1, 3}, {1, 2, 2, 3}, {2, 1, 2, 3},
+    {3, 1, 1, 3}, {4, 1, 0, 3}, {2, 2, 0, 4}, {3, 1, 0, 4}};
+
+static bool box_fits(const wl_ctx* c, int shape) {
+  if (shape <= 0 || shape > 8) return false;
+  for (int mu = 0; mu < 4; ++mu)
+    if (c->geom.L[mu] % (1 << kBoxShapes[shape][mu]) != 0) return false;
+  return true;
+}
+
 int wl_ctx_set_tuning(wl_ctx* c, int prec, int multi_rhs, const wl_tuning* t) {
   CHK(check_ctx(c));
   if (!t) return fail(WL_ERR_BAD_ARG, "null tuning");
@@ -244,6 +263,11 @@ int wl_ctx_set_tuning(wl_ctx* c, int prec, int multi_rhs, const wl_tuning* t) {
     return fail(WL_ERR_BAD_ARG, "march %d must be 0 (automatic) or divide L0=%d", t->march, c->geom.L[0]);
   if (t->rhs_per_thread != 0 && t->rhs_per_thread != 1 && t->rhs_per_thread != 2 && t->rhs_per_thread != 4)
     return fail(WL_ERR_BAD_ARG, "rhs_per_thread must be 0 (automatic), 1, 2 or 4");
+  if (t->block_shape < 0 || t->block_shape > 8 || (t->block_shape > 0 && !box_fits(c, t->block_shape)))
+    return fail(WL_ERR_BAD_ARG, "block_shape %d: must be 0 or a preset 1..8 whose extents divide the lattice",
+                t->block_shape);
+  if (t->blocks_per_cu < 0 || t->blocks_per_cu > 32)
+    return fail(WL_ERR_BAD_ARG, "blocks_per_cu must be 0 (no cap) .. 32");
   if (t->slab < 0 || (t->slab > 0 && c->geom.L[1] % t->slab != 0))
     return fail(WL_ERR_BAD_ARG, "slab %d must be 0 (flat) or divide L1=%d", t->slab, c->geom.L[1]);
   std::lock_guard<std::mutex> lk(c->mu);
@@ -252,6 +276,8 @@ int wl_ctx_set_tuning(wl_ctx* c, int prec, int multi_rhs, const wl_tuning* t) {
   c->xcd_remap[m][prec] = t->xcd_remap ? 1 : 0;
   c->slab[m][prec] = t->slab;
   c->cols_outer[m][prec] = t->cols_outer ? 1 : 0;
+  c->blocks_per_cu[m][prec] = t->blocks_per_cu;
+  c->block_shape[m][prec] = t->block_shape;
   if (multi_rhs) c->mrhs[prec] = t->rhs_per_thread;
   return WL_OK;
 }
@@ -267,6 +293,8 @@ int wl_ctx_get_tuning(wl_ctx* c, int prec, int nrhs, wl_tuning* t) {
   t->xcd_remap = c->xcd_remap[m][prec];
   t->slab = t->march == 1 ? c->slab[m][prec] : 0;
   t->cols_outer = c->cols_outer[m][prec];
+  t->blocks_per_cu = c->blocks_per_cu[m][prec];
+  t->block_shape = t->march == 1 ? c->block_shape[m][prec] : 0;
   return WL_OK;
 }
 
@@ -362,10 +390,30 @@ static void set_launch_shape(const wl_ctx* c, DslashArgs& a, int prec) {
   a.march = a.mrhs > 1 ? 1 : march_for(c, prec, a.cols_per_cfg);
   a.xcd_remap = c->xcd_remap[a.cols_per_cfg > 1 ? 1 : 0][prec];
   a.slab = a.march == 1 ? c->slab[a.cols_per_cfg > 1 ? 1 : 0][prec] : 0;
+  {
+    const int bs = a.march == 1 ? c->block_shape[a.cols_per_cfg > 1 ? 1 : 0][prec] : 0;
+    a.box = box_fits(c, bs) ? 1 : 0;
+    for (int mu = 0; mu < 4; ++mu) {
+      a.bsh[mu] = a.box ? kBoxShapes[bs][mu] : 0;
+      a.bn[mu] = c->geom.L[mu] >> a.bsh[mu];
+    }
+    if (a.box) a.slab = 0;
+  }
   a.cols_outer = c->cols_outer[a.cols_per_cfg > 1 ? 1 : 0][prec];
+  {
+    // LDS reservation that leaves room for at most n resident workgroups per CU
+    const int n = c->blocks_per_cu[a.cols_per_cfg > 1 ? 1 : 0][prec];
+    long long b = 0;
+    if (n > 0 && c->lds_per_cu > 0) {
+      b = (long long)c->lds_per_cu / n - 1024;  // the kernels' own reduction scratch is well under 1 KiB
+      if (b > c->lds_per_block) b = c->lds_per_block;
+      if (b < 0) b = 0;
+    }
+    a.lds_bytes = (int)b;
+  }
   a.slab_tiles = a.slab ? (int)(((long long)a.slab * c->geom.L[2] * c->geom.L[3] + NT - 1) / NT) : 0;
   a.nseg = c->geom.L[0] / a.march;
-  a.ntiles = a.march == 1 ? site_tiles(c, a.slab) : nchunk_of(c) * a.nseg;
+  a.ntiles = a.march == 1 ? (a.box ? (int)(c->V / NT) : site_tiles(c, a.slab)) : nchunk_of(c) * a.nseg;
 }
 
 static void fill_dslash_common(const wl_ctx* c, DslashArgs& a, int prec) {

@@ -42,7 +42,12 @@ __device__ __forceinline__ void store_nt(void* base, size_t V, size_t site, cons
 // neighbours are read by the tiles of the adjacent slabs moments before or
 // after it, and each XCD (contiguous tile range) sweeps x0 over its own x1
 // range.  `valid` is false for the padding threads of a slab's last tile.
-__device__ __forceinline__ size_t site_of_tile(const DslashArgs& a, int tile, bool& valid) {
+//
+// Box order: tile t is the 4D box (t0, t1, t2, t3) of extents 2^bsh (t3
+// fastest); thread (l0, l1, l2, l3) of it, l3 fastest, so a wavefront reads
+// whole 128-byte lines (x3 extent >= 8) and about half of the neighbours of
+// a box lie inside it.  site_of_tile also returns the site's coordinates.
+__device__ __forceinline__ size_t site_of_tile_linear(const DslashArgs& a, int tile, bool& valid) {
   if (a.slab == 0) {
     const size_t site = (size_t)tile * NT + threadIdx.x;
     valid = site < (size_t)a.g.V;
@@ -59,6 +64,43 @@ __device__ __forceinline__ size_t site_of_tile(const DslashArgs& a, int tile, bo
   const unsigned q = ii / plane;
   return (size_t)x0 * (size_t)a.g.stride[0] + (size_t)(x1b * a.slab + (int)q) * plane + (ii - q * plane);
 }
+
+__device__ __forceinline__ size_t site_of_tile(const DslashArgs& a, int tile, bool& valid, int (&x)[4]) {
+  if (a.box) {
+    int t = tile;
+    const int t3 = t % a.bn[3];
+    t /= a.bn[3];
+    const int t2 = t % a.bn[2];
+    t /= a.bn[2];
+    const int t1 = t % a.bn[1];
+    const int t0 = t / a.bn[1];
+    unsigned l = threadIdx.x;
+    const unsigned l3 = l & ((1u << a.bsh[3]) - 1u);
+    l >>= a.bsh[3];
+    const unsigned l2 = l & ((1u << a.bsh[2]) - 1u);
+    l >>= a.bsh[2];
+    const unsigned l1 = l & ((1u << a.bsh[1]) - 1u);
+    const unsigned l0 = l >> a.bsh[1];
+    const unsigned x0 = ((unsigned)t0 << a.bsh[0]) + l0, x1 = ((unsigned)t1 << a.bsh[1]) + l1;
+    const unsigned x2 = ((unsigned)t2 << a.bsh[2]) + l2, x3 = ((unsigned)t3 << a.bsh[3]) + l3;
+    valid = true;
+    x[0] = (int)x0;
+    x[1] = (int)x1;
+    x[2] = (int)x2;
+    x[3] = (int)x3;
+    return (((size_t)x0 * a.g.L[1] + x1) * a.g.L[2] + x2) * a.g.L[3] + x3;
+  }
+  const size_t s = site_of_tile_linear(a, tile, valid);
+  size_t rem = s;
+  x[3] = rem % a.g.L[3];
+  rem /= a.g.L[3];
+  x[2] = rem % a.g.L[2];
+  rem /= a.g.L[2];
+  x[1] = rem % a.g.L[1];
+  x[0] = rem / a.g.L[1];
+  return s;
+}
+
 
 template <int EPI>
 struct EpiAccum {
@@ -133,23 +175,12 @@ __global__ __launch_bounds__(NT) void k_dslash_site(DslashArgs a) {
 
   const size_t V = (size_t)a.g.Vp;  // plane stride
   bool valid;
-  const size_t s = site_of_tile(a, tile, valid);
+  int x[4];  // coordinates (last axis fastest)
+  const size_t s = site_of_tile(a, tile, valid, x);
 
   const int cfg = col / a.cols_per_cfg;
   const char* Ucfg = reinterpret_cast<const char*>(a.U) + (size_t)cfg * a.cfg_bytes;
   const char* inb = reinterpret_cast<const char*>(a.in) + (size_t)col * a.col_bytes;
-
-  // coordinates (last axis fastest)
-  int x[4];
-  {
-    size_t rem = s;
-    x[3] = rem % a.g.L[3];
-    rem /= a.g.L[3];
-    x[2] = rem % a.g.L[2];
-    rem /= a.g.L[2];
-    x[1] = rem % a.g.L[1];
-    x[0] = rem / a.g.L[1];
-  }
 
   cx<T> acc[4 * NC];
 #pragma unroll
@@ -207,20 +238,10 @@ __global__ __launch_bounds__(NT) void k_dslash_mrhs(DslashArgs a) {
 
   const size_t V = (size_t)a.g.Vp;  // plane stride
   bool valid;
-  const size_t s = site_of_tile(a, tile, valid);
+  int x[4];
+  const size_t s = site_of_tile(a, tile, valid, x);
   const int cfg = col0 / a.cols_per_cfg;
   const char* Ucfg = reinterpret_cast<const char*>(a.U) + (size_t)cfg * a.cfg_bytes;
-
-  int x[4];
-  {
-    size_t rem = s;
-    x[3] = rem % a.g.L[3];
-    rem /= a.g.L[3];
-    x[2] = rem % a.g.L[2];
-    rem /= a.g.L[2];
-    x[1] = rem % a.g.L[1];
-    x[0] = rem / a.g.L[1];
-  }
   cx<T> acc[K][4 * NC];
 #pragma unroll
   for (int k = 0; k < K; ++k)

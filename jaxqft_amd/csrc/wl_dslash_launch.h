@@ -8,15 +8,15 @@ namespace wl {
 template <int P, int NC, int RECON, bool DAG, bool DIAG, bool G5, int EPI>
 static void launch_k(const DslashArgs& a, dim3 grid, hipStream_t stream) {
   if (NC == 3 && a.mrhs == 4)
-    hipLaunchKernelGGL((k_dslash_mrhs<P, NC, RECON, DAG, DIAG, G5, EPI, 4>), dim3(a.ntiles * (a.ncol / 4)), dim3(NT), 0,
-                       stream, a);
+    hipLaunchKernelGGL((k_dslash_mrhs<P, NC, RECON, DAG, DIAG, G5, EPI, 4>), dim3(a.ntiles * (a.ncol / 4)), dim3(NT),
+                       a.lds_bytes, stream, a);
   else if (NC == 3 && a.mrhs == 2)
-    hipLaunchKernelGGL((k_dslash_mrhs<P, NC, RECON, DAG, DIAG, G5, EPI, 2>), dim3(a.ntiles * (a.ncol / 2)), dim3(NT), 0,
-                       stream, a);
+    hipLaunchKernelGGL((k_dslash_mrhs<P, NC, RECON, DAG, DIAG, G5, EPI, 2>), dim3(a.ntiles * (a.ncol / 2)), dim3(NT),
+                       a.lds_bytes, stream, a);
   else if (a.march == 1)
-    hipLaunchKernelGGL((k_dslash_site<P, NC, RECON, DAG, DIAG, G5, EPI>), grid, dim3(NT), 0, stream, a);
+    hipLaunchKernelGGL((k_dslash_site<P, NC, RECON, DAG, DIAG, G5, EPI>), grid, dim3(NT), a.lds_bytes, stream, a);
   else
-    hipLaunchKernelGGL((k_dslash<P, NC, RECON, DAG, DIAG, G5, EPI>), grid, dim3(NT), 0, stream, a);
+    hipLaunchKernelGGL((k_dslash<P, NC, RECON, DAG, DIAG, G5, EPI>), grid, dim3(NT), a.lds_bytes, stream, a);
 }
 
 // op: 0 D, 1 D^dag, 2 gamma5 D, 3 K, 4 K^dag.  The reducing epilogues exist for

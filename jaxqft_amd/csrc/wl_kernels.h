@@ -91,7 +91,11 @@ struct DslashArgs {
   int mrhs;               // right-hand sides per thread (1, 2 or 4; > 1 implies one site per thread)
   int slab;               // one site per thread: 0 flat order, else x1 extent of a slab (x0-fastest slabs)
   int slab_tiles;         // tiles per slab
+  int box;                // 1: workgroups cover compact 4D boxes (bsh, bn below); overrides slab
+  int bsh[4];             // log2 of the box extent per direction
+  int bn[4];              // boxes per direction (L_mu >> bsh_mu)
   int cols_outer;         // workgroup index = column (group) * ntiles + tile instead of tile * ncol + column
+  int lds_bytes;          // dynamic LDS reserved per workgroup (caps resident workgroups; unused by the code)
   double bc_re[4], bc_im[4];
   double m0;              // diagonal coefficient (used when m0_from_state == 0)
   double r;               // Wilson r

@@ -144,11 +144,11 @@ def test_multi_rhs_same_alone_and_in_the_mix(prec):
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32", "f16"])
-@pytest.mark.parametrize("march,slab", [(2, 0), (8, 0), (1, 2)])
-def test_cg_with_marching_stencil(prec, march, slab):
+@pytest.mark.parametrize("march,slab,box", [(2, 0, 0), (8, 0, 0), (1, 2, 0), (1, 0, 1)])
+def test_cg_with_marching_stencil(prec, march, slab, box):
     """The fused CG epilogues of the marching kernel and of the x1-slab site
     order drive CG to the same FP64 true residual."""
-    print(f"[gpu] cg march={march} slab={slab} {prec}", flush=True)
+    print(f"[gpu] cg march={march} slab={slab} box={box} {prec}", flush=True)
     from jaxqft_amd import _native as nat
     from jaxqft_amd import fields
     from jaxqft_amd.su3_wilson_nf2 import SU3WilsonNf2
@@ -160,7 +160,7 @@ def test_cg_with_marching_stencil(prec, march, slab):
     th = SU3WilsonNf2(lattice_shape=L, mass=0.05, fermion_bc=(1, 1, 1, -1), dtype=np.complex128, cg_tol=1e-10,
                       cg_maxiter=4000, solver_precision=prec)
     for p in (nat.WL_F64, nat.WL_F32, nat.WL_F16):
-        th._context().set_tuning(p, False, march, 0, True, slab)
+        th._context().set_tuning(p, False, march, 0, True, slab, False, 0, box)
     x = th.solve_normal(U, b)
     assert th.last_solve_stats[0]["status"] == "converged"
     assert true_resid(L, 3, 0.05, (1, 1, 1, -1), U, b, x) <= 1e-10

@@ -179,6 +179,51 @@ def test_march_variants_match_oracle(march, slab, nc):
     assert rel(th.apply_offdiag(U, psi), ora.apply_offdiag(U, psi)) < TOL
 
 
+@pytest.mark.parametrize("box", [1, 2, 3, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-13), ("f16", 3e-3)])
+def test_box_orders_match_oracle(box, prec, tol):
+    """Every 4D workgroup box preset (16 x 8 x 4 x 16 lattice, so all fit) gives
+    the oracle's D and D^dag, single-RHS and three right-hand sides."""
+    print(f"[gpu] box={box} {prec}", flush=True)
+    import torch
+
+    from jaxqft_amd import _native as nat
+    from jaxqft_amd import fields
+    from oracle.wilson_oracle import WilsonOracle
+
+    L = (16, 8, 4, 16)
+    bc = (1, -1, 1, -1)
+    rng = fields.rng_for(1400 + box)
+    U = fields.su3_hot_start(rng, 1, L, 0.5)
+    psi = fields.random_fermion(rng, (3, *L, 4, 3))
+    ctx = nat.Context(L, 3, 1, 0.05, 1.0, bc)
+    pc = nat.PREC[prec]
+    ctx.set_tuning(pc, False, 1, 0, True, 0, False, 0, box)
+    ctx.set_tuning(pc, True, 1, 1, True, 0, False, 0, box)
+    assert ctx.get_tuning(pc, 1)["block_shape"] == box and ctx.get_tuning(pc, 3)["block_shape"] == box
+    g = nat.Gauge(ctx, torch.from_numpy(U).cuda(), nat.WL_LAYOUT_BMXYIJ)
+    ora = WilsonOracle(L, 3, 0.05, fermion_bc=bc)
+    for nrhs in (1, 3):
+        fi, fo = nat.Field(ctx, pc, nrhs), nat.Field(ctx, pc, nrhs)
+        fi.import_(torch.from_numpy(psi[:nrhs].copy()).cuda())
+        out = torch.empty((nrhs, *L, 4, 3), dtype=torch.complex128, device="cuda")
+        for op, fn in ((nat.WL_OP_D, ora.apply_D), (nat.WL_OP_DDAG, ora.apply_Ddag)):
+            nat.check(ctx._lib.wl_dslash(ctx.h, g.h, op, 18, fi.h, fo.h, nrhs, nat.c_void_p(nat.current_stream())))
+            fo.export(out)
+            got = out.cpu().numpy()
+            for j in range(nrhs):
+                assert rel(got[j:j + 1], fn(U, psi[j:j + 1])) < tol, (op, nrhs, j)
+
+
+def test_box_shape_rejected_when_it_does_not_fit():
+    """A box preset whose extents do not divide the lattice is refused."""
+    from jaxqft_amd import _native as nat
+
+    ctx = nat.Context((8, 4, 6, 4), 3, 1, 0.05, 1.0, (1, 1, 1, -1))
+    with pytest.raises(RuntimeError, match="block_shape"):
+        ctx.set_tuning(nat.WL_F64, False, 1, 0, True, 0, False, 0, 1)
+
+
 @pytest.mark.parametrize("prec,recon,tol", [("f64", 12, 1e-13), ("f32", 18, 2e-6), ("f32", 12, 2e-6),
                                             ("f16", 18, 3e-3), ("f16", 12, 3e-3)])
 @pytest.mark.parametrize("march", [1, 4])
