@@ -68,9 +68,21 @@ extern "C" {
 #define WL_OP_KDAG 4
 #define WL_OP_DDAGD 5
 
+/* even-odd (checkerboard) operators of wl_eo_apply; K = D - m0 (the hopping
+ * term), P_e / P_o the parity projections, S_e = m0 - K_eo K_oe / m0 and
+ * S_o = m0 - K_oe K_eo / m0 (normalized: divided by m0) */
+#define WL_EO_K_OE 0        /* P_o K P_e          su3_wilson_nf2.py:1057-1061 */
+#define WL_EO_K_EO 1        /* P_e K P_o          :1063-1067 */
+#define WL_EO_KDAG_OE 2     /* P_o K^dag P_e      :1069-1073 */
+#define WL_EO_KDAG_EO 3     /* P_e K^dag P_o      :1075-1079 */
+#define WL_EO_SCHUR_E 4     /* S_e                :1112-1121 */
+#define WL_EO_SCHUR_E_DAG 5 /* S_e^dag            :1138-1146 */
+#define WL_EO_SCHUR_O 6     /* S_o                :1148-1156 */
+#define WL_EO_SCHUR_O_DAG 7 /* S_o^dag (no reference counterpart; completes the set) */
+
 /* CG stopping rule */
 #define WL_STOP_TRUE_RESIDUAL 0 /* ||b - A x|| <= tol ||b||, recomputed in FP64 */
-#define WL_STOP_ITERATED 1      /* reference rule: iterated residual only (App. jax cg) */
+#define WL_STOP_ITERATED 1      /* reference rule: iterated residual only (jax.scipy.sparse.linalg.cg) */
 
 /* per-column solve status */
 #define WL_STATUS_RUNNING (-1)
@@ -190,6 +202,23 @@ int wl_cg_normal(wl_ctx* ctx, wl_gauge* g, const void* b, void* x, const void* x
 /* Same on resident FP64 fields; x is used as the initial guess when use_x0. */
 int wl_cg_normal_field(wl_ctx* ctx, wl_gauge* g, const wl_field* b, wl_field* x, int use_x0, int nrhs,
                        const wl_solve_params* sp, wl_solve_stats* stats, void* stream);
+
+/* Even-odd operators (replaces SU3WilsonNf2.apply_K_oe / apply_K_eo /
+ * apply_eo_schur_even / apply_eo_schur_even_dagger / apply_eo_schur_odd,
+ * su3_wilson_nf2.py:1057-1156, and the compact blocks of :934-988).  in, out:
+ * full-lattice reference-layout arrays (batch * nrhs columns); the sites of the
+ * source parity of in are read and out is written everywhere (zero on the
+ * other parity).  normalized applies to the Schur operators.  Needs every
+ * lattice extent even. */
+int wl_eo_apply(wl_ctx* ctx, wl_gauge* g, int op, int normalized, const void* in, void* out, int nrhs, int dtype,
+                void* stream);
+/* Solve S_e^dag S_e x_e = P_e b with the normalized even Schur complement
+ * (replaces solve_eop_even_normal / solve_eop_even_normal_with_guess,
+ * su3_wilson_nf2.py:1373-1392); same solver parameters and statistics as
+ * wl_cg_normal, the true residual being that of the Schur normal equations.
+ * b, x, x0: full-lattice arrays; even sites are read, x is zero on odd sites. */
+int wl_cg_eo_normal(wl_ctx* ctx, wl_gauge* g, const void* b, void* x, const void* x0, int nrhs, int dtype,
+                    const wl_solve_params* sp, wl_solve_stats* stats, void* stream);
 
 /* Continuous-batching solve engine (no counterpart in the reference; it serves
  * the point-source propagator loop of core/measurements.py:277-371 and request

@@ -33,6 +33,14 @@ hipError_t pack_gauge_f64(int nc, const void* src, int dtype, int layout, void* 
 hipError_t convert_gauge(int prec, int nc, int recon, const void* src, void* dst, long long V, long long Vp, int ncfg,
                          size_t src_cfg_bytes, size_t src_dir_bytes, size_t cfg_bytes, size_t dir_bytes,
                          hipStream_t st);
+hipError_t launch_eo(int prec, int nc, int recon, int dag, int mode, int epi, const DslashArgs& a, hipStream_t stream);
+hipError_t pack_spinor_eo(int prec, int nc, const void* src, int dtype, void* dst, const Geom& g, long long Vh,
+                          long long Vhp, int q, int ncol, size_t col_bytes, hipStream_t st);
+hipError_t unpack_spinor_eo(int prec, int nc, const void* src, void* dst, int dtype, const Geom& g, long long Vhp, int q,
+                            int ncol, size_t col_bytes, hipStream_t st);
+hipError_t convert_gauge_eo(int prec, int nc, int recon, const void* src, void* dst, const Geom& g, long long Vh,
+                            long long Vhp, int ncfg, size_t src_cfg_bytes, size_t src_dir_bytes, size_t cfg_bytes,
+                            size_t dir_bytes, hipStream_t st);
 }  // namespace wl
 
 using namespace wl;
@@ -101,7 +109,9 @@ struct wl_ctx {
   std::mutex mu;
   // workspace
   DevBuf partials, cols;
-  DevBuf ws[10];
+  DevBuf ws[11];
+  long long Vh = 0;   // sites of one parity (0: even-odd operators unavailable, some L odd)
+  long long Vhp = 0;  // plane stride of compact (one-parity) fields
   ColState* host_cols = nullptr;  // pinned mirror
   size_t host_cols_n = 0;
   // stencil launch tuning per [single-RHS / multi-RHS launch][precision]
@@ -135,6 +145,11 @@ static inline size_t cfg_bytes(const wl_ctx* c, int prec, int recon) {
   return 4 * link_dir_bytes(prec, c->geom.nc, recon, (size_t)c->Vp);
 }
 static inline size_t col_bytes_of(const wl_ctx* c, int prec) { return spinor_col_bytes(prec, c->geom.nc, (size_t)c->Vp); }
+// compact one-parity fields
+static inline size_t col_bytes_eo(const wl_ctx* c, int prec) { return spinor_col_bytes(prec, c->geom.nc, (size_t)c->Vhp); }
+static inline size_t cfg_bytes_eo(const wl_ctx* c, int prec, int recon) {
+  return 8 * link_dir_bytes(prec, c->geom.nc, recon, (size_t)c->Vhp);
+}
 
 static int check_ctx(const wl_ctx* c) {
   if (!c) return fail(WL_ERR_BAD_ARG, "null context");
@@ -209,6 +224,12 @@ int wl_ctx_create(const wl_geom* geom, const wl_params* params, wl_ctx** out) {
   c->params = *params;
   c->V = V;
   c->Vp = V + PLANE_PAD;
+  {
+    bool all_even = true;
+    for (int i = 0; i < 4; ++i) all_even = all_even && (geom->L[i] % 2 == 0);
+    c->Vh = all_even ? V / 2 : 0;
+    c->Vhp = all_even ? V / 2 + PLANE_PAD : 0;
+  }
   for (int i = 0; i < 4; ++i) c->g.L[i] = geom->L[i];
   c->g.stride[3] = 1;
   c->g.stride[2] = geom->L[3];
@@ -384,6 +405,34 @@ static int gauge_encoding(wl_gauge* g, int prec, int recon, hipStream_t st, cons
   return WL_OK;
 }
 
+// Links per parity in compact order (the even-odd kernels), built from the
+// FP64 master on first use.
+static int gauge_encoding_eo(wl_gauge* g, int prec, int recon, hipStream_t st, const void** out) {
+  wl_ctx* c = g->ctx;
+  if (c->Vh == 0) return fail(WL_ERR_UNSUPPORTED, "even-odd operators need every lattice extent even");
+  if (c->geom.nc == 1) recon = 18;
+  if (recon != 18 && recon != 12) return fail(WL_ERR_BAD_ARG, "recon must be 18 or 12 (got %d)", recon);
+  const int key = 1000 + prec * 100 + recon;
+  auto it = g->enc.find(key);
+  if (it != g->enc.end()) {
+    *out = it->second.p;
+    return WL_OK;
+  }
+  DevBuf b;
+  const size_t cb = cfg_bytes_eo(c, prec, recon);
+  CHK(b.ensure(cb * g->ncfg));
+  const size_t mcb = cfg_bytes(c, F64, 18);
+  hipError_t e = convert_gauge_eo(prec, c->geom.nc, recon, g->master.p, b.p, c->g, c->Vh, c->Vhp, g->ncfg, mcb, mcb / 4,
+                                  cb, cb / 8, st);
+  if (e != hipSuccess) {
+    b.release();
+    return fail(WL_ERR_HIP, "gauge convert (even-odd): %s", hipGetErrorString(e));
+  }
+  g->enc[key] = b;
+  *out = b.p;
+  return WL_OK;
+}
+
 // launch geometry once ncol and cols_per_cfg are set
 static void set_launch_shape(const wl_ctx* c, DslashArgs& a, int prec) {
   a.mrhs = mrhs_for(c, prec, a.cols_per_cfg);
@@ -428,6 +477,26 @@ static void fill_dslash_common(const wl_ctx* c, DslashArgs& a, int prec) {
   a.march = 1;  // set_launch_shape fills the launch shape once ncol is known
   a.nseg = c->geom.L[0];
   a.ntiles = (int)((c->V + NT - 1) / NT);
+}
+
+// Stencil arguments of the even-odd kernels: compact fields of Vh sites with
+// plane stride Vhp, links per parity.
+static void fill_eo_args(const wl_ctx* c, DslashArgs& a, int prec, const void* U, int recon, int ncol, int nrhs) {
+  fill_dslash_common(c, a, prec);
+  const int rc = (c->geom.nc == 1) ? 18 : recon;
+  a.U = U;
+  a.cfg_bytes = cfg_bytes_eo(c, prec, rc);
+  a.dir_bytes = a.cfg_bytes / 8;
+  a.ncol = ncol;
+  a.cols_per_cfg = nrhs;
+  a.g.Vp = c->Vhp;
+  a.Vh = c->Vh;
+  a.ntiles = (int)((c->Vh + NT - 1) / NT);
+  a.march = 1;
+  a.nseg = c->geom.L[0];
+  a.mrhs = 1;
+  a.xcd_remap = c->xcd_remap[nrhs > 1 ? 1 : 0][prec];
+  a.normalized = 1;
 }
 
 // op on internal fields of precision prec (no gating, no reduction)
@@ -583,35 +652,71 @@ struct Solver {
   int PY;         // precision of the inner solution accumulator
   bool mixed;
   bool fused;     // reliable updates run inside every iteration (device-driven)
+  bool eo = false;  // operator M = the normalized even Schur complement S_e (else M = D)
   size_t cb64, cbP, cbY;
   const void* U64;
   const void* UP;
   void *b, *x;    // F64 caller-side internal fields
   void *rhi, *thi;
   void *r, *p, *t, *y;
+  void *mid64 = nullptr, *midP = nullptr;  // odd-parity intermediates of S_e (even-odd only)
   Reduction red;
 
   void dargs(DslashArgs& a, int prec, const void* U, int recon) {
-    fill_dslash_common(c, a, prec);
-    a.U = U;
-    a.cfg_bytes = cfg_bytes(c, prec, nc == 1 ? 18 : recon);
-    a.dir_bytes = a.cfg_bytes / 4;
-    a.ncol = ncol;
-    a.cols_per_cfg = nrhs;
-    set_launch_shape(c, a, prec);
+    if (eo) {
+      fill_eo_args(c, a, prec, U, recon, ncol, nrhs);
+    } else {
+      fill_dslash_common(c, a, prec);
+      a.U = U;
+      a.cfg_bytes = cfg_bytes(c, prec, nc == 1 ? 18 : recon);
+      a.dir_bytes = a.cfg_bytes / 4;
+      a.ncol = ncol;
+      a.cols_per_cfg = nrhs;
+      set_launch_shape(c, a, prec);
+    }
     a.m0_from_state = 1;
     a.red = red;
   }
   BlasArgs bargs(int gate) {
     BlasArgs a;
     memset(&a, 0, sizeof(a));
-    a.V = c->V;
-    a.Vp = c->Vp;
+    a.V = eo ? c->Vh : c->V;
+    a.Vp = eo ? c->Vhp : c->Vp;
     a.ncol = ncol;
     a.ntiles = ntiles;
     a.gate = gate;
     a.red = red;
     return a;
+  }
+
+  // out = M in (dag = false) or M^dag in, with epilogue epi and reduction
+  // action; d carries the remaining arguments (gate, w, r, y).  The even-odd
+  // operator is two launches through the odd intermediate mid.
+  int apply(DslashArgs d, int prec, int recon, bool dag, const void* in, void* out, void* mid, int epi, int action) {
+    hipError_t e;
+    if (!eo) {
+      d.in = in;
+      d.out = out;
+      d.red.action = action;
+      e = launch_dslash(prec, nc, nc == 1 ? 18 : recon, dag ? WL_OP_DDAG : WL_OP_D, epi, d, st);
+      if (e != hipSuccess) return fail(WL_ERR_HIP, "dslash: %s", hipGetErrorString(e));
+      return WL_OK;
+    }
+    DslashArgs h = d;
+    h.pt = 1;
+    h.in = in;
+    h.out = mid;
+    h.red.action = RA_NONE;
+    e = launch_eo(prec, nc, recon, dag ? 1 : 0, 0, EPI_STORE, h, st);
+    if (e != hipSuccess) return fail(WL_ERR_HIP, "even-odd hop: %s", hipGetErrorString(e));
+    d.pt = 0;
+    d.in = mid;
+    d.ctr = in;
+    d.out = out;
+    d.red.action = action;
+    e = launch_eo(prec, nc, recon, dag ? 1 : 0, 1, epi, d, st);
+    if (e != hipSuccess) return fail(WL_ERR_HIP, "even-odd Schur: %s", hipGetErrorString(e));
+    return WL_OK;
   }
 
   // gated reliable update: x += y; r_hi = b - A x; r = r_hi; p = r + beta p
@@ -627,13 +732,10 @@ struct Solver {
     DslashArgs d;
     dargs(d, F64, U64, 18);
     d.gate = 2;
-    d.in = x; d.out = thi; d.col_bytes = cb64;
-    e = launch_dslash(F64, nc, 18, WL_OP_D, EPI_STORE, d, st);
-    if (e != hipSuccess) return fail(WL_ERR_HIP, "dslash: %s", hipGetErrorString(e));
-    d.in = thi; d.out = rhi; d.w = b;
-    d.red.action = RA_TRUERES;
-    e = launch_dslash(F64, nc, 18, WL_OP_DDAG, EPI_RESID, d, st);
-    if (e != hipSuccess) return fail(WL_ERR_HIP, "dslash: %s", hipGetErrorString(e));
+    d.col_bytes = cb64;
+    CHK(apply(d, F64, 18, false, x, thi, mid64, EPI_STORE, RA_NONE));
+    d.w = b;
+    CHK(apply(d, F64, 18, true, thi, rhi, mid64, EPI_RESID, RA_TRUERES));
     if (mixed) {
       BlasArgs a = bargs(2);
       a.a = r; a.col_bytes_a = cbP;
@@ -662,15 +764,11 @@ struct Solver {
     DslashArgs d;
     dargs(d, P, UP, R);
     d.gate = 1;
-    d.in = p; d.out = t; d.col_bytes = cbP;
-    d.red.action = RA_PAP;
-    e = launch_dslash(P, nc, nc == 1 ? 18 : R, WL_OP_D, EPI_NORM, d, st);
-    if (e != hipSuccess) return fail(WL_ERR_HIP, "dslash: %s", hipGetErrorString(e));
-    d.in = t; d.out = nullptr; d.w = p; d.res = r; d.y = y; d.col_bytes_y = cbY;
-    d.red.action = RA_ITER;
+    d.col_bytes = cbP;
+    CHK(apply(d, P, R, false, p, t, midP, EPI_NORM, RA_PAP));
+    d.w = p; d.res = r; d.y = y; d.col_bytes_y = cbY;
     const int epi = PY == F64 ? EPI_CG64 : (PY == F32 ? EPI_CG32 : EPI_CG16);
-    e = launch_dslash(P, nc, nc == 1 ? 18 : R, WL_OP_DDAG, epi, d, st);
-    if (e != hipSuccess) return fail(WL_ERR_HIP, "dslash: %s", hipGetErrorString(e));
+    CHK(apply(d, P, R, true, t, nullptr, midP, epi, RA_ITER));
     {
       BlasArgs a = bargs(1);
       a.a = p; a.col_bytes_a = cbP;
@@ -698,8 +796,10 @@ static int accum_prec_of(const wl_solve_params* sp, int inner) {
   }
 }
 
+// CG on M^dag M x = b for internal FP64 fields b, x: M = D (full lattice) or,
+// with eo, M = the normalized even Schur complement on compact even fields.
 static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int nrhs, const wl_solve_params* sp,
-                     wl_solve_stats* stats, hipStream_t st) {
+                     wl_solve_stats* stats, hipStream_t st, bool eo = false) {
   const int nc = c->geom.nc;
   const int ncol = c->geom.batch * nrhs;
   if (sp->inner_prec < F64 || sp->inner_prec > F16) return fail(WL_ERR_BAD_ARG, "bad inner precision %d", sp->inner_prec);
@@ -713,17 +813,23 @@ static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int n
   s.nc = nc;
   s.ncol = ncol;
   s.nrhs = nrhs;
-  s.ntiles = (int)((c->V + NT - 1) / NT);
+  s.eo = eo;
+  s.ntiles = (int)(((eo ? c->Vh : c->V) + NT - 1) / NT);
   s.P = sp->inner_prec;
   s.R = (nc == 1) ? 18 : (sp->inner_recon == 12 ? 12 : 18);
   s.mixed = s.P != F64;
   s.fused = s.mixed;
   s.PY = accum_prec_of(sp, s.P);
-  s.cb64 = col_bytes_of(c, F64);
-  s.cbP = col_bytes_of(c, s.P);
-  s.cbY = col_bytes_of(c, s.PY);
-  CHK(gauge_encoding(g, F64, 18, st, &s.U64));
-  CHK(gauge_encoding(g, s.P, s.R, st, &s.UP));
+  s.cb64 = eo ? col_bytes_eo(c, F64) : col_bytes_of(c, F64);
+  s.cbP = eo ? col_bytes_eo(c, s.P) : col_bytes_of(c, s.P);
+  s.cbY = eo ? col_bytes_eo(c, s.PY) : col_bytes_of(c, s.PY);
+  if (eo) {
+    CHK(gauge_encoding_eo(g, F64, 18, st, &s.U64));
+    CHK(gauge_encoding_eo(g, s.P, s.R, st, &s.UP));
+  } else {
+    CHK(gauge_encoding(g, F64, 18, st, &s.U64));
+    CHK(gauge_encoding(g, s.P, s.R, st, &s.UP));
+  }
   s.b = b;
   s.x = x;
   // workspace
@@ -735,6 +841,12 @@ static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int n
   s.thi = c->ws[1].p;
   s.p = c->ws[2].p;
   s.t = c->ws[4].p;
+  if (eo) {
+    CHK(c->ws[3].ensure(s.cbP * ncol));    // odd intermediate, inner precision
+    CHK(c->ws[10].ensure(s.cb64 * ncol));  // odd intermediate, FP64
+    s.midP = c->ws[3].p;
+    s.mid64 = c->ws[10].p;
+  }
   if (s.mixed) {
     CHK(c->ws[5].ensure(s.cbP * ncol));  // r
     CHK(c->ws[6].ensure(s.cbY * ncol));  // y
@@ -948,6 +1060,82 @@ int wl_cg_normal(wl_ctx* c, wl_gauge* g, const void* b, void* x, const void* x0,
   }
   CHK(cg_fields(c, g, c->ws[7].p, c->ws[8].p, x0 != nullptr, nrhs, sp, stats, st));
   e = unpack_spinor(F64, nc, c->ws[8].p, x, dtype, c->V, c->Vp, ncol, cb, st);
+  if (e != hipSuccess) return fail(WL_ERR_HIP, "unpack: %s", hipGetErrorString(e));
+  return WL_OK;
+}
+
+int wl_eo_apply(wl_ctx* c, wl_gauge* g, int op, int normalized, const void* in, void* out, int nrhs, int dtype,
+                void* stream) {
+  CHK(check_ctx(c));
+  if (!g || g->ctx != c) return fail(WL_ERR_BAD_ARG, "gauge handle does not belong to this context");
+  if (!in || !out) return fail(WL_ERR_BAD_ARG, "null field pointer");
+  if (nrhs < 1) return fail(WL_ERR_BAD_ARG, "nrhs must be >= 1");
+  if (dtype != WL_C128 && dtype != WL_C64) return fail(WL_ERR_BAD_ARG, "bad dtype %d", dtype);
+  if (op < WL_EO_K_OE || op > WL_EO_SCHUR_O_DAG) return fail(WL_ERR_BAD_ARG, "bad even-odd op %d", op);
+  if (c->Vh == 0) return fail(WL_ERR_UNSUPPORTED, "even-odd operators need every lattice extent even");
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = (hipStream_t)stream;
+  const int nc = c->geom.nc;
+  const int ncol = c->geom.batch * nrhs;
+  const size_t cb = col_bytes_eo(c, F64);
+  const void* U;
+  CHK(gauge_encoding_eo(g, F64, 18, st, &U));
+  CHK(c->ws[0].ensure(cb * ncol));
+  CHK(c->ws[1].ensure(cb * ncol));
+  CHK(c->ws[2].ensure(cb * ncol));
+  // source parity of the input and target parity of the output
+  const bool hop = op <= WL_EO_KDAG_EO;
+  const bool dag = (op == WL_EO_KDAG_OE || op == WL_EO_KDAG_EO || op == WL_EO_SCHUR_E_DAG || op == WL_EO_SCHUR_O_DAG);
+  int target;
+  if (hop) target = (op == WL_EO_K_OE || op == WL_EO_KDAG_OE) ? 1 : 0;
+  else target = (op == WL_EO_SCHUR_E || op == WL_EO_SCHUR_E_DAG) ? 0 : 1;
+  const int source = hop ? 1 - target : target;
+  hipError_t e = pack_spinor_eo(F64, nc, in, dtype, c->ws[0].p, c->g, c->Vh, c->Vhp, source, ncol, cb, st);
+  if (e != hipSuccess) return fail(WL_ERR_HIP, "pack: %s", hipGetErrorString(e));
+  DslashArgs d;
+  fill_eo_args(c, d, F64, U, 18, ncol, nrhs);
+  d.col_bytes = cb;
+  d.normalized = normalized ? 1 : 0;
+  d.pt = hop ? target : 1 - target;
+  d.in = c->ws[0].p;
+  d.out = hop ? c->ws[1].p : c->ws[2].p;
+  e = launch_eo(F64, nc, 18, dag ? 1 : 0, 0, EPI_STORE, d, st);
+  if (e != hipSuccess) return fail(WL_ERR_HIP, "even-odd hop: %s", hipGetErrorString(e));
+  if (!hop) {
+    d.pt = target;
+    d.in = c->ws[2].p;
+    d.ctr = c->ws[0].p;
+    d.out = c->ws[1].p;
+    e = launch_eo(F64, nc, 18, dag ? 1 : 0, 1, EPI_STORE, d, st);
+    if (e != hipSuccess) return fail(WL_ERR_HIP, "even-odd Schur: %s", hipGetErrorString(e));
+  }
+  e = unpack_spinor_eo(F64, nc, c->ws[1].p, out, dtype, c->g, c->Vhp, target, ncol, cb, st);
+  if (e != hipSuccess) return fail(WL_ERR_HIP, "unpack: %s", hipGetErrorString(e));
+  return WL_OK;
+}
+
+int wl_cg_eo_normal(wl_ctx* c, wl_gauge* g, const void* b, void* x, const void* x0, int nrhs, int dtype,
+                    const wl_solve_params* sp, wl_solve_stats* stats, void* stream) {
+  CHK(check_ctx(c));
+  if (!g || g->ctx != c || !b || !x || !sp) return fail(WL_ERR_BAD_ARG, "null or foreign handle");
+  if (dtype != WL_C128 && dtype != WL_C64) return fail(WL_ERR_BAD_ARG, "bad dtype %d", dtype);
+  if (nrhs < 1) return fail(WL_ERR_BAD_ARG, "nrhs must be >= 1");
+  if (c->Vh == 0) return fail(WL_ERR_UNSUPPORTED, "even-odd operators need every lattice extent even");
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t st = (hipStream_t)stream;
+  const int nc = c->geom.nc;
+  const int ncol = c->geom.batch * nrhs;
+  const size_t cb = col_bytes_eo(c, F64);
+  CHK(c->ws[7].ensure(cb * ncol));
+  CHK(c->ws[8].ensure(cb * ncol));
+  hipError_t e = pack_spinor_eo(F64, nc, b, dtype, c->ws[7].p, c->g, c->Vh, c->Vhp, 0, ncol, cb, st);
+  if (e != hipSuccess) return fail(WL_ERR_HIP, "pack: %s", hipGetErrorString(e));
+  if (x0) {
+    e = pack_spinor_eo(F64, nc, x0, dtype, c->ws[8].p, c->g, c->Vh, c->Vhp, 0, ncol, cb, st);
+    if (e != hipSuccess) return fail(WL_ERR_HIP, "pack: %s", hipGetErrorString(e));
+  }
+  CHK(cg_fields(c, g, c->ws[7].p, c->ws[8].p, x0 != nullptr, nrhs, sp, stats, st, true));
+  e = unpack_spinor_eo(F64, nc, c->ws[8].p, x, dtype, c->g, c->Vhp, 0, ncol, cb, st);
   if (e != hipSuccess) return fail(WL_ERR_HIP, "unpack: %s", hipGetErrorString(e));
   return WL_OK;
 }

@@ -2,6 +2,7 @@
 // translation units wl_dslash_{f64,f32,f16}.hip so they compile in parallel.
 #pragma once
 #include "wl_dslash_kernels.h"
+#include "wl_eo_kernels.h"
 
 namespace wl {
 
@@ -52,6 +53,52 @@ static hipError_t launch_p(int op, int epi, const DslashArgs& a, hipStream_t str
     else return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+// Even-odd hop: dag (K or K^dag), mode 0 (hop) / 1 (Schur combine), epilogue
+// EPI_STORE, or for the combine EPI_NORM (forward), EPI_RESID (FP64) and the CG
+// steps (dagger).
+template <int P, int NC, int RECON>
+static hipError_t launch_eo_p(int dag, int mode, int epi, const DslashArgs& a, hipStream_t stream) {
+  dim3 grid((unsigned)(a.ntiles * a.ncol)), block(NT);
+#define WL_EOL(D, M, E) hipLaunchKernelGGL((k_hop_eo<P, NC, RECON, D, M, E>), grid, block, 0, stream, a)
+  if (epi == EPI_STORE) {
+    if (mode == 0) {
+      if (dag) WL_EOL(true, 0, EPI_STORE);
+      else WL_EOL(false, 0, EPI_STORE);
+    } else {
+      if (dag) WL_EOL(true, 1, EPI_STORE);
+      else WL_EOL(false, 1, EPI_STORE);
+    }
+    return hipGetLastError();
+  }
+  if (mode != 1) return hipErrorInvalidValue;
+  if (epi == EPI_NORM && !dag) {
+    WL_EOL(false, 1, EPI_NORM);
+    return hipGetLastError();
+  }
+  if (!dag) return hipErrorInvalidValue;
+  if constexpr (P == F64) {
+    if (epi == EPI_RESID) WL_EOL(true, 1, EPI_RESID);
+    else if (epi == EPI_CG64) WL_EOL(true, 1, EPI_CG64);
+    else return hipErrorInvalidValue;
+  } else {
+    if (epi == EPI_CG64) WL_EOL(true, 1, EPI_CG64);
+    else if (epi == EPI_CG32) WL_EOL(true, 1, EPI_CG32);
+    else if (epi == EPI_CG16) WL_EOL(true, 1, EPI_CG16);
+    else return hipErrorInvalidValue;
+  }
+#undef WL_EOL
+  return hipGetLastError();
+}
+
+template <int P>
+static hipError_t launch_eo_prec(int nc, int recon, int dag, int mode, int epi, const DslashArgs& a,
+                                 hipStream_t stream) {
+  if (nc == 3)
+    return recon == 12 ? launch_eo_p<P, 3, 12>(dag, mode, epi, a, stream) : launch_eo_p<P, 3, 18>(dag, mode, epi, a, stream);
+  if (nc == 1) return launch_eo_p<P, 1, 18>(dag, mode, epi, a, stream);
+  return hipErrorInvalidValue;
 }
 
 template <int P>

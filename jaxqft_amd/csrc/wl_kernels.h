@@ -102,7 +102,27 @@ struct DslashArgs {
   int m0_from_state;      // take m0 per column from red.cols[col].m0
   int gate;               // 0 all, 1 active columns, 2 columns awaiting a reliable update, 3 column red.only_col
   Reduction red;
+  // even-odd (checkerboard) kernels: compact fields of one parity hold the
+  // sites of that parity in ascending full-lattice order (compact index =
+  // full index / 2 when L3 is even); links are stored per parity the same way.
+  int pt;                 // parity of the target sites
+  int normalized;         // Schur combine: 1 - K K / m0^2 (1) or m0 - K K / m0 (0)
+  long long Vh;           // sites of one parity
+  const void* ctr;        // Schur combine: the field the Schur operator acts on (target parity)
 };
+
+// Full-lattice index of compact index i of parity q (L3 even; see the
+// compact layout above).
+__host__ __device__ inline size_t eo_full_index(const Geom& g, int q, size_t i) {
+  const size_t L3h = (size_t)(g.L[3] >> 1);
+  const size_t row = i / L3h;
+  const size_t j = i - row * L3h;
+  const size_t x2 = row % (size_t)g.L[2];
+  const size_t r2 = row / (size_t)g.L[2];
+  const size_t x1 = r2 % (size_t)g.L[1];
+  const size_t x0 = r2 / (size_t)g.L[1];
+  return row * (size_t)g.L[3] + 2 * j + ((x0 + x1 + x2 + (size_t)q) & 1);
+}
 
 struct BlasArgs {
   long long V;

@@ -130,6 +130,77 @@ __global__ __launch_bounds__(NT) void k_convert_gauge(const void* __restrict__ s
   LinkStore<P, NC>::template store<RECON>(d, Vp, site, u);
 }
 
+// Even-odd layouts.  Spinors: the sites of parity q of a full-lattice
+// reference array, compact (ascending full index).  Unpacking writes the
+// whole full-lattice array: the compact values at parity-q sites and zero at
+// the others (the parity projection of su3_wilson_nf2.py:884-892).
+template <int P, int NC, class S>
+__global__ __launch_bounds__(NT) void k_pack_spinor_eo(const S* __restrict__ src, void* __restrict__ dst, Geom g,
+                                                       long long Vh, long long Vhp, int q, size_t col_bytes) {
+  const size_t i = (size_t)blockIdx.x * NT + threadIdx.x;
+  const int col = blockIdx.y;
+  if (i >= (size_t)Vh) return;
+  using T = typename PT<P>::acc;
+  const size_t site = eo_full_index(g, q, i);
+  const S* s = src + ((size_t)col * g.V + site) * (4 * NC) * 2;
+  cx<T> v[4 * NC];
+#pragma unroll
+  for (int c = 0; c < 4 * NC; ++c) v[c] = {(T)s[2 * c], (T)s[2 * c + 1]};
+  SpinorIO<P, NC>::store(reinterpret_cast<char*>(dst) + col * col_bytes, Vhp, i, v);
+}
+
+template <int P, int NC, class S>
+__global__ __launch_bounds__(NT) void k_unpack_spinor_eo(const void* __restrict__ src, S* __restrict__ dst, Geom g,
+                                                         long long Vhp, int q, size_t col_bytes) {
+  const size_t site = (size_t)blockIdx.x * NT + threadIdx.x;
+  const int col = blockIdx.y;
+  if (site >= (size_t)g.V) return;
+  using T = typename PT<P>::acc;
+  size_t rem = site;
+  const int x3 = (int)(rem % g.L[3]);
+  rem /= g.L[3];
+  const int x2 = (int)(rem % g.L[2]);
+  rem /= g.L[2];
+  const int x1 = (int)(rem % g.L[1]);
+  const int x0 = (int)(rem / g.L[1]);
+  cx<T> v[4 * NC];
+  if (((x0 + x1 + x2 + x3) & 1) == q) {
+    SpinorIO<P, NC>::load(reinterpret_cast<const char*>(src) + col * col_bytes, Vhp, site >> 1, v);
+  } else {
+#pragma unroll
+    for (int c = 0; c < 4 * NC; ++c) v[c] = {T(0), T(0)};
+  }
+  S* d = dst + ((size_t)col * g.V + site) * (4 * NC) * 2;
+#pragma unroll
+  for (int c = 0; c < 4 * NC; ++c) {
+    d[2 * c] = (S)v[c].re;
+    d[2 * c + 1] = (S)v[c].im;
+  }
+}
+
+// Links per parity: dst[cfg][q][mu] holds U_mu at the parity-q sites, compact.
+template <int P, int NC, int RECON>
+__global__ __launch_bounds__(NT) void k_convert_gauge_eo(const void* __restrict__ src, void* __restrict__ dst, Geom g,
+                                                         long long Vh, long long Vhp, size_t src_cfg_bytes,
+                                                         size_t src_dir_bytes, size_t cfg_bytes, size_t dir_bytes) {
+  const size_t i = (size_t)blockIdx.x * NT + threadIdx.x;
+  const int mu = blockIdx.y & 3;
+  const int q = blockIdx.y >> 2;
+  const int cfg = blockIdx.z;
+  if (i >= (size_t)Vh) return;
+  const size_t site = eo_full_index(g, q, i);
+  const double2* s =
+      reinterpret_cast<const double2*>(reinterpret_cast<const char*>(src) + cfg * src_cfg_bytes + mu * src_dir_bytes);
+  cx<double> u[NC][NC];
+  for (int a = 0; a < NC; ++a)
+    for (int b = 0; b < NC; ++b) {
+      double2 t = s[(a * NC + b) * g.Vp + site];
+      u[a][b] = {t.x, t.y};
+    }
+  char* d = reinterpret_cast<char*>(dst) + cfg * cfg_bytes + (q * 4 + mu) * dir_bytes;
+  LinkStore<P, NC>::template store<RECON>(d, Vhp, i, u);
+}
+
 // ---------------------------------------------------------------------------
 // host dispatch.  dtype: 0 complex128, 1 complex64
 // ---------------------------------------------------------------------------
@@ -220,6 +291,77 @@ hipError_t convert_gauge(int prec, int nc, int recon, const void* src, void* dst
     else WL_CG(F16, 1, 18);
   }
 #undef WL_CG
+  return hipGetLastError();
+}
+
+template <int P, int NC>
+static hipError_t pack_spinor_eo_t(const void* src, int dtype, void* dst, const Geom& g, long long Vh, long long Vhp,
+                                   int q, int ncol, size_t col_bytes, hipStream_t st) {
+  dim3 grid((unsigned)((Vh + NT - 1) / NT), (unsigned)ncol), block(NT);
+  if (dtype == 0)
+    hipLaunchKernelGGL((k_pack_spinor_eo<P, NC, double>), grid, block, 0, st, (const double*)src, dst, g, Vh, Vhp, q,
+                       col_bytes);
+  else
+    hipLaunchKernelGGL((k_pack_spinor_eo<P, NC, float>), grid, block, 0, st, (const float*)src, dst, g, Vh, Vhp, q,
+                       col_bytes);
+  return hipGetLastError();
+}
+
+template <int P, int NC>
+static hipError_t unpack_spinor_eo_t(const void* src, void* dst, int dtype, const Geom& g, long long Vhp, int q,
+                                     int ncol, size_t col_bytes, hipStream_t st) {
+  dim3 grid((unsigned)((g.V + NT - 1) / NT), (unsigned)ncol), block(NT);
+  if (dtype == 0)
+    hipLaunchKernelGGL((k_unpack_spinor_eo<P, NC, double>), grid, block, 0, st, src, (double*)dst, g, Vhp, q, col_bytes);
+  else
+    hipLaunchKernelGGL((k_unpack_spinor_eo<P, NC, float>), grid, block, 0, st, src, (float*)dst, g, Vhp, q, col_bytes);
+  return hipGetLastError();
+}
+
+hipError_t pack_spinor_eo(int prec, int nc, const void* src, int dtype, void* dst, const Geom& g, long long Vh,
+                          long long Vhp, int q, int ncol, size_t col_bytes, hipStream_t st) {
+  if (nc == 3) {
+    if (prec == F64) return pack_spinor_eo_t<F64, 3>(src, dtype, dst, g, Vh, Vhp, q, ncol, col_bytes, st);
+    if (prec == F32) return pack_spinor_eo_t<F32, 3>(src, dtype, dst, g, Vh, Vhp, q, ncol, col_bytes, st);
+    return pack_spinor_eo_t<F16, 3>(src, dtype, dst, g, Vh, Vhp, q, ncol, col_bytes, st);
+  }
+  if (prec == F64) return pack_spinor_eo_t<F64, 1>(src, dtype, dst, g, Vh, Vhp, q, ncol, col_bytes, st);
+  if (prec == F32) return pack_spinor_eo_t<F32, 1>(src, dtype, dst, g, Vh, Vhp, q, ncol, col_bytes, st);
+  return pack_spinor_eo_t<F16, 1>(src, dtype, dst, g, Vh, Vhp, q, ncol, col_bytes, st);
+}
+
+hipError_t unpack_spinor_eo(int prec, int nc, const void* src, void* dst, int dtype, const Geom& g, long long Vhp, int q,
+                            int ncol, size_t col_bytes, hipStream_t st) {
+  if (nc == 3) {
+    if (prec == F64) return unpack_spinor_eo_t<F64, 3>(src, dst, dtype, g, Vhp, q, ncol, col_bytes, st);
+    if (prec == F32) return unpack_spinor_eo_t<F32, 3>(src, dst, dtype, g, Vhp, q, ncol, col_bytes, st);
+    return unpack_spinor_eo_t<F16, 3>(src, dst, dtype, g, Vhp, q, ncol, col_bytes, st);
+  }
+  if (prec == F64) return unpack_spinor_eo_t<F64, 1>(src, dst, dtype, g, Vhp, q, ncol, col_bytes, st);
+  if (prec == F32) return unpack_spinor_eo_t<F32, 1>(src, dst, dtype, g, Vhp, q, ncol, col_bytes, st);
+  return unpack_spinor_eo_t<F16, 1>(src, dst, dtype, g, Vhp, q, ncol, col_bytes, st);
+}
+
+hipError_t convert_gauge_eo(int prec, int nc, int recon, const void* src, void* dst, const Geom& g, long long Vh,
+                            long long Vhp, int ncfg, size_t src_cfg_bytes, size_t src_dir_bytes, size_t cfg_bytes,
+                            size_t dir_bytes, hipStream_t st) {
+  dim3 grid((unsigned)((Vh + NT - 1) / NT), 8, (unsigned)ncfg), block(NT);
+#define WL_CGE(P, NCV, R)                                                                                      \
+  hipLaunchKernelGGL((k_convert_gauge_eo<P, NCV, R>), grid, block, 0, st, src, dst, g, Vh, Vhp, src_cfg_bytes, \
+                     src_dir_bytes, cfg_bytes, dir_bytes)
+  if (nc == 3) {
+    if (prec == F64 && recon == 12) WL_CGE(F64, 3, 12);
+    else if (prec == F64) WL_CGE(F64, 3, 18);
+    else if (prec == F32 && recon == 12) WL_CGE(F32, 3, 12);
+    else if (prec == F32) WL_CGE(F32, 3, 18);
+    else if (prec == F16 && recon == 12) WL_CGE(F16, 3, 12);
+    else WL_CGE(F16, 3, 18);
+  } else {
+    if (prec == F64) WL_CGE(F64, 1, 18);
+    else if (prec == F32) WL_CGE(F32, 1, 18);
+    else WL_CGE(F16, 1, 18);
+  }
+#undef WL_CGE
   return hipGetLastError();
 }
 

@@ -274,7 +274,7 @@ __device__ __forceinline__ void spin_reconstruct_fma(cx<float> (&acc)[4 * NC], c
 
 // One neighbour of one direction for one spinor column, given its link u:
 // forward (ADJ = false, projector sign S, phase ph on the boundary hop) or
-// backward (ADJ = true, conj(ph)).  FOLD: accumulate -1/2 of the term (F16
+// backward (ADJ = true, conj(ph)).  FOLD: accumulate fk times the term (F16
 // path only; other precisions always accumulate the plain term).
 // Link registers of one direction: FP32/FP64 complex, or stored half2 (F16).
 template <int P, int NC>
@@ -296,7 +296,7 @@ __device__ __forceinline__ void load_link(const void* __restrict__ base, size_t 
 template <int P, int NC, int MU, int S, bool ADJ, bool FOLD>
 __device__ __forceinline__ void hop_one(const typename LinkRegs<P, NC>::type& u, const void* __restrict__ in,
                                         size_t V, size_t s_nb, bool at_bnd, cx<typename PT<P>::acc> ph,
-                                        cx<typename PT<P>::acc> (&acc)[4 * NC]) {
+                                        cx<typename PT<P>::acc> (&acc)[4 * NC], float fk = -0.5f) {
   using T = typename PT<P>::acc;
   cx<T> h[2][NC], o[2][NC];
   const cx<T> phe = ADJ ? cconj(ph) : ph;
@@ -304,7 +304,7 @@ __device__ __forceinline__ void hop_one(const typename LinkRegs<P, NC>::type& u,
     HalfSpinor<NC> r;
     load_half_spinor<NC>(in, V, s_nb, r);
     const bool realbc = ph.im == T(0);
-    float f = (FOLD ? -0.5f : 1.0f) * r.s;
+    float f = (FOLD ? fk : 1.0f) * r.s;
     if (realbc && at_bnd) f *= ph.re;
     hf2 hA[2][NC], hB[2][NC];
     project_forms<MU, S, ADJ>(r, hA, hB);
@@ -344,15 +344,17 @@ __device__ __forceinline__ void hop_dir(const char* __restrict__ Ucfg, size_t di
   }
 }
 
-// Diagonal and hopping terms at the site: res = m0 psi(x) + c acc with
-// c = 1 when the hops were accumulated with the -1/2 folded in (FOLD), else
-// c = -1/2; DIAG = false gives the hopping term K alone.
+// Diagonal and hopping terms at the site: res = m0 psi(x) + hc acc, where
+// the hops were accumulated plain (or, with FOLD, already times hc, so that
+// res = m0 psi(x) + acc); DIAG = false gives the hopping term alone.
+// hc = -1/2 is the Wilson operator's hopping coefficient.
 template <int P, int NC, bool DIAG, bool FOLD>
 __device__ __forceinline__ void site_result(const void* __restrict__ in, size_t V, size_t site,
                                             typename PT<P>::acc m0, const cx<typename PT<P>::acc> (&acc)[4 * NC],
-                                            cx<typename PT<P>::acc> (&res)[4 * NC]) {
+                                            cx<typename PT<P>::acc> (&res)[4 * NC],
+                                            typename PT<P>::acc hc = typename PT<P>::acc(-0.5)) {
   using T = typename PT<P>::acc;
-  const T c = FOLD ? T(1) : T(-0.5);
+  const T c = FOLD ? T(1) : hc;
   if (!DIAG) {
 #pragma unroll
     for (int q = 0; q < 4 * NC; ++q) res[q] = FOLD ? acc[q] : cx<T>{c * acc[q].re, c * acc[q].im};

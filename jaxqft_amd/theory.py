@@ -215,6 +215,109 @@ class WilsonNf2Theory:
             raise ValueError("kernel must be one of: optimized, reference")
         return self.apply_normal(U, psi)
 
+    # ------------------------------------------------- even-odd (checkerboard)
+    def _eo(self, U, psi, op: int, normalized: bool = False, nrhs: int = 1):
+        ctx, g = self._gauge(U)
+        pt, kind, orig = to_device(psi)
+        out = torch.empty_like(pt)
+        dt = nat.WL_C128 if pt.dtype == torch.complex128 else nat.WL_C64
+        nat.check(ctx._lib.wl_eo_apply(ctx.h, g.h, int(op), 1 if normalized else 0, nat.c_void_p(pt.data_ptr()),
+                                       nat.c_void_p(out.data_ptr()), int(nrhs), dt, nat.c_void_p(nat.current_stream())))
+        return from_device(out, kind, orig)
+
+    def apply_K_oe(self, U, psi_even):
+        """K_oe = P_o K P_e, even -> odd hopping block (su3_wilson_nf2.py:1057-1061, :1081-1084)."""
+        self._check_fermion(psi_even, "psi_even")
+        return self._eo(U, psi_even, nat.WL_EO_K_OE)
+
+    def apply_K_eo(self, U, psi_odd):
+        """K_eo = P_e K P_o (su3_wilson_nf2.py:1063-1067)."""
+        self._check_fermion(psi_odd, "psi_odd")
+        return self._eo(U, psi_odd, nat.WL_EO_K_EO)
+
+    def apply_Kdag_oe(self, U, psi_even):
+        """P_o K^dag P_e (su3_wilson_nf2.py:1069-1073)."""
+        self._check_fermion(psi_even, "psi_even")
+        return self._eo(U, psi_even, nat.WL_EO_KDAG_OE)
+
+    def apply_Kdag_eo(self, U, psi_odd):
+        """P_e K^dag P_o (su3_wilson_nf2.py:1075-1079)."""
+        self._check_fermion(psi_odd, "psi_odd")
+        return self._eo(U, psi_odd, nat.WL_EO_KDAG_EO)
+
+    def apply_eo_schur_even(self, U, psi_even, normalized: bool = False):
+        """S_e = m0 - K_eo K_oe / m0 on the even sites of psi_even (normalized:
+        1 - K_eo K_oe / m0^2); full-lattice result, zero on odd sites
+        (su3_wilson_nf2.py:1112-1121)."""
+        self._check_fermion(psi_even, "psi_even")
+        return self._eo(U, psi_even, nat.WL_EO_SCHUR_E, normalized)
+
+    def apply_eo_schur_even_dagger(self, U, psi_even, normalized: bool = False):
+        """S_e^dag (su3_wilson_nf2.py:1138-1146)."""
+        self._check_fermion(psi_even, "psi_even")
+        return self._eo(U, psi_even, nat.WL_EO_SCHUR_E_DAG, normalized)
+
+    def apply_eo_schur_odd(self, U, psi_odd, normalized: bool = False):
+        """S_o = m0 - K_oe K_eo / m0 (su3_wilson_nf2.py:1148-1156)."""
+        self._check_fermion(psi_odd, "psi_odd")
+        return self._eo(U, psi_odd, nat.WL_EO_SCHUR_O, normalized)
+
+    def apply_eo_schur_even_compact(self, U, psi_even, daggered: bool = False, normalized: bool = True):
+        """Compact-path Schur matvec with a full-lattice interface
+        (su3_wilson_nf2.py:1123-1136); the same operator as apply_eo_schur_even."""
+        self._check_fermion(psi_even, "psi_even")
+        return self._eo(U, psi_even, nat.WL_EO_SCHUR_E_DAG if daggered else nat.WL_EO_SCHUR_E, normalized)
+
+    def _solve_eo(self, U, rhs, x0, nrhs: int = 1, masses=None, **overrides):
+        ctx, g = self._gauge(U)
+        pt, kind, orig = to_device(rhs)
+        x = torch.empty_like(pt)
+        x0t = None
+        if x0 is not None:
+            x0t, _, _ = to_device(x0, dtype=pt.dtype)
+        ncol = self.Bs * nrhs
+        sp, keep = self._solve_params(masses=masses, **overrides)
+        stats = (nat.WlSolveStats * ncol)()
+        dt = nat.WL_C128 if pt.dtype == torch.complex128 else nat.WL_C64
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            nat.check(ctx._lib.wl_cg_eo_normal(ctx.h, g.h, nat.c_void_p(pt.data_ptr()), nat.c_void_p(x.data_ptr()),
+                                               nat.c_void_p(x0t.data_ptr() if x0t is not None else 0), nrhs, dt,
+                                               nat.byref(sp), stats, nat.c_void_p(side.cuda_stream)))
+        torch.cuda.current_stream().wait_stream(side)
+        del keep
+        self.last_solve_stats = nat.stats_list(stats, ncol)
+        return from_device(x, kind, orig)
+
+    def solve_eop_even_normal(self, U, rhs):
+        """x_e = (S_e^dag S_e)^-1 P_e rhs with the normalized even Schur complement
+        (su3_wilson_nf2.py:1373-1389); full-lattice result, zero on odd sites."""
+        self._check_fermion(rhs, "rhs")
+        return self._solve_eo(U, rhs, None)
+
+    def solve_eop_even_normal_with_guess(self, U, rhs, x0):
+        """Same with initial guess P_e x0 (su3_wilson_nf2.py:1391-1392)."""
+        self._check_fermion(rhs, "rhs")
+        self._check_fermion(x0, "x0")
+        return self._solve_eo(U, rhs, x0)
+
+    def solve_eop_even(self, U, phi):
+        """solver_form 'normal' dispatch of su3_wilson_nf2.py:1403-1408."""
+        if self.solver_form != "normal":
+            raise ValueError(f"solver_form {self.solver_form!r}: only 'normal' has a native even-odd solver")
+        return self.solve_eop_even_normal(U, phi)
+
+    def solve_eop_even_normal_multi(self, U, rhs, masses: Optional[Sequence[float]] = None, **overrides):
+        """k right-hand sides per configuration through one even-odd solve.
+        rhs: (B, k, *L, Ns, Nc)."""
+        shp = tuple(rhs.shape)
+        if len(shp) != self.Nd + 4 or shp[0] != self.Bs or shp[2:] != tuple(self.fermion_shape()[1:]):
+            raise ValueError(f"rhs must have shape (B, k, *L, Ns, Nc); got {shp}")
+        k = int(shp[1])
+        x = self._solve_eo(U, rhs.reshape((self.Bs * k, *shp[2:])), None, nrhs=k, masses=masses, **overrides)
+        return x.reshape(shp)
+
     # ----------------------------------------------------------------- solvers
     def _solve_params(self, masses=None, **overrides):
         pol = dict(

@@ -286,6 +286,111 @@ class WilsonOracle:
     def apply_g5D(self, U, psi):
         return self.apply_gamma5(self.apply_D(U, psi))
 
+    # ----------------------------------------------------------- even-odd blocks
+    def parity_masks(self):
+        """Even/odd masks of shape (1, *L, 1, 1) (su3_wilson_nf2.py:819-829)."""
+        even = (parity_field(self.lattice_shape) % 2 == 0).astype(np.float64)[None, ..., None, None]
+        return even, 1.0 - even
+
+    def project_even(self, psi):
+        """su3_wilson_nf2.py:884-887."""
+        return np.asarray(psi) * self.parity_masks()[0]
+
+    def project_odd(self, psi):
+        """su3_wilson_nf2.py:889-892."""
+        return np.asarray(psi) * self.parity_masks()[1]
+
+    def apply_K_oe(self, U, psi_even):
+        """P_o K P_e (su3_wilson_nf2.py:1057-1061)."""
+        return self.project_odd(self.apply_offdiag(U, self.project_even(psi_even)))
+
+    def apply_K_eo(self, U, psi_odd):
+        """P_e K P_o (su3_wilson_nf2.py:1063-1067)."""
+        return self.project_even(self.apply_offdiag(U, self.project_odd(psi_odd)))
+
+    def apply_Kdag_oe(self, U, psi_even):
+        """su3_wilson_nf2.py:1069-1073."""
+        return self.project_odd(self.apply_offdiag_dagger(U, self.project_even(psi_even)))
+
+    def apply_Kdag_eo(self, U, psi_odd):
+        """su3_wilson_nf2.py:1075-1079."""
+        return self.project_even(self.apply_offdiag_dagger(U, self.project_odd(psi_odd)))
+
+    def apply_eo_schur_even(self, U, psi_even, normalized: bool = False):
+        """S_e = m0 - K_eo K_oe / m0, or 1 - K_eo K_oe / m0^2 (su3_wilson_nf2.py:1112-1121)."""
+        xe = self.project_even(psi_even)
+        z = self.apply_K_eo(U, self.apply_K_oe(U, xe))
+        return xe - z / (self.m0 * self.m0) if normalized else self.m0 * xe - z / self.m0
+
+    def apply_eo_schur_even_dagger(self, U, psi_even, normalized: bool = False):
+        """S_e^dag = m0 - K^dag_eo K^dag_oe / m0 (su3_wilson_nf2.py:1138-1146)."""
+        xe = self.project_even(psi_even)
+        z = self.apply_Kdag_eo(U, self.apply_Kdag_oe(U, xe))
+        return xe - z / (self.m0 * self.m0) if normalized else self.m0 * xe - z / self.m0
+
+    def apply_eo_schur_odd(self, U, psi_odd, normalized: bool = False):
+        """S_o = m0 - K_oe K_eo / m0 (su3_wilson_nf2.py:1148-1156)."""
+        xo = self.project_odd(psi_odd)
+        z = self.apply_K_oe(U, self.apply_K_eo(U, xo))
+        return xo - z / (self.m0 * self.m0) if normalized else self.m0 * xo - z / self.m0
+
+    def pack_parity(self, psi, parity: int):
+        """(B, *L, Ns, Nc) -> (B, V/2, Ns, Nc), sites of one parity in ascending
+        flat order (su3_wilson_nf2.py:894-910)."""
+        idx = checkerboard_indices(self.lattice_shape)[parity]
+        psi = np.asarray(psi)
+        return psi.reshape((psi.shape[0], -1) + psi.shape[-2:])[:, idx]
+
+    def unpack_parity(self, psi_c, parity: int):
+        """Inverse of pack_parity, zero on the other parity (su3_wilson_nf2.py:912-932)."""
+        idx = checkerboard_indices(self.lattice_shape)[parity]
+        V = int(np.prod(self.lattice_shape))
+        out = np.zeros((psi_c.shape[0], V) + psi_c.shape[-2:], dtype=np.complex128)
+        out[:, idx] = psi_c
+        return out.reshape((psi_c.shape[0],) + self.lattice_shape + psi_c.shape[-2:])
+
+    def _parity_compact_hop(self, Ul_b, src_c, sign: int, target_block: str):
+        """Literal restatement of _apply_dslash_parity_compact_one
+        (su3_wilson_nf2.py:934-965), with the exact projectors (r + coeff g_mu)
+        of projected_hop (wilson.py:163-169).  Ul_b: links of one configuration
+        with the boundary phase, (Nd, *L[, Nc, Nc])."""
+        maps = checkerboard_neighbor_maps(self.lattice_shape)
+        blk = "oe" if target_block == "oe" else "eo"
+        target, plus, minus, bwd = maps["target_" + blk], maps["plus_" + blk], maps["minus_" + blk], maps["bwd_" + blk]
+        V = int(np.prod(self.lattice_shape))
+        scalar = self.nc == 1
+        out = np.zeros_like(src_c, dtype=np.complex128)
+        for mu in range(self.Nd):
+            U_mu = Ul_b[mu].reshape((V,) if scalar else (V, self.nc, self.nc))
+            U_f = U_mu[target]
+            U_b = np.conj(U_mu[bwd[mu]]) if scalar else np.conj(np.swapaxes(U_mu[bwd[mu]], -1, -2))
+            for U_l, psi_n, coeff in ((U_f, src_c[plus[mu]], float(+sign)), (U_b, src_c[minus[mu]], float(-sign))):
+                upsi = U_l[:, None, None] * psi_n if scalar else np.einsum("nij,nsj->nsi", U_l, psi_n)
+                out = out - 0.5 * (self.wilson_r * upsi + coeff * np.einsum("st,ntc->nsc", self.gamma[mu], upsi))
+        return out
+
+    def apply_eo_schur_even_compact_one(self, U_b, xe_c, daggered: bool = False, normalized: bool = True):
+        """Compact even Schur matvec for one configuration (su3_wilson_nf2.py:967-988)."""
+        Ul = links_with_fermion_bc(links_bmxyij(np.asarray(U_b)[None], self.Nd, self.layout), self.fermion_bc)[0]
+        sign = +1 if daggered else -1
+        y = self._parity_compact_hop(Ul, np.asarray(xe_c, dtype=np.complex128), sign, "oe")
+        z = self._parity_compact_hop(Ul, y, sign, "eo")
+        return xe_c - z / (self.m0 * self.m0) if normalized else self.m0 * xe_c - z / self.m0
+
+    def solve_eop_even_normal(self, U, rhs, x0=None, tol: float = 1e-10, maxiter: int = 5000):
+        """CG on S_e^dag S_e (normalized) in the compact even basis, per batch
+        element (su3_wilson_nf2.py:1373-1389), with the jax cg rule."""
+        out = []
+        pb = self.pack_parity(self.project_even(rhs), 0)
+        xs = None if x0 is None else self.pack_parity(self.project_even(x0), 0)
+        for b in range(pb.shape[0]):
+            def A(v, b=b):
+                y = self.apply_eo_schur_even_compact_one(np.asarray(U)[b], v, daggered=False, normalized=True)
+                return self.apply_eo_schur_even_compact_one(np.asarray(U)[b], y, daggered=True, normalized=True)
+            xb, _ = cg(A, pb[b], None if xs is None else xs[b], tol=tol, maxiter=maxiter)
+            out.append(xb)
+        return self.unpack_parity(np.stack(out), 0)
+
     def normal_residual_stats(self, U, phi, x) -> Dict[str, float]:
         """su3_wilson_nf2.py:1456-1466."""
         r = self.apply_normal(U, x) - phi
