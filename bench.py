@@ -99,6 +99,27 @@ class Bench:
         del keep
         return nat.stats_list(st, b.ncol)
 
+    def solve_eo(self, ctx, gauge, bt, xt, nrhs, prec, recon):
+        """Even-odd normal equations (wl_cg_eo_normal) on reference-layout device arrays."""
+        nat = self.nat
+        a = self.args
+        sp, keep = nat.solve_params(a.tol, 20000, nat.WL_STOP_TRUE_RESIDUAL, nat.PREC[prec], recon, a.delta, a.poll,
+                                    bool(a.graph), None, accum=a.accum)
+        st = (nat.WlSolveStats * nrhs)()
+        nat.check(self.lib.wl_cg_eo_normal(ctx.h, gauge.h, c_void_p(bt.data_ptr()), c_void_p(xt.data_ptr()), c_void_p(0),
+                                           nrhs, nat.WL_C128, byref(sp), st, self.h))
+        del keep
+        return nat.stats_list(st, nrhs)
+
+    def timed(self, n, fn, *fargs):
+        torch = self.torch
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            st = fn(*fargs)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / n, st
+
     def timed_solves(self, n, *sargs):
         torch = self.torch
         torch.cuda.synchronize()
@@ -263,6 +284,25 @@ def main():
         err1 = float(torch.linalg.norm(xa - xb) / torch.linalg.norm(xb))
         errk = float(torch.linalg.norm(xkt[:1] - xb) / torch.linalg.norm(xb))
 
+        # even-odd preconditioned normal equations (solve_eop_even_normal): the
+        # same sources' even parts, FP64 and mixed, one and k right-hand sides
+        eo = {}
+        with torch.cuda.stream(stream):
+            xe1 = torch.empty_like(b1_t)
+            xek = torch.empty_like(bk_t)
+        for name, bt, xt, nr, prec, recon, n in (("fp64_single_rhs", b1_t, xe1, 1, "f64", 18, args.fp64_steps),
+                                                ("mixed_single_rhs", b1_t, xe1, 1, args.inner, args.recon,
+                                                 args.fp64_steps),
+                                                ("mixed_multi_rhs", bk_t, xek, k, args.inner, args.recon, args.steps)):
+            B.solve_eo(ctx, gauge, bt, xt, nr, prec, recon)
+            t, st = B.timed(n, B.solve_eo, ctx, gauge, bt, xt, nr, prec, recon)
+            eo[name] = {**summarize(st), "nrhs": nr, "s_per_call": round(t, 4), "solves_per_s": round(nr / t, 3)}
+            log(f"even-odd {name}: {t*1e3:.1f} ms {summarize(st)}")
+        eo["speedup_vs_full_fp64_single"] = round(eo["fp64_single_rhs"]["solves_per_s"] * fp64_s, 3)
+        eo["speedup_vs_full_mixed_multi"] = round(eo["mixed_multi_rhs"]["solves_per_s"] * mixk_s / k, 3)
+        eo["system"] = "S_e^dag S_e x_e = b_e, normalized even Schur complement, FP64 true residual %g" % args.tol
+        del xe1, xek
+
         # inner stencil of the headline solve, timed on its own (HIP events)
         pc = nat.PREC[args.inner]
         fi, fo = nat.Field(ctx, pc, k), nat.Field(ctx, pc, k)
@@ -314,6 +354,7 @@ def main():
             "speedup_mixed_multi_vs_fp64_single": round((k / mixk_s) / (1 / fp64_s), 3),
             "speedup_mixed_single_vs_fp64_single": round(fp64_s / mix1_s, 3),
         },
+        "even_odd": eo,
         "config3_single_rhs_delta0.1": config3,
         "dslash_single_rhs": dslash,
         "dslash_multi_rhs": dslash_k,

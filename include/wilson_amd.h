@@ -115,7 +115,9 @@ typedef struct {
   int stop_rule;    /* WL_STOP_* */
   int inner_prec;   /* WL_F64 (plain CG) / WL_F32 / WL_F16 (reliable-update mixed precision) */
   int inner_recon;  /* 18 or 12 (12-real link reconstruction) for the inner operator */
-  double delta;     /* reliable-update trigger on the residual drop (mixed precision), e.g. 0.1 */
+  double delta;     /* reliable-update trigger on the residual drop (mixed precision), e.g. 0.1;
+                       at least 3e-3 (FP16 inner) or 1e-5 (FP32 inner) is used, and an update
+                       is also forced after 100 inner iterations without one */
   int poll_every;   /* iterations queued between host polls of the convergence flags */
   int use_graph;    /* capture the poll_every-iteration body as a HIP graph */
   const double* mass; /* host array of per-column masses, or NULL for the context mass */

@@ -494,7 +494,8 @@ static void fill_eo_args(const wl_ctx* c, DslashArgs& a, int prec, const void* U
   a.ntiles = (int)((c->Vh + NT - 1) / NT);
   a.march = 1;
   a.nseg = c->geom.L[0];
-  a.mrhs = 1;
+  // right-hand sides per thread: the multi-RHS setting, at most 2
+  a.mrhs = (nrhs > 1 && c->geom.nc == 3 && nrhs % 2 == 0 && mrhs_for(c, prec, nrhs) >= 2) ? 2 : 1;
   a.xcd_remap = c->xcd_remap[nrhs > 1 ? 1 : 0][prec];
   a.normalized = 1;
 }
@@ -786,6 +787,16 @@ struct Solver {
 
 }  // namespace
 
+// Reliable-update trigger actually used: the requested delta, but not below
+// a floor set by the inner storage precision (an iterated residual kept in
+// FP16 with per-site scales cannot follow a drop much below a few 1e-3 of its
+// starting value, and the trigger would then never fire).
+static double effective_delta(double delta, int inner) {
+  if (!(delta > 0.0)) return 0.0;
+  const double floor = inner == F16 ? 3e-3 : (inner == F32 ? 1e-5 : 0.0);
+  return delta < floor ? floor : delta;
+}
+
 // storage of the inner solution accumulator y (x itself for a plain FP64 solve)
 static int accum_prec_of(const wl_solve_params* sp, int inner) {
   if (inner == F64) return F64;
@@ -881,7 +892,7 @@ static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int n
     cs.m0 = m + 4.0 * c->params.wilson_r;
     cs.maxiter = sp->maxiter;
     cs.mode = sp->stop_rule == WL_STOP_ITERATED ? 1 : 0;
-    cs.delta = s.mixed ? sp->delta : 0.0;
+    cs.delta = s.mixed ? effective_delta(sp->delta, s.P) : 0.0;
     cs.need_update = 1;
     cs.active = 0;
     cs.status = WL_STATUS_RUNNING;
@@ -1316,7 +1327,7 @@ int wl_batch_admit(wl_batch* bt, int slot, const void* b, int dtype, double mass
   cs.m0 = mass + 4.0 * c->params.wilson_r;
   cs.maxiter = bt->sp.maxiter;
   cs.mode = bt->sp.stop_rule == WL_STOP_ITERATED ? 1 : 0;
-  cs.delta = s.mixed ? bt->sp.delta : 0.0;
+  cs.delta = s.mixed ? effective_delta(bt->sp.delta, s.P) : 0.0;
   cs.status = WL_STATUS_RUNNING;
   e = hipMemcpyAsync((ColState*)bt->cols.p + slot, &cs, sizeof(ColState), hipMemcpyHostToDevice, st);
   if (e != hipSuccess) return fail(WL_ERR_HIP, "admit: %s", hipGetErrorString(e));

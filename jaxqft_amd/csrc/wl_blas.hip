@@ -136,6 +136,7 @@ __global__ void k_finalize_update(ColState* cols, int ncol) {
     s.rr = s.rtrue2;
     s.r0norm = sqrt(s.rtrue2);
     s.nupdates += 1;
+    s.last_update = s.iter;
     s.active = 1;
   }
   s.need_update = 0;

@@ -58,10 +58,19 @@ static hipError_t launch_p(int op, int epi, const DslashArgs& a, hipStream_t str
 // Even-odd hop: dag (K or K^dag), mode 0 (hop) / 1 (Schur combine), epilogue
 // EPI_STORE, or for the combine EPI_NORM (forward), EPI_RESID (FP64) and the CG
 // steps (dagger).
+template <int P, int NC, int RECON, bool D, int M, int E>
+static void launch_eo_k(const DslashArgs& a, hipStream_t stream) {
+  if (NC == 3 && a.mrhs == 2)
+    hipLaunchKernelGGL((k_hop_eo<P, NC, RECON, D, M, E, 2>), dim3((unsigned)(a.ntiles * (a.ncol / 2))), dim3(NT), 0,
+                       stream, a);
+  else
+    hipLaunchKernelGGL((k_hop_eo<P, NC, RECON, D, M, E, 1>), dim3((unsigned)(a.ntiles * a.ncol)), dim3(NT), 0, stream,
+                       a);
+}
+
 template <int P, int NC, int RECON>
 static hipError_t launch_eo_p(int dag, int mode, int epi, const DslashArgs& a, hipStream_t stream) {
-  dim3 grid((unsigned)(a.ntiles * a.ncol)), block(NT);
-#define WL_EOL(D, M, E) hipLaunchKernelGGL((k_hop_eo<P, NC, RECON, D, M, E>), grid, block, 0, stream, a)
+#define WL_EOL(D, M, E) launch_eo_k<P, NC, RECON, D, M, E>(a, stream)
   if (epi == EPI_STORE) {
     if (mode == 0) {
       if (dag) WL_EOL(true, 0, EPI_STORE);

@@ -25,14 +25,25 @@
 
 namespace wl {
 
-template <int P, int NC, int RECON, bool DAG, int MODE, int EPI>
+// K right-hand sides of the same configuration per thread (columns
+// col0 .. col0 + K - 1, K = a.mrhs): each link is loaded once for all K.
+template <int P, int NC, int RECON, bool DAG, int MODE, int EPI, int K>
 __global__ __launch_bounds__(NT) void k_hop_eo(DslashArgs a) {
   using T = typename PT<P>::acc;
+  const int ngrp = a.ncol / K;
   const unsigned logical = xcd_tile(blockIdx.x, gridDim.x, a.xcd_remap);
-  const int col = (int)(logical % a.ncol);
-  const int tile = (int)(logical / a.ncol);
+  const int grp = (int)(logical % ngrp);
+  const int tile = (int)(logical / ngrp);
+  const int col0 = grp * K;
   const ColState* st = a.red.cols;
-  if (a.gate && gate_skip_col(a.gate, a.red, col)) return;
+  bool on[K];
+  bool any = false;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    on[k] = !(a.gate && gate_skip_col(a.gate, a.red, col0 + k));
+    any |= on[k];
+  }
+  if (!any) return;
 
   const size_t V = (size_t)a.g.Vp;  // plane stride of the compact fields
   const size_t i0 = (size_t)tile * NT + threadIdx.x;
@@ -50,28 +61,32 @@ __global__ __launch_bounds__(NT) void k_hop_eo(DslashArgs a) {
   x[3] = (int)(2 * j) + ((x[0] + x[1] + x[2] + a.pt) & 1);
   const size_t flat = row * (size_t)a.g.L[3] + (size_t)x[3];
 
-  const int cfg = col / a.cols_per_cfg;
+  const int cfg = col0 / a.cols_per_cfg;
   const char* Ucfg = reinterpret_cast<const char*>(a.U) + (size_t)cfg * a.cfg_bytes;
   const char* Ut = Ucfg + (size_t)a.pt * 4 * a.dir_bytes;        // links at target-parity sites
   const char* Us = Ucfg + (size_t)(1 - a.pt) * 4 * a.dir_bytes;  // links at source-parity sites
-  const char* inb = reinterpret_cast<const char*>(a.in) + (size_t)col * a.col_bytes;
 
-  const T m0 = a.m0_from_state ? T(st[col].m0) : T(a.m0);
-  T dc, hc;
-  if (MODE == 0) {
-    dc = T(0);
-    hc = T(-0.5);
-  } else if (a.normalized) {
-    dc = T(1);
-    hc = T(0.5) / (m0 * m0);
-  } else {
-    dc = m0;
-    hc = T(0.5) / m0;
+  T dc[K], hc[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const T m0 = a.m0_from_state ? T(st[col0 + k].m0) : T(a.m0);
+    if (MODE == 0) {
+      dc[k] = T(0);
+      hc[k] = T(-0.5);
+    } else if (a.normalized) {
+      dc[k] = T(1);
+      hc[k] = T(0.5) / (m0 * m0);
+    } else {
+      dc[k] = m0;
+      hc[k] = T(0.5) / m0;
+    }
   }
 
-  cx<T> acc[4 * NC];
+  cx<T> acc[K][4 * NC];
 #pragma unroll
-  for (int q = 0; q < 4 * NC; ++q) acc[q] = {T(0), T(0)};
+  for (int k = 0; k < K; ++k)
+#pragma unroll
+    for (int q = 0; q < 4 * NC; ++q) acc[k][q] = {T(0), T(0)};
   constexpr int SF = DAG ? +1 : -1;
   constexpr bool FOLD = P == F16;
 #define WL_EOHOP(MU)                                                                                      \
@@ -85,12 +100,20 @@ __global__ __launch_bounds__(NT) void k_hop_eo(DslashArgs a) {
     {                                                                                                     \
       typename LinkRegs<P, NC>::type u;                                                                   \
       load_link<P, NC, RECON>(Ut + MU * a.dir_bytes, V, i, u);                                            \
-      hop_one<P, NC, MU, SF, false, FOLD>(u, inb, V, ff >> 1, top, ph, acc, (float)hc);                  \
+      _Pragma("unroll") for (int k = 0; k < K; ++k) {                                                     \
+        if (!on[k]) continue;                                                                             \
+        hop_one<P, NC, MU, SF, false, FOLD>(u, reinterpret_cast<const char*>(a.in) + (size_t)(col0 + k) * a.col_bytes, \
+                                            V, ff >> 1, top, ph, acc[k], (float)hc[k]);                   \
+      }                                                                                                   \
     }                                                                                                     \
     {                                                                                                     \
       typename LinkRegs<P, NC>::type u;                                                                   \
       load_link<P, NC, RECON>(Us + MU * a.dir_bytes, V, fb >> 1, u);                                      \
-      hop_one<P, NC, MU, -SF, true, FOLD>(u, inb, V, fb >> 1, bot, ph, acc, (float)hc);                  \
+      _Pragma("unroll") for (int k = 0; k < K; ++k) {                                                     \
+        if (!on[k]) continue;                                                                             \
+        hop_one<P, NC, MU, -SF, true, FOLD>(u, reinterpret_cast<const char*>(a.in) + (size_t)(col0 + k) * a.col_bytes, \
+                                            V, fb >> 1, bot, ph, acc[k], (float)hc[k]);                   \
+      }                                                                                                   \
     }                                                                                                     \
   }
   WL_EOHOP(0)
@@ -99,14 +122,21 @@ __global__ __launch_bounds__(NT) void k_hop_eo(DslashArgs a) {
   WL_EOHOP(3)
 #undef WL_EOHOP
 
-  cx<T> res[4 * NC];
-  if (MODE == 0)
-    site_result<P, NC, false, FOLD>(nullptr, V, i, T(0), acc, res, hc);
-  else
-    site_result<P, NC, true, FOLD>(reinterpret_cast<const char*>(a.ctr) + (size_t)col * a.col_bytes, V, i, dc, acc,
-                                   res, hc);
-  const double part = site_epilogue<P, NC, EPI>(a, col, V, i, valid, res);
-  if (EPI != EPI_STORE) write_partial(a.red, col, tile, a.ntiles, part);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int col = col0 + k;
+    double part = 0.0;
+    if (on[k]) {
+      cx<T> res[4 * NC];
+      if (MODE == 0)
+        site_result<P, NC, false, FOLD>(nullptr, V, i, T(0), acc[k], res, hc[k]);
+      else
+        site_result<P, NC, true, FOLD>(reinterpret_cast<const char*>(a.ctr) + (size_t)col * a.col_bytes, V, i, dc[k],
+                                       acc[k], res, hc[k]);
+      part = site_epilogue<P, NC, EPI>(a, col, V, i, valid, res);
+    }
+    if (EPI != EPI_STORE && on[k]) write_partial(a.red, col, tile, a.ntiles, part);
+  }
 }
 
 }  // namespace wl

@@ -41,7 +41,13 @@ struct ColState {
   int need_update;  // 1: reliable update, 2: convergence check
   int nupdates;
   int mode;         // 0: stop on true residual, 1: stop on iterated residual (reference rule)
+  int last_update;  // iteration of the last reliable update
+  int pad_;
 };
+
+// A reliable update is also forced after this many inner iterations without
+// one (guards against an iterated residual that stalls above the trigger).
+constexpr int MAX_INNER_ITERS = 100;
 
 // Which per-column action the last workgroup of a reduction performs.
 enum RedAction {

@@ -63,7 +63,7 @@ __device__ __forceinline__ void apply_action(int action, ColState& s, double sum
           s.need_update = 2;
           s.active = 0;
         }
-      } else if (s.delta > 0.0 && sqrt(sum) < s.delta * s.r0norm) {
+      } else if (s.delta > 0.0 && (sqrt(sum) < s.delta * s.r0norm || s.iter - s.last_update >= MAX_INNER_ITERS)) {
         s.need_update = 1;
         s.active = 0;
       } else if (s.iter >= s.maxiter) {

@@ -164,3 +164,26 @@ def test_cg_with_marching_stencil(prec, march, slab, box):
     x = th.solve_normal(U, b)
     assert th.last_solve_stats[0]["status"] == "converged"
     assert true_resid(L, 3, 0.05, (1, 1, 1, -1), U, b, x) <= 1e-10
+
+
+@pytest.mark.parametrize("delta", [1e-4, 1e-3])
+def test_small_delta_still_converges(delta):
+    """A reliable-update trigger below what FP16 storage can follow is raised
+    to the precision floor and backed by a forced update every 100 inner
+    iterations, so the solve converges instead of running to maxiter."""
+    print(f"[gpu] small delta {delta}", flush=True)
+    from jaxqft_amd import fields
+    from jaxqft_amd.su3_wilson_nf2 import SU3WilsonNf2
+
+    L = (8, 8, 8, 8)
+    rng = fields.rng_for(1600)
+    U = fields.su3_hot_start(rng, 1, L, 0.6)
+    b = fields.random_fermion(rng, (1, *L, 4, 3))
+    th = SU3WilsonNf2(lattice_shape=L, mass=0.02, fermion_bc=(1, 1, 1, -1), dtype=np.complex128, cg_tol=1e-10,
+                      cg_maxiter=3000, solver_precision="f16", solver_recon=12, reliable_delta=delta)
+    x = th.solve_normal(U, b)
+    st = th.last_solve_stats[0]
+    assert st["status"] == "converged", st
+    assert st["iters"] < 1500, st
+    assert true_resid(L, 3, 0.02, (1, 1, 1, -1), U, b, x) <= 1e-10
+

@@ -78,9 +78,10 @@ def test_eo_schur_odd_dagger_is_adjoint():
     assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
 
 
-@pytest.mark.parametrize("nrhs", [1, 3])
+@pytest.mark.parametrize("nrhs", [1, 3, 4])
 def test_eo_apply_multi_rhs_columns(nrhs):
-    """k right-hand sides per configuration through the C ABI, column by column."""
+    """k right-hand sides per configuration through the C ABI, column by column
+    (an even k shares each link load between two right-hand sides)."""
     print(f"[gpu] eo multi-rhs nrhs={nrhs}", flush=True)
     import torch
 
@@ -174,13 +175,13 @@ def test_eo_multi_rhs_alone_vs_mix_bitwise():
     L = (4, 4, 4, 8)
     rng = fields.rng_for(1550)
     U = fields.su3_hot_start(rng, 1, L, 0.5)
-    rhs = fields.random_fermion(rng, (1, 3, *L, 4, 3))
-    masses = [0.05, 0.2, 0.1]
+    rhs = fields.random_fermion(rng, (1, 4, *L, 4, 3))
+    masses = [0.05, 0.2, 0.1, 0.3]
     th = theory(L, 3, 0.1, (1, 1, 1, -1), cg_tol=1e-10, cg_maxiter=3000, solver_precision="f16", solver_recon=12)
     X = np.asarray(th.solve_eop_even_normal_multi(U, rhs, masses=masses))
     stats = th.last_solve_stats
     assert len({s["iters"] for s in stats}) > 1
-    for j in range(3):
+    for j in range(4):
         xj = np.asarray(th.solve_eop_even_normal_multi(U, rhs[:, j:j + 1], masses=[masses[j]]))
         assert np.array_equal(xj[:, 0], X[:, j]), j
         assert th.last_solve_stats[0]["iters"] == stats[j]["iters"]
