@@ -214,6 +214,14 @@ int wl_cg_normal_field(wl_ctx* ctx, wl_gauge* g, const wl_field* b, wl_field* x,
  * lattice extent even. */
 int wl_eo_apply(wl_ctx* ctx, wl_gauge* g, int op, int normalized, const void* in, void* out, int nrhs, int dtype,
                 void* stream);
+/* Analytic force of the Nf=2 pseudofermion action phi^dag (D^dag D)^-1 phi
+ * from X = (D^dag D)^-1 phi and Y = D X (replaces
+ * SU3WilsonNf2.pseudofermion_force_from_solution, su3_wilson_nf2.py:1556-1576,
+ * and the U(1) one, u1_wilson_nf2.py:1004-1025): F_mu(x) in the su(3) (u(1))
+ * algebra for every link.  X, Y: (batch, *L, 4, Nc) reference-layout arrays;
+ * F: (batch, 4, *L, Nc, Nc) for WL_LAYOUT_BMXYIJ or (batch, *L, 4, Nc, Nc) for
+ * WL_LAYOUT_BXYMIJ (U(1): no (Nc, Nc) axes); all three of element type dtype. */
+int wl_pf_force(wl_ctx* ctx, wl_gauge* g, const void* X, const void* Y, void* F, int layout, int dtype, void* stream);
 /* Solve S_e^dag S_e x_e = P_e b with the normalized even Schur complement
  * (replaces solve_eop_even_normal / solve_eop_even_normal_with_guess,
  * su3_wilson_nf2.py:1373-1392); same solver parameters and statistics as

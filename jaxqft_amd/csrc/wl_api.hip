@@ -34,6 +34,7 @@ hipError_t convert_gauge(int prec, int nc, int recon, const void* src, void* dst
                          size_t src_cfg_bytes, size_t src_dir_bytes, size_t cfg_bytes, size_t dir_bytes,
                          hipStream_t st);
 hipError_t launch_eo(int prec, int nc, int recon, int dag, int mode, int epi, const DslashArgs& a, hipStream_t stream);
+hipError_t launch_pf_force(int nc, int dtype, const ForceArgs& a, int ncfg, hipStream_t st);
 hipError_t pack_spinor_eo(int prec, int nc, const void* src, int dtype, void* dst, const Geom& g, long long Vh,
                           long long Vhp, int q, int ncol, size_t col_bytes, hipStream_t st);
 hipError_t unpack_spinor_eo(int prec, int nc, const void* src, void* dst, int dtype, const Geom& g, long long Vhp, int q,
@@ -1122,6 +1123,33 @@ int wl_eo_apply(wl_ctx* c, wl_gauge* g, int op, int normalized, const void* in, 
   }
   e = unpack_spinor_eo(F64, nc, c->ws[1].p, out, dtype, c->g, c->Vhp, target, ncol, cb, st);
   if (e != hipSuccess) return fail(WL_ERR_HIP, "unpack: %s", hipGetErrorString(e));
+  return WL_OK;
+}
+
+int wl_pf_force(wl_ctx* c, wl_gauge* g, const void* X, const void* Y, void* F, int layout, int dtype, void* stream) {
+  CHK(check_ctx(c));
+  if (!g || g->ctx != c) return fail(WL_ERR_BAD_ARG, "gauge handle does not belong to this context");
+  if (!X || !Y || !F) return fail(WL_ERR_BAD_ARG, "null field pointer");
+  if (dtype != WL_C128 && dtype != WL_C64) return fail(WL_ERR_BAD_ARG, "bad dtype %d", dtype);
+  if (layout != WL_LAYOUT_BMXYIJ && layout != WL_LAYOUT_BXYMIJ) return fail(WL_ERR_BAD_ARG, "bad layout %d", layout);
+  std::lock_guard<std::mutex> lk(c->mu);
+  ForceArgs a;
+  memset(&a, 0, sizeof(a));
+  a.g = c->g;
+  a.U = g->master.p;
+  a.cfg_bytes = cfg_bytes(c, F64, 18);
+  a.dir_bytes = a.cfg_bytes / 4;
+  a.X = X;
+  a.Y = Y;
+  a.F = F;
+  a.layout = layout;
+  for (int i = 0; i < 4; ++i) {
+    a.bc_re[i] = c->params.bc_re[i];
+    a.bc_im[i] = c->params.bc_im[i];
+  }
+  a.r = c->params.wilson_r;
+  hipError_t e = launch_pf_force(c->geom.nc, dtype, a, c->geom.batch, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(WL_ERR_HIP, "force: %s", hipGetErrorString(e));
   return WL_OK;
 }
 

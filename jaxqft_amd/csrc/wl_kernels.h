@@ -130,6 +130,21 @@ __host__ __device__ inline size_t eo_full_index(const Geom& g, int q, size_t i) 
   return row * (size_t)g.L[3] + 2 * j + ((x0 + x1 + x2 + (size_t)q) & 1);
 }
 
+// Analytic pseudofermion force (wl_force.hip): reference-layout X, Y and F,
+// FP64 internal links.
+struct ForceArgs {
+  Geom g;
+  const void* U;     // FP64 links, internal layout
+  size_t cfg_bytes;  // bytes of one configuration
+  size_t dir_bytes;  // bytes of one direction
+  const void* X;     // (B, *L, 4, NC) complex
+  const void* Y;
+  void* F;           // (B, 4, *L, NC, NC) (layout 0) or (B, *L, 4, NC, NC) (layout 1); U(1): no (NC, NC)
+  int layout;
+  double bc_re[4], bc_im[4];
+  double r;
+};
+
 struct BlasArgs {
   long long V;
   long long Vp;

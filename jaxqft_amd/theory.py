@@ -29,6 +29,13 @@ from .fermions.wilson import WilsonDiracOperator
 from . import fields as _fields
 
 
+def _as_like(a, ref):
+    """a as the same kind of array as ref (torch tensor on ref's device, or NumPy)."""
+    if isinstance(ref, torch.Tensor):
+        return a.to(ref.device) if isinstance(a, torch.Tensor) else torch.as_tensor(np.asarray(a), device=ref.device)
+    return np.asarray(a.cpu()) if isinstance(a, torch.Tensor) else np.asarray(a)
+
+
 def parse_fermion_bc(bc, nd: int, default_antiperiodic_t: bool) -> Tuple[complex, ...]:
     """su3_wilson_nf2.py:711-724 / :1645-1660 and u1_wilson_nf2.py:461-471."""
     if bc is None:
@@ -317,6 +324,45 @@ class WilsonNf2Theory:
         k = int(shp[1])
         x = self._solve_eo(U, rhs.reshape((self.Bs * k, *shp[2:])), None, nrhs=k, masses=masses, **overrides)
         return x.reshape(shp)
+
+    # ------------------------------------------------------ pseudofermion force
+    def pseudofermion_force_from_solution(self, U, X, Y):
+        """Analytic force of phi^dag (D^dag D)^-1 phi from X = (D^dag D)^-1 phi and
+        Y = D X, in the link layout (su3_wilson_nf2.py:1556-1576)."""
+        self._check_fermion(X, "X")
+        self._check_fermion(Y, "Y")
+        ctx, g = self._gauge(U)
+        xt, kind, orig = to_device(X)
+        yt, _, _ = to_device(Y, dtype=xt.dtype)
+        shp = tuple(self.field_shape())
+        F = torch.empty(shp, dtype=xt.dtype, device=xt.device)
+        dt = nat.WL_C128 if xt.dtype == torch.complex128 else nat.WL_C64
+        nat.check(ctx._lib.wl_pf_force(ctx.h, g.h, nat.c_void_p(xt.data_ptr()), nat.c_void_p(yt.data_ptr()),
+                                       nat.c_void_p(F.data_ptr()), self._layout_code(), dt,
+                                       nat.c_void_p(nat.current_stream())))
+        return from_device(F, kind, orig)
+
+    def eop_pseudofermion_force_from_solution(self, U, X_e, Y_e):
+        """Force of phi_e^dag (S^dag S)^-1 phi_e (normalized even Schur complement)
+        from X_e and Y_e = S X_e: two standard force terms,
+        F(K_oe X_e, -Y_e / m0^2) + F(X_e, -K^dag_oe Y_e / m0^2)
+        (su3_wilson_nf2.py:1578-1597)."""
+        alpha = 1.0 / (self._m0_value() ** 2)
+        kx = self.apply_K_oe(U, X_e)
+        ky = self.apply_Kdag_oe(U, Y_e)
+        f1 = self.pseudofermion_force_from_solution(U, kx, -alpha * _as_like(Y_e, kx))
+        f2 = self.pseudofermion_force_from_solution(U, X_e, -alpha * ky)
+        return f1 + f2
+
+    def pseudofermion_force(self, U, phi):
+        """The analytic monomial force: X = solve_normal(U, phi), Y = D X, then
+        the force from the solution (su3_wilson_nf2.py:378-392)."""
+        X = self.solve_normal(U, phi)
+        Y = self.apply_D(U, X)
+        return self.pseudofermion_force_from_solution(U, X, Y)
+
+    def _m0_value(self) -> float:
+        return float(self.mass) + float(self.wilson_r) * self.Nd
 
     # ----------------------------------------------------------------- solvers
     def _solve_params(self, masses=None, **overrides):

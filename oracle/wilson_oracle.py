@@ -391,6 +391,55 @@ class WilsonOracle:
             out.append(xb)
         return self.unpack_parity(np.stack(out), 0)
 
+    # ------------------------------------------------------ pseudofermion force
+    def pseudofermion_force_from_solution(self, U, X, Y):
+        """Analytic force of S = phi^dag (D^dag D)^-1 phi from X = (D^dag D)^-1 phi
+        and Y = D X (su3_wilson_nf2.py:1556-1576; U(1): u1_wilson_nf2.py:1004-1025):
+        per link F_mu(x) = TA(k^T), k = c1 - c2,
+        c1[a,c] = sum_st conj(Y(x)[s,a]) (r - g_mu)[s,t] (U~_mu(x) X(x+mu))[t,c],
+        c2[d,c] = sum_sta conj(Y(x+mu)[s,a]) (r + g_mu)[s,t] U~_mu(x)^dag[a,d] X(x)[t,c],
+        TA the traceless anti-Hermitian part (su3_ym.py:63-67) or 1/2 (k - conj k)
+        for U(1) (u1_ym.py:21-23).  Returned in the layout of U."""
+        Ul = self._links(U)  # BMXYIJ, with the boundary phase
+        X = np.asarray(X, dtype=np.complex128)
+        Y = np.asarray(Y, dtype=np.complex128)
+        eye = np.eye(self.Ns, dtype=np.complex128)
+        out = []
+        for mu in range(self.Nd):
+            U_mu = Ul[:, mu]
+            Xp = _roll_site(X, -1, mu)
+            Yp = _roll_site(Y, -1, mu)
+            p_fwd = self.wilson_r * eye - self.gamma[mu]
+            p_bwd = self.wilson_r * eye + self.gamma[mu]
+            if self.nc == 1:
+                UX = U_mu[..., None, None] * Xp
+                Udag = np.conj(U_mu)[..., None, None]
+            else:
+                UX = np.einsum("...ij,...sj->...si", U_mu, Xp)
+                Udag = np.conj(np.swapaxes(U_mu, -1, -2))
+            c1 = np.einsum("...sa,st,...tc->...ac", np.conj(Y), p_fwd, UX)
+            c2 = np.einsum("...sa,st,...ad,...tc->...dc", np.conj(Yp), p_bwd, Udag, X)
+            if self.nc == 1:
+                k = (c1 - c2)[..., 0, 0]
+                out.append(0.5 * (k - np.conj(k)))
+            else:
+                k = np.swapaxes(c1 - c2, -1, -2)
+                y = 0.5 * (k - np.conj(np.swapaxes(k, -1, -2)))
+                tr = np.trace(y, axis1=-2, axis2=-1)[..., None, None] / 3.0
+                out.append(y - tr * np.eye(3, dtype=np.complex128))
+        F = np.stack(out, axis=1)
+        if self.layout.upper() == "BXYMIJ":
+            F = np.moveaxis(F, 1, 1 + self.Nd)
+        return F
+
+    def eop_pseudofermion_force_from_solution(self, U, X_e, Y_e):
+        """Force of S = phi_e^dag (S^dag S)^-1 phi_e with the normalized even
+        Schur complement, from X_e and Y_e = S X_e (su3_wilson_nf2.py:1578-1597)."""
+        alpha = 1.0 / (self.m0 * self.m0)
+        f1 = self.pseudofermion_force_from_solution(U, self.apply_K_oe(U, X_e), -alpha * np.asarray(Y_e))
+        f2 = self.pseudofermion_force_from_solution(U, X_e, -alpha * self.apply_Kdag_oe(U, Y_e))
+        return f1 + f2
+
     def normal_residual_stats(self, U, phi, x) -> Dict[str, float]:
         """su3_wilson_nf2.py:1456-1466."""
         r = self.apply_normal(U, x) - phi

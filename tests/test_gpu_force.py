@@ -1,0 +1,115 @@
+"""GPU parity of the analytic pseudofermion force (su3_wilson_nf2.py:1556-1597)
+against the oracle (itself pinned by finite differences of the action): both
+layouts, SU(3) and U(1), twisted boundary phases, two configurations, complex64
+and complex128 inputs, the even-odd force, and at 16^4 the identity
+dS/d eps = sum_links Re tr(H F) for U -> exp(eps H) U with the action
+evaluated by the GPU solver."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def theory(L, nc, mass, bc, batch=1, layout="BMXYIJ", **kw):
+    from jaxqft_amd.su3_wilson_nf2 import SU3WilsonNf2
+    from jaxqft_amd.u1_wilson_nf2 import U1WilsonNf2
+
+    cls = SU3WilsonNf2 if nc == 3 else U1WilsonNf2
+    return cls(lattice_shape=L, mass=mass, fermion_bc=bc, batch_size=batch, dtype=np.complex128, layout=layout, **kw)
+
+
+@pytest.mark.parametrize("layout", ["BMXYIJ", "BXYMIJ"])
+@pytest.mark.parametrize("nc", [3, 1])
+def test_force_matches_oracle(nc, layout):
+    print(f"[gpu] force nc={nc} {layout}", flush=True)
+    from jaxqft_amd import fields
+    from oracle.wilson_oracle import WilsonOracle
+
+    L = (4, 6, 4, 8)
+    bc = (np.exp(0.3j), 1, -1, -1)
+    rng = fields.rng_for(1800 + nc)
+    U = fields.su3_hot_start(rng, 2, L, 0.6, layout=layout) if nc == 3 else fields.u1_hot_start(rng, 2, L, 0.6,
+                                                                                                 layout=layout)
+    X = fields.random_fermion(rng, (2, *L, 4, nc))
+    Y = fields.random_fermion(rng, (2, *L, 4, nc))
+    th = theory(L, nc, 0.05, bc, batch=2, layout=layout)
+    ora = WilsonOracle(L, nc, 0.05, fermion_bc=bc, layout=layout)
+    ref = ora.pseudofermion_force_from_solution(U, X, Y)
+    assert rel(th.pseudofermion_force_from_solution(U, X, Y), ref) < 1e-13
+    got64 = th.pseudofermion_force_from_solution(U, X.astype(np.complex64), Y.astype(np.complex64))
+    assert rel(got64, ref) < 2e-6
+
+
+@pytest.mark.parametrize("nc", [3, 1])
+def test_eo_force_matches_oracle(nc):
+    print(f"[gpu] eo force nc={nc}", flush=True)
+    from jaxqft_amd import fields
+    from oracle.wilson_oracle import WilsonOracle
+
+    L = (4, 4, 4, 8)
+    bc = (1, 1, 1, -1)
+    rng = fields.rng_for(1810 + nc)
+    U = fields.su3_hot_start(rng, 1, L, 0.5) if nc == 3 else fields.u1_hot_start(rng, 1, L, 0.5)
+    ora = WilsonOracle(L, nc, 0.07, fermion_bc=bc)
+    Xe = ora.project_even(fields.random_fermion(rng, (1, *L, 4, nc)))
+    Ye = ora.apply_eo_schur_even(U, Xe, normalized=True)
+    th = theory(L, nc, 0.07, bc)
+    got = th.eop_pseudofermion_force_from_solution(U, Xe, Ye)
+    assert rel(got, ora.eop_pseudofermion_force_from_solution(U, Xe, Ye)) < 1e-12
+
+
+def test_monomial_force_composition():
+    print("[gpu] force from phi", flush=True)
+    from jaxqft_amd import fields
+    from oracle.wilson_oracle import WilsonOracle
+
+    L = (4, 4, 4, 4)
+    rng = fields.rng_for(1820)
+    U = fields.su3_hot_start(rng, 1, L, 0.5)
+    phi = fields.random_fermion(rng, (1, *L, 4, 3))
+    th = theory(L, 3, 0.1, (1, 1, 1, -1), cg_tol=1e-12, cg_maxiter=3000)
+    F = th.pseudofermion_force(U, phi)
+    ora = WilsonOracle(L, 3, 0.1, fermion_bc=(1, 1, 1, -1))
+    X = np.asarray(th.solve_normal(U, phi))
+    ref = ora.pseudofermion_force_from_solution(U, X, ora.apply_D(U, X))
+    assert rel(F, ref) < 1e-12
+
+
+def test_force_is_action_derivative_16x4():
+    """At 16^4, with the action S = phi^dag (D^dag D)^-1 phi evaluated by the GPU
+    solver (FP64, tol 1e-13), (S(eps) - S(-eps)) / 2 eps for U -> exp(eps H) U
+    (H a random su(3) field on every link) equals sum Re tr(H F)."""
+    print("[gpu] force derivative 16^4", flush=True)
+    import torch
+
+    from jaxqft_amd import fields
+
+    L = (16, 16, 16, 16)
+    rng = fields.rng_for(1830)
+    U = torch.from_numpy(fields.su3_hot_start(rng, 1, L, 0.5)).cuda()
+    phi = torch.from_numpy(fields.random_fermion(rng, (1, *L, 4, 3))).cuda()
+    a = torch.from_numpy(rng.standard_normal((1, 4, *L, 3, 3)) + 1j * rng.standard_normal((1, 4, *L, 3, 3))).cuda()
+    H = 0.5 * (a - a.conj().transpose(-1, -2))
+    H = H - (torch.diagonal(H, dim1=-2, dim2=-1).sum(-1) / 3.0)[..., None, None] * torch.eye(3, dtype=H.dtype,
+                                                                                           device=H.device)
+    th = theory(L, 3, 0.1, (1, 1, 1, -1), cg_tol=1e-13, cg_maxiter=4000)
+
+    def action(Uc):
+        x = th.solve_normal(Uc, phi)
+        return float(torch.vdot(phi.flatten(), x.flatten()).real)
+
+    F = th.pseudofermion_force(U, phi)
+    an = float(torch.einsum("...ij,...ji->", H, F).real)
+    eps = 1e-4
+    ex = torch.linalg.matrix_exp(eps * H.reshape(-1, 3, 3)).reshape(H.shape)
+    exm = torch.linalg.matrix_exp(-eps * H.reshape(-1, 3, 3)).reshape(H.shape)
+    fd = (action(ex @ U) - action(exm @ U)) / (2 * eps)
+    assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
+    # the force is traceless anti-Hermitian
+    assert float((F + F.conj().transpose(-1, -2)).abs().max()) < 1e-13 * float(F.abs().max())
