@@ -303,6 +303,31 @@ def main():
         eo["system"] = "S_e^dag S_e x_e = b_e, normalized even Schur complement, FP64 true residual %g" % args.tol
         del xe1, xek
 
+        # analytic pseudofermion force from a solve (X, Y resident, FP64,
+        # reference layouts): HIP events around back-to-back launches
+        with torch.cuda.stream(stream):
+            Xf = bk_t[:1].contiguous()
+            Yf = bk_t[1:2].contiguous()
+            Ff = torch.empty((1, 4, *L, 3, 3), dtype=torch.complex128, device="cuda")
+            for _ in range(2):
+                nat.check(B.lib.wl_pf_force(ctx.h, gauge.h, c_void_p(Xf.data_ptr()), c_void_p(Yf.data_ptr()),
+                                            c_void_p(Ff.data_ptr()), nat.WL_LAYOUT_BMXYIJ, nat.WL_C128, B.h))
+            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            ev0.record(stream)
+            nrep = max(5, args.dslash_reps // 5)
+            for _ in range(nrep):
+                nat.check(B.lib.wl_pf_force(ctx.h, gauge.h, c_void_p(Xf.data_ptr()), c_void_p(Yf.data_ptr()),
+                                            c_void_p(Ff.data_ptr()), nat.WL_LAYOUT_BMXYIJ, nat.WL_C128, B.h))
+            ev1.record(stream)
+            ev1.synchronize()
+        force_us = ev0.elapsed_time(ev1) * 1e3 / nrep
+        force_bytes = (2 * 192 + 4 * 144 + 4 * 144) * V  # X, Y, links read once; F written once
+        force = {"us": round(force_us, 2), "GB/s": round(force_bytes / (force_us * 1e-6) / 1e9, 1),
+                 "frac_hbm_peak": round(force_bytes / (force_us * 1e-6) / 1e9 / PEAK_HBM_GBS, 4),
+                 "bytes_per_site": 2 * 192 + 8 * 144}
+        log(f"force: {force_us:.1f} us {force['GB/s']} GB/s")
+        del Xf, Yf, Ff
+
         # inner stencil of the headline solve, timed on its own (HIP events)
         pc = nat.PREC[args.inner]
         fi, fo = nat.Field(ctx, pc, k), nat.Field(ctx, pc, k)
@@ -355,6 +380,7 @@ def main():
             "speedup_mixed_single_vs_fp64_single": round(fp64_s / mix1_s, 3),
         },
         "even_odd": eo,
+        "pseudofermion_force_fp64": force,
         "config3_single_rhs_delta0.1": config3,
         "dslash_single_rhs": dslash,
         "dslash_multi_rhs": dslash_k,
