@@ -495,8 +495,9 @@ static void fill_eo_args(const wl_ctx* c, DslashArgs& a, int prec, const void* U
   a.ntiles = (int)((c->Vh + NT - 1) / NT);
   a.march = 1;
   a.nseg = c->geom.L[0];
-  // right-hand sides per thread: the multi-RHS setting, at most 2
-  a.mrhs = (nrhs > 1 && c->geom.nc == 3 && nrhs % 2 == 0 && mrhs_for(c, prec, nrhs) >= 2) ? 2 : 1;
+  // two right-hand sides per thread share each link load whenever the
+  // right-hand sides per configuration pair up (SU(3))
+  a.mrhs = (nrhs > 1 && c->geom.nc == 3 && nrhs % 2 == 0) ? 2 : 1;
   a.xcd_remap = c->xcd_remap[nrhs > 1 ? 1 : 0][prec];
   a.normalized = 1;
 }
