@@ -19,7 +19,7 @@ import json
 import os
 import sys
 import time
-from ctypes import byref, c_void_p
+from ctypes import byref, c_int, c_void_p
 from pathlib import Path
 
 import numpy as np
@@ -66,6 +66,7 @@ def parse():
     ap.add_argument("--serving-requests", type=int, default=60)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-lime", action="store_true", help="skip the SciDAC payload decode")
     ap.add_argument("--profile-only", action="store_true",
                     help="run only the Dslash variants and a calibration copy (for rocprofv3 counter passes)")
     return ap.parse_args()
@@ -229,12 +230,14 @@ def main():
             dslash[f"{args.inner}_r{args.recon}_k{k}"] = {"us": round(us_k, 2), "tuning": ctx.get_tuning(pc, k),
                                                           "timed_launches": args.dslash_reps}
             del fi, fo
+            lime_prof = lime_decode(B, args, L, False)
             src = torch.empty(32 << 20, dtype=torch.float64, device="cuda")
             dst = torch.empty_like(src)
             for _ in range(5):
                 dst.copy_(src)
             torch.cuda.synchronize()
-            print(json.dumps({"profile_only": True, "dslash": dslash, "copy_bytes_each_way": src.numel() * 8}))
+            print(json.dumps({"profile_only": True, "dslash": dslash, "lime_decode": lime_prof,
+                              "copy_bytes_each_way": src.numel() * 8}))
             return
 
         # ---------------------------------------- Dslash (k RHS, shared links)
@@ -414,6 +417,8 @@ def main():
         out["multi_rhs_24x24x24x48"] = config4(B, args)
     if not args.no_serving:
         out["serving"] = serving(B, args)
+    if not args.no_lime:
+        out["lime_decode"] = lime_decode(B, args, L, not args.no_cpu)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(U, b_host[:1], L, args, st64[0]["iters"])
     out["bench_wall_s"] = round(time.perf_counter() - t_start, 1)
@@ -499,6 +504,81 @@ def serving(B, args):
         "throughput_ratio": round(res["rhs_per_s"] / res64["rhs_per_s"], 3),
         "max_rel_diff_vs_fp64_queue": diff,
     }
+
+
+def lime_decode(B, args, L, with_cpu):
+    """SciDAC gauge payload decode at the bench lattice: precision D, four
+    links per site, sites lexicographic and big-endian, decoded by
+    wl_lime_decode into BM...IJ complex128 (byte swap + axis reversal + the
+    direction axis moved first).  Algorithmic bytes: the payload read once
+    and the links written once, 576 + 576 B per site.  Also times the
+    payload staged from pinned host memory, and the NumPy oracle decode of
+    the same payload on one host core."""
+    import torch
+
+    nat = B.nat
+    V = int(np.prod(L))
+    stream = B.stream
+    with torch.cuda.stream(stream):
+        g = torch.Generator(device="cuda").manual_seed(11)
+        U = torch.randn((4, *L, 3, 3, 2), generator=g, device="cuda", dtype=torch.float64)
+        pay = U.permute(4, 3, 2, 1, 0, 5, 6, 7).contiguous().view(torch.uint8).view(-1, 8).flip(-1).reshape(-1)
+        pay = pay.contiguous()
+        Uc = torch.view_as_complex(U).reshape(4, *L, 9)
+        out = torch.empty_like(Uc)
+        cd = (c_int * 4)(*L)
+
+        def run(src):
+            nat.check(B.lib.wl_lime_decode(c_void_p(src.data_ptr()), src.numel(), 8, cd, 4, 4, 9, nat.WL_LIME_COUNT,
+                                           c_void_p(out.data_ptr()), nat.WL_C128, B.h))
+
+        run(pay)
+        stream.synchronize()
+        exact = bool(torch.equal(out, Uc))
+        nrep = max(5, args.dslash_reps // 2)
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        for _ in range(nrep):
+            run(pay)
+        ev1.record(stream)
+        ev1.synchronize()
+        us = ev0.elapsed_time(ev1) * 1e3 / nrep
+        nb = pay.numel()
+        host = torch.empty(nb, dtype=torch.uint8, pin_memory=True)
+        host.copy_(pay)
+        dev = torch.empty_like(pay)
+        ev0.record(stream)
+        for _ in range(3):
+            dev.copy_(host, non_blocking=True)
+            run(dev)
+        ev1.record(stream)
+        ev1.synchronize()
+        staged_ms = ev0.elapsed_time(ev1) / 3
+        exact = exact and bool(torch.equal(out, Uc))
+    res = {
+        "payload_bytes": nb, "kernel_us": round(us, 2), "launches": nrep,
+        "GB/s": round(2 * nb / (us * 1e-6) / 1e9, 1), "frac_hbm_peak": round(2 * nb / (us * 1e-6) / 1e9 / PEAK_HBM_GBS, 4),
+        "bytes_per_site": 2 * 576, "bit_exact": exact,
+        "pinned_host_to_links_ms": round(staged_ms, 3),
+        "pinned_host_to_links_GB/s": round(nb / (staged_ms * 1e-3) / 1e9, 2),
+    }
+    log(f"lime decode: {us:.1f} us {res['GB/s']} GB/s, from pinned host {staged_ms:.2f} ms, exact {exact}")
+    if with_cpu:
+        from oracle import lime_oracle as lo
+
+        raw = host.numpy()
+        t0 = time.perf_counter()
+        lat = lo.decode_payload(raw, "D", 144, 4, L, "QDP_D3_ColorMatrix", 3, 4)["lattice_data"]
+        Uh = np.ascontiguousarray(lo.gauge_from_lattice(lat, 4))
+        cpu_s = time.perf_counter() - t0
+        res["cpu_oracle"] = {"s": round(cpu_s, 3), "GB/s": round(nb / cpu_s / 1e9, 3), "cores": 1,
+                             "sample": "the whole payload decoded to a contiguous BM...IJ array by the NumPy oracle",
+                             "matches_gpu": bool(np.array_equal(Uh[0].reshape(4, *L, 9), out.cpu().numpy()))}
+        log(f"lime decode cpu oracle: {cpu_s:.2f} s")
+        del lat, Uh
+    del U, pay, Uc, out, host, dev
+    torch.cuda.synchronize()
+    return res
 
 
 def cpu_baseline(U, b, L, args, gpu_iters):

@@ -230,6 +230,21 @@ int wl_pf_force(wl_ctx* ctx, wl_gauge* g, const void* X, const void* Y, void* F,
 int wl_cg_eo_normal(wl_ctx* ctx, wl_gauge* g, const void* b, void* x, const void* x0, int nrhs, int dtype,
                     const wl_solve_params* sp, wl_solve_stats* stats, void* stream);
 
+/* Output orders of wl_lime_decode. */
+#define WL_LIME_LEXI 0  /* (V, datacount, ncomplex), sites lexicographic (x0 fastest): site_data, lime.py:296-297 */
+#define WL_LIME_SITE 1  /* (L0, .., L[nd-1], datacount, ncomplex), last axis fastest: lattice_data, :236-244 */
+#define WL_LIME_COUNT 2 /* (datacount, L0, .., L[nd-1], ncomplex): BM...IJ links, decode_scidac_gauge :334-352 */
+/* Decode the binary payload of a SciDAC field record (replaces the payload
+ * decode of decode_scidac_field, jaxqft/io/lime.py:247-331).  payload: device
+ * copy of the record's nbytes, big-endian IEEE reals of word_bytes 4 (SciDAC
+ * precision F) or 8 (D); for each site in lexicographic order, datacount items
+ * of ncomplex (re, im) values.  dims[nd] (nd 1..4): lattice extents, x0 first.
+ * out: device array of complex128 (WL_C128) or complex64 (WL_C64) in the given
+ * order; narrowing D to complex64 rounds to nearest.  nbytes must equal
+ * V * datacount * ncomplex * 2 * word_bytes. */
+int wl_lime_decode(const void* payload, size_t nbytes, int word_bytes, const int* dims, int nd, int datacount,
+                   int ncomplex, int order, void* out, int dtype, void* stream);
+
 /* Continuous-batching solve engine (no counterpart in the reference; it serves
  * the point-source propagator loop of core/measurements.py:277-371 and request
  * streams).  A batch holds `capacity` columns on one resident configuration

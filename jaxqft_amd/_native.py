@@ -26,6 +26,7 @@ WL_OP_D, WL_OP_DDAG, WL_OP_G5D, WL_OP_K, WL_OP_KDAG, WL_OP_DDAGD = 0, 1, 2, 3, 4
 (WL_EO_K_OE, WL_EO_K_EO, WL_EO_KDAG_OE, WL_EO_KDAG_EO, WL_EO_SCHUR_E, WL_EO_SCHUR_E_DAG, WL_EO_SCHUR_O,
  WL_EO_SCHUR_O_DAG) = range(8)
 WL_STOP_TRUE_RESIDUAL, WL_STOP_ITERATED = 0, 1
+WL_LIME_LEXI, WL_LIME_SITE, WL_LIME_COUNT = 0, 1, 2
 WL_ACCUM_AUTO, WL_ACCUM_F64, WL_ACCUM_F32, WL_ACCUM_F16 = 0, 1, 2, 3
 ACCUM = {None: WL_ACCUM_AUTO, "auto": WL_ACCUM_AUTO, "f64": WL_ACCUM_F64, "f32": WL_ACCUM_F32, "f16": WL_ACCUM_F16}
 STATUS_NAMES = {-1: "running", 0: "converged", 1: "maxiter", 2: "nan", 3: "breakdown"}
@@ -121,6 +122,7 @@ def load() -> ctypes.CDLL:
         "wl_eo_apply": (c_int, [vp, vp, c_int, c_int, vp, vp, c_int, c_int, vp]),
         "wl_pf_force": (c_int, [vp, vp, vp, vp, vp, c_int, c_int, vp]),
         "wl_cg_eo_normal": (c_int, [vp, vp, vp, vp, vp, c_int, c_int, POINTER(WlSolveParams), POINTER(WlSolveStats), vp]),
+        "wl_lime_decode": (c_int, [vp, c_size_t, c_int, POINTER(c_int), c_int, c_int, c_int, c_int, vp, c_int, vp]),
     }
     sig.update({
         "wl_batch_bytes": (c_size_t, [vp, c_int, c_int]),

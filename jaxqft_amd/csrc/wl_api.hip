@@ -35,6 +35,7 @@ hipError_t convert_gauge(int prec, int nc, int recon, const void* src, void* dst
                          hipStream_t st);
 hipError_t launch_eo(int prec, int nc, int recon, int dag, int mode, int epi, const DslashArgs& a, hipStream_t stream);
 hipError_t launch_pf_force(int nc, int dtype, const ForceArgs& a, int ncfg, hipStream_t st);
+hipError_t launch_lime_decode(int word_bytes, int out_double, const LimeArgs& a, size_t lds_bytes, hipStream_t st);
 hipError_t pack_spinor_eo(int prec, int nc, const void* src, int dtype, void* dst, const Geom& g, long long Vh,
                           long long Vhp, int q, int ncol, size_t col_bytes, hipStream_t st);
 hipError_t unpack_spinor_eo(int prec, int nc, const void* src, void* dst, int dtype, const Geom& g, long long Vhp, int q,
@@ -1433,6 +1434,64 @@ int wl_batch_retire(wl_batch* bt, int slot, void* x, int dtype) {
   }
   if (e != hipSuccess) return fail(WL_ERR_HIP, "retire: %s", hipGetErrorString(e));
   bt->busy[slot] = 0;
+  return WL_OK;
+}
+
+int wl_lime_decode(const void* payload, size_t nbytes, int word_bytes, const int* dims, int nd, int datacount,
+                   int ncomplex, int order, void* out, int dtype, void* stream) {
+  if (!payload || !out || !dims) return fail(WL_ERR_BAD_ARG, "null argument");
+  if (word_bytes != 4 && word_bytes != 8) return fail(WL_ERR_BAD_ARG, "word_bytes must be 4 or 8, got %d", word_bytes);
+  if (dtype != WL_C128 && dtype != WL_C64) return fail(WL_ERR_BAD_ARG, "bad dtype %d", dtype);
+  if (order < WL_LIME_LEXI || order > WL_LIME_COUNT) return fail(WL_ERR_BAD_ARG, "bad order %d", order);
+  if (nd < 1 || nd > 4) return fail(WL_ERR_UNSUPPORTED, "lattices of 1 to 4 dimensions, got %d", nd);
+  if (datacount < 1 || ncomplex < 1) return fail(WL_ERR_BAD_ARG, "datacount and ncomplex must be >= 1");
+  size_t V = 1;
+  for (int i = 0; i < nd; ++i) {
+    if (dims[i] < 1) return fail(WL_ERR_BAD_ARG, "dims[%d] = %d", i, dims[i]);
+    V *= (size_t)dims[i];
+  }
+  const size_t Wc = (size_t)datacount * (size_t)ncomplex;
+  if (Wc > (size_t)1 << 24 || V > (size_t)1 << 36) return fail(WL_ERR_UNSUPPORTED, "field too large");
+  const size_t want = V * Wc * 2 * (size_t)word_bytes;
+  if (nbytes != want) return fail(WL_ERR_BAD_ARG, "payload holds %zu bytes, the field needs %zu", nbytes, want);
+  if (((uintptr_t)payload) % (2 * word_bytes) != 0) return fail(WL_ERR_BAD_ARG, "payload not aligned to one complex value");
+  const int osz = dtype == WL_C128 ? 16 : 8;
+  if (((uintptr_t)out) % osz != 0) return fail(WL_ERR_BAD_ARG, "output not aligned to one complex value");
+  LimeArgs a;
+  memset(&a, 0, sizeof(a));
+  a.in = payload;
+  a.out = out;
+  // pad to four axes with x0 first and the last axis in x3
+  int L[4] = {1, 1, 1, 1};
+  L[0] = dims[0];
+  if (nd >= 2) L[3] = dims[nd - 1];
+  if (nd >= 3) L[1] = dims[1];
+  if (nd == 4) L[2] = dims[2];
+  for (int i = 0; i < 4; ++i) a.L[i] = L[i];
+  a.V = (long long)V;
+  a.Wc = (int)Wc;
+  a.item = ncomplex;
+  a.dc = datacount;
+  a.order = order;
+  // tile of up to 16 x 16 site records, halved until it fits 40 KiB of LDS
+  const size_t rec = Wc * 2 * (size_t)word_bytes;
+  int T0 = std::min(L[0], 16), T3 = std::min(L[3], 16);
+  while ((size_t)T0 * T3 * rec > 40960 && (T0 > 1 || T3 > 1)) {
+    if (T3 >= T0 && T3 > 1)
+      T3 = (T3 + 1) / 2;
+    else
+      T0 = (T0 + 1) / 2;
+  }
+  const size_t lds = (size_t)T0 * T3 * rec;
+  if (lds > 65536) return fail(WL_ERR_UNSUPPORTED, "a site record of %zu bytes does not fit in LDS", rec);
+  a.T0 = T0;
+  a.T3 = T3;
+  a.nt0 = (L[0] + T0 - 1) / T0;
+  a.nt3 = (L[3] + T3 - 1) / T3;
+  const long long nblk = (long long)a.nt0 * a.nt3 * L[1] * L[2];
+  if (nblk > 0x7fffffffLL) return fail(WL_ERR_UNSUPPORTED, "field too large");
+  hipError_t e = launch_lime_decode(word_bytes, dtype == WL_C128, a, lds, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(WL_ERR_HIP, "lime decode: %s", hipGetErrorString(e));
   return WL_OK;
 }
 

@@ -145,6 +145,20 @@ struct ForceArgs {
   double r;
 };
 
+// SciDAC LIME payload decode (wl_lime.hip).
+struct LimeArgs {
+  const void* in;  // big-endian payload, sites lexicographic
+  void* out;       // complex output in the chosen order
+  int L[4];        // lattice padded to four axes
+  long long V;
+  int Wc;          // complex values per site record (dc * item)
+  int item;        // complex values per item
+  int dc;          // items per site
+  int order;       // 0 LEXI, 1 SITE, 2 COUNT
+  int T0, T3;      // tile extents along x0 and x3
+  int nt0, nt3;    // tiles along x0 and x3
+};
+
 struct BlasArgs {
   long long V;
   long long Vp;
