@@ -1473,14 +1473,16 @@ int wl_lime_decode(const void* payload, size_t nbytes, int word_bytes, const int
   a.item = ncomplex;
   a.dc = datacount;
   a.order = order;
-  // tile of up to 16 x 16 site records, halved until it fits 40 KiB of LDS
+  // tile of up to 16 x 16 site records shrunk to 20 KiB of LDS, x0 first:
+  // the output runs along x3 (n3 items of a direction for links) stay long,
+  // and several workgroups share a CU (a 32^4 FP64 link field: 4 x 8)
   const size_t rec = Wc * 2 * (size_t)word_bytes;
   int T0 = std::min(L[0], 16), T3 = std::min(L[3], 16);
-  while ((size_t)T0 * T3 * rec > 40960 && (T0 > 1 || T3 > 1)) {
-    if (T3 >= T0 && T3 > 1)
-      T3 = (T3 + 1) / 2;
-    else
+  while ((size_t)T0 * T3 * rec > 20480 && (T0 > 1 || T3 > 1)) {
+    if (T0 > 1 && 2 * T0 >= T3)
       T0 = (T0 + 1) / 2;
+    else
+      T3 = (T3 + 1) / 2;
   }
   const size_t lds = (size_t)T0 * T3 * rec;
   if (lds > 65536) return fail(WL_ERR_UNSUPPORTED, "a site record of %zu bytes does not fit in LDS", rec);

@@ -60,7 +60,10 @@ struct OutComplex<double> {
   static __device__ __forceinline__ T make(V v) { return make_double2((double)v.x, (double)v.y); }
 };
 
-constexpr int kLimeThreads = 256;
+#ifndef WL_LIME_THREADS
+#define WL_LIME_THREADS 256
+#endif
+constexpr int kLimeThreads = WL_LIME_THREADS;
 
 template <int WB, class O>
 __global__ __launch_bounds__(kLimeThreads) void k_lime_decode(LimeArgs a) {

@@ -14,9 +14,9 @@ pytestmark = pytest.mark.gpu
 CASES = [
     # datatype, precision, colors, spins, ncomplex, datacount, dims
     ("USQCD_F3_ColorMatrix", "F", 3, 4, 9, 4, (4, 3, 2, 5)),
-    ("USQCD_D3_ColorMatrix", "D", 3, 4, 9, 4, (10, 3, 2, 13)),   # 10 x 7 tiles: x3 in runs of 7 and 6
-    ("QDP_D3_DiracFermion", "D", 3, 4, 12, 1, (18, 2, 3, 11)),   # 16 x 11 tiles: x0 in runs of 16 and 2
-    ("Lattice", "F", 3, 4, 12, 1, (33, 2, 2, 35)),               # 16 x 16 tiles, ragged on both
+    ("USQCD_D3_ColorMatrix", "D", 3, 4, 9, 4, (10, 3, 2, 13)),   # 5 x 7 tiles: x3 in runs of 7 and 6
+    ("QDP_D3_DiracFermion", "D", 3, 4, 12, 1, (18, 2, 3, 11)),   # 8 x 11 tiles: x0 in runs of 8, 8 and 2
+    ("Lattice", "F", 3, 4, 12, 1, (33, 2, 2, 35)),               # 8 x 16 tiles, ragged on both axes
     ("Lattice", "F", 3, 4, 12, 1, (5, 1, 3, 2)),
     ("QDP_F3_ColorVector", "F", 3, 0, 3, 2, (3, 4, 5)),
     ("QDP_D1_ColorMatrix", "D", 1, 0, 1, 2, (6, 5)),
