@@ -146,7 +146,8 @@ def measured_traffic(args, k, L, tuning):
     if any(d.get(key) != val for key, val in want.items()):
         return None, None
     t = d.get("tuning", {})
-    if any(t.get(key) != tuning.get(key) for key in ("march", "rhs_per_thread", "slab", "block_shape", "cols_outer")):
+    if any(t.get(key, 0) != tuning.get(key, 0)
+           for key in ("march", "rhs_per_thread", "slab", "block_shape", "cols_outer", "plane")):
         return None, None
     return int(d["fetch_bytes"] + d["write_bytes"]), "profiles/" + PROFILE_TRAFFIC.name
 

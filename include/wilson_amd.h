@@ -158,9 +158,13 @@ typedef struct wl_tuning {
   int slab;           /* one-site-per-thread kernels: 0 = flat site order; s > 0 (must
                          divide L1) = tiles walk slabs of s consecutive x1 values,
                          x0 fastest, so that x0 neighbours are reused from L2 */
-  int cols_outer;     /* several spinor columns: 0 = the columns of one tile run back to
-                         back (links shared in L2); 1 = one column group sweeps the
-                         lattice before the next (spinor neighbours shared in L2) */
+  int cols_outer;     /* several spinor columns: 0 = the columns (column groups of a
+                         multi-RHS kernel) of one tile run back to back (links shared in
+                         L2); n = 1..64 = blocks of n columns (groups) sweep the lattice
+                         in turn, the columns of a block back to back per tile, so that
+                         a block's spinor neighbours fit in L2 and its links are shared
+                         (1 = one column sweeps the lattice before the next); a size
+                         that does not divide the columns is lowered until it does */
   int block_shape;    /* one-site-per-thread kernels: 0 = flat / slab order; k = 1..8 =
                          each workgroup covers a compact 4D box of 256 sites (preset k,
                          x3 extent >= 8 so loads stay whole cache lines), which keeps
@@ -172,6 +176,11 @@ typedef struct wl_tuning {
   int blocks_per_cu;  /* 0 = as many resident workgroups per CU as registers allow;
                          n > 0 = at most n (by reserving LDS), which narrows the window
                          of lattice sites in flight on each XCD to what its L2 holds */
+  int plane;          /* FP16, one right-hand side per thread: 0 = off; n > 0 (must divide
+                         L1) = the LDS-staged plane kernel, each workgroup marching n x1
+                         values over a tile of full x3 rows with its x2 and x3 neighbours
+                         read from LDS (needs 2 L3 <= 256); takes precedence over march,
+                         slab and block_shape */
 } wl_tuning;
 /* wl_ctx_get_tuning reports the values in effect for nrhs right-hand sides. */
 int wl_ctx_set_tuning(wl_ctx* ctx, int prec, int multi_rhs, const wl_tuning* t);

@@ -40,16 +40,22 @@ def time_dslash(ctx: nat.Context, gauge: nat.Gauge, prec: int, recon: int, fin: 
 def tune(ctx: nat.Context, gauge: nat.Gauge, prec: int, recon: int, fin: nat.Field, fout: nat.Field,
          nrhs: int = 1, marches: Iterable[int] = (1, 2, 4, 8), reps: int = 20,
          stream: Optional[torch.cuda.Stream] = None, slabs: Iterable[int] = (0, 2, 4, 8),
-         cols_outer: Iterable[bool] = (False,), boxes: Iterable[int] = (1, 2, 7)) -> Dict[str, object]:
+         cols_outer: Iterable[int] = (0, 2, 3), boxes: Iterable[int] = (1, 2, 7),
+         mrhs_marches: Iterable[int] = (2, 4, 8, 16), planes: Iterable[int] = (2, 4, 8)) -> Dict[str, object]:
     """Pick the fastest launch shape for (prec, recon) at nrhs right-hand sides
     per configuration and install it.  Candidates: every march length dividing
     L0 with one right-hand side per thread, plus 2 and 4 right-hand sides per
-    thread when nrhs > 1; the one-site-per-thread shapes in flat and x1-slab
+    thread when nrhs > 1 (for FP16 also two per thread marching the lengths of
+    `mrhs_marches` along x0, and the LDS-staged plane kernel marching the
+    lengths of `planes` along x1); the one-site-per-thread shapes in flat and x1-slab
     site order (slab extents dividing L1); each in XCD-contiguous and hardware
     workgroup order, and for nrhs > 1 in each column order of `cols_outer`
-    (column-fastest only by default: sweeping the lattice once per column
-    group re-reads the links and measured slower at 32^4).
-    Returns {"march", "rhs_per_thread", "xcd_remap", "slab", "cols_outer", "us": {label: us}}."""
+    (0: the columns of a tile back to back; n: blocks of n columns, or column
+    groups, sweeping the lattice in turn, which keeps a block's spinor
+    neighbours in L2 while its links stay shared; sweeping one column at a
+    time, n = 1, re-reads the links and measured slower at 32^4).
+    Returns {"march", "rhs_per_thread", "xcd_remap", "slab", "block_shape", "plane", "cols_outer",
+    "us": {label: us}}."""
     stream = stream or torch.cuda.current_stream()
     L0, L1 = ctx.lattice_shape[0], ctx.lattice_shape[1]
     multi = nrhs > 1
@@ -62,13 +68,21 @@ def tune(ctx: nat.Context, gauge: nat.Gauge, prec: int, recon: int, fin: nat.Fie
         if L0 % m:
             continue
         shapes += [(m, 1, o) for o in (orders1 if m == 1 else [("s", 0)])]
+    if prec == nat.WL_F16:
+        # the LDS-staged plane kernel, marching the lengths of `planes` along x1
+        L1v, L3v = ctx.lattice_shape[1], ctx.lattice_shape[3]
+        if 2 * L3v <= 256:
+            shapes += [(1, 1, ("p", p)) for p in planes if L1v % p == 0]
     if multi:
         shapes += [(1, k, o) for k in (2, 4) if nrhs % k == 0 for o in orders1]
-    corders = [bool(c) for c in cols_outer] if multi else [False]
+        if prec == nat.WL_F16 and nrhs % 2 == 0:
+            # the FP16 two-RHS kernel marching along x0
+            shapes += [(m, 2, ("s", 0)) for m in mrhs_marches if m > 1 and L0 % m == 0]
+    corders = [int(c) for c in cols_outer] if multi else [0]
 
     def install(m, k, r, o, co):
         ctx.set_tuning(prec, multi, m, k if multi else 0, bool(r), o[1] if o[0] == "s" else 0, co, 0,
-                       o[1] if o[0] == "b" else 0)
+                       o[1] if o[0] == "b" else 0, o[1] if o[0] == "p" else 0)
 
     times = {}
     for co in corders:
@@ -80,8 +94,8 @@ def tune(ctx: nat.Context, gauge: nat.Gauge, prec: int, recon: int, fin: nat.Fie
     install(*best)
     m, k, r, o, co = best
     return {"march": m, "rhs_per_thread": k, "xcd_remap": r, "slab": o[1] if o[0] == "s" else 0,
-            "block_shape": o[1] if o[0] == "b" else 0, "cols_outer": co,
-            "us": {f"m{m}k{k}x{r}{o[0]}{o[1]}" + ("c" if co else ""): t for (m, k, r, o, co), t in times.items()}}
+            "block_shape": o[1] if o[0] == "b" else 0, "plane": o[1] if o[0] == "p" else 0, "cols_outer": co,
+            "us": {f"m{m}k{k}x{r}{o[0]}{o[1]}" + (f"c{co}" if co else ""): t for (m, k, r, o, co), t in times.items()}}
 
 
 # 4D workgroup box presets (log2 extents per direction), as in the library
@@ -99,4 +113,4 @@ def tune_march(ctx, gauge, prec, recon, fin, fout, nrhs: int = 1, candidates=(1,
     """Single-RHS form of `tune`: {"best": march, "xcd_remap": r, "us": ...}."""
     t = tune(ctx, gauge, prec, recon, fin, fout, nrhs, candidates, reps, stream)
     return {"best": t["march"], "xcd_remap": t["xcd_remap"], "slab": t["slab"], "block_shape": t["block_shape"],
-            "us": t["us"]}
+            "plane": t["plane"], "us": t["us"]}

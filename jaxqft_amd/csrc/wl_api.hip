@@ -121,9 +121,10 @@ struct wl_ctx {
   int mrhs[3] = {0, 0, 0};                       // right-hand sides per thread (0 = automatic)
   int xcd_remap[2][3] = {{1, 1, 1}, {1, 1, 1}};  // XCD-contiguous workgroup order
   int slab[2][3] = {{0, 0, 0}, {0, 0, 0}};       // x1 extent of a slab (0 = flat site order)
-  int cols_outer[2][3] = {{0, 0, 0}, {0, 0, 0}}; // column group is the slowest workgroup index
+  int cols_outer[2][3] = {{0, 0, 0}, {0, 0, 0}}; // column order (0 fastest, n >= 1 blocks of n columns or groups)
   int blocks_per_cu[2][3] = {{0, 0, 0}, {0, 0, 0}};  // resident workgroup cap (0 = none)
   int block_shape[2][3] = {{0, 0, 0}, {0, 0, 0}};    // 4D box preset of one-site kernels (0 = none)
+  int plane[2][3] = {{0, 0, 0}, {0, 0, 0}};          // x1 march of the LDS plane kernel (0 = off)
   int lds_per_cu = 0;     // bytes of LDS per CU
   int lds_per_block = 0;  // largest LDS allocation of one workgroup
 };
@@ -172,6 +173,13 @@ static int march_for(const wl_ctx* c, int prec, int nrhs) {
   return 1;
 }
 
+// x0 sites marched per thread by a multi-RHS launch: only the FP16 two-RHS
+// march kernel marches, and only when a march length is set explicitly
+static int march_mrhs(const wl_ctx* c, int prec, int k) {
+  const int m = c->march[1][prec];
+  return (prec == F16 && k == 2 && m > 1) ? m : 1;
+}
+
 // right-hand sides per thread: the largest of (4, 2) dividing the number of
 // right-hand sides per configuration (FP64: 2), unless set explicitly.
 static int mrhs_for(const wl_ctx* c, int prec, int nrhs) {
@@ -198,7 +206,20 @@ static inline int site_tiles(const wl_ctx* c, int slab) {
 static inline size_t max_tiles(const wl_ctx* c) {
   size_t a = (size_t)nchunk_of(c) * c->geom.L[0];
   size_t b = (size_t)((c->V + NT - 1) / NT) + (size_t)c->geom.L[0] * c->geom.L[1];
-  return a > b ? a : b;
+  // the plane kernel: at most one tile per (x0, x1, x2)
+  const size_t d = (size_t)c->geom.L[0] * c->geom.L[1] * c->geom.L[2];
+  if (b > a) a = b;
+  return a > d ? a : d;
+}
+
+// x2 rows of a plane-kernel tile: the largest divisor of L2 whose rows fit one
+// workgroup, or 0 when the lattice does not fit the kernel
+static int plane_rows(const wl_ctx* c) {
+  const int L2 = c->geom.L[2], L3 = c->geom.L[3];
+  if (2 * L3 > NT) return 0;
+  for (int p = NT / L3; p >= 1; --p)
+    if (L2 % p == 0) return p;
+  return 0;
 }
 
 
@@ -291,6 +312,9 @@ int wl_ctx_set_tuning(wl_ctx* c, int prec, int multi_rhs, const wl_tuning* t) {
                 t->block_shape);
   if (t->blocks_per_cu < 0 || t->blocks_per_cu > 32)
     return fail(WL_ERR_BAD_ARG, "blocks_per_cu must be 0 (no cap) .. 32");
+  if (t->cols_outer < 0 || t->cols_outer > 64) return fail(WL_ERR_BAD_ARG, "cols_outer must be 0 .. 64");
+  if (t->plane < 0 || (t->plane > 0 && c->geom.L[1] % t->plane != 0))
+    return fail(WL_ERR_BAD_ARG, "plane %d must be 0 (off) or divide L1=%d", t->plane, c->geom.L[1]);
   if (t->slab < 0 || (t->slab > 0 && c->geom.L[1] % t->slab != 0))
     return fail(WL_ERR_BAD_ARG, "slab %d must be 0 (flat) or divide L1=%d", t->slab, c->geom.L[1]);
   std::lock_guard<std::mutex> lk(c->mu);
@@ -298,9 +322,10 @@ int wl_ctx_set_tuning(wl_ctx* c, int prec, int multi_rhs, const wl_tuning* t) {
   c->march[m][prec] = t->march;
   c->xcd_remap[m][prec] = t->xcd_remap ? 1 : 0;
   c->slab[m][prec] = t->slab;
-  c->cols_outer[m][prec] = t->cols_outer ? 1 : 0;
+  c->cols_outer[m][prec] = t->cols_outer;
   c->blocks_per_cu[m][prec] = t->blocks_per_cu;
   c->block_shape[m][prec] = t->block_shape;
+  c->plane[m][prec] = t->plane;
   if (multi_rhs) c->mrhs[prec] = t->rhs_per_thread;
   return WL_OK;
 }
@@ -311,13 +336,14 @@ int wl_ctx_get_tuning(wl_ctx* c, int prec, int nrhs, wl_tuning* t) {
   if (prec < F64 || prec > F16 || nrhs < 1) return fail(WL_ERR_BAD_ARG, "bad precision or nrhs");
   const int k = mrhs_for(c, prec, nrhs);
   const int m = nrhs > 1 ? 1 : 0;
-  t->march = k > 1 ? 1 : march_for(c, prec, nrhs);
+  t->march = k > 1 ? march_mrhs(c, prec, k) : march_for(c, prec, nrhs);
   t->rhs_per_thread = k;
   t->xcd_remap = c->xcd_remap[m][prec];
   t->slab = t->march == 1 ? c->slab[m][prec] : 0;
   t->cols_outer = c->cols_outer[m][prec];
   t->blocks_per_cu = c->blocks_per_cu[m][prec];
   t->block_shape = t->march == 1 ? c->block_shape[m][prec] : 0;
+  t->plane = c->plane[m][prec];
   return WL_OK;
 }
 
@@ -438,7 +464,7 @@ static int gauge_encoding_eo(wl_gauge* g, int prec, int recon, hipStream_t st, c
 // launch geometry once ncol and cols_per_cfg are set
 static void set_launch_shape(const wl_ctx* c, DslashArgs& a, int prec) {
   a.mrhs = mrhs_for(c, prec, a.cols_per_cfg);
-  a.march = a.mrhs > 1 ? 1 : march_for(c, prec, a.cols_per_cfg);
+  a.march = a.mrhs > 1 ? march_mrhs(c, prec, a.mrhs) : march_for(c, prec, a.cols_per_cfg);
   a.xcd_remap = c->xcd_remap[a.cols_per_cfg > 1 ? 1 : 0][prec];
   a.slab = a.march == 1 ? c->slab[a.cols_per_cfg > 1 ? 1 : 0][prec] : 0;
   {
@@ -450,7 +476,15 @@ static void set_launch_shape(const wl_ctx* c, DslashArgs& a, int prec) {
     }
     if (a.box) a.slab = 0;
   }
-  a.cols_outer = c->cols_outer[a.cols_per_cfg > 1 ? 1 : 0][prec];
+  {
+    // column blocks: the largest block size not above the setting that
+    // divides the number of column units (columns, or groups of mrhs)
+    const int units = a.ncol / (a.mrhs > 1 ? a.mrhs : 1);
+    int cb = c->cols_outer[a.cols_per_cfg > 1 ? 1 : 0][prec];
+    if (cb > units) cb = units;
+    while (cb > 1 && units % cb != 0) --cb;
+    a.cols_outer = cb;
+  }
   {
     // LDS reservation that leaves room for at most n resident workgroups per CU
     const int n = c->blocks_per_cu[a.cols_per_cfg > 1 ? 1 : 0][prec];
@@ -465,6 +499,22 @@ static void set_launch_shape(const wl_ctx* c, DslashArgs& a, int prec) {
   a.slab_tiles = a.slab ? (int)(((long long)a.slab * c->geom.L[2] * c->geom.L[3] + NT - 1) / NT) : 0;
   a.nseg = c->geom.L[0] / a.march;
   a.ntiles = a.march == 1 ? (a.box ? (int)(c->V / NT) : site_tiles(c, a.slab)) : nchunk_of(c) * a.nseg;
+  // LDS-staged plane kernel (FP16, one column per thread): replaces the shape above
+  a.plane = 0;
+  a.prow = 0;
+  const int pm = c->plane[a.cols_per_cfg > 1 ? 1 : 0][prec];
+  const int prow = plane_rows(c);
+  if (pm > 0 && prec == F16 && a.mrhs == 1 && prow > 0 && c->geom.L[1] % pm == 0) {
+    a.plane = 1;
+    a.prow = prow;
+    a.march = pm;
+    a.slab = 0;
+    a.box = 0;
+    a.nseg = c->geom.L[1] / pm;
+    a.ntiles = c->geom.L[0] * (c->geom.L[2] / prow) * a.nseg;
+    const int need = (prow + 2) * c->geom.L[3] * (32 * c->geom.nc + 8);
+    if (a.lds_bytes < need) a.lds_bytes = need;
+  }
 }
 
 static void fill_dslash_common(const wl_ctx* c, DslashArgs& a, int prec) {

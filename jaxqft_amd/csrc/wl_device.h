@@ -330,4 +330,21 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned n, int remap =
   return (xcd < rem) ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
 }
 
+// Split a logical workgroup index over (column unit, tile) for n units and
+// ntiles tiles.  cb = 0: units fastest (the units of one tile run back to
+// back); cb >= 1 (dividing n): blocks of cb units, each block sweeping every
+// tile with its units fastest, blocks slowest (cb = 1: units slowest).
+__device__ __forceinline__ void split_units(unsigned logical, int ntiles, int n, int cb, int& unit, int& tile) {
+  if (cb <= 0) {
+    unit = (int)(logical % (unsigned)n);
+    tile = (int)(logical / (unsigned)n);
+    return;
+  }
+  const unsigned per = (unsigned)ntiles * (unsigned)cb;
+  const unsigned blk = logical / per;
+  const unsigned r = logical - blk * per;
+  unit = (int)(blk * (unsigned)cb + r % (unsigned)cb);
+  tile = (int)(r / (unsigned)cb);
+}
+
 }  // namespace wl

@@ -168,8 +168,8 @@ template <int P, int NC, int RECON, bool DAG, bool DIAG, bool G5, int EPI>
 __global__ __launch_bounds__(NT) void k_dslash_site(DslashArgs a) {
   using T = typename PT<P>::acc;
   const unsigned logical = xcd_tile(blockIdx.x, gridDim.x, a.xcd_remap);
-  const int col = a.cols_outer ? (int)(logical / a.ntiles) : (int)(logical % a.ncol);
-  const int tile = a.cols_outer ? (int)(logical % a.ntiles) : (int)(logical / a.ncol);
+  int col, tile;
+  split_units(logical, a.ntiles, a.ncol, a.cols_outer, col, tile);
   const ColState* st = a.red.cols;
   if (a.gate && gate_skip_col(a.gate, a.red, col)) return;
 
@@ -223,8 +223,8 @@ __global__ __launch_bounds__(NT) void k_dslash_mrhs(DslashArgs a) {
   using T = typename PT<P>::acc;
   const int ngrp = a.ncol / K;
   const unsigned logical = xcd_tile(blockIdx.x, gridDim.x, a.xcd_remap);
-  const int grp = a.cols_outer ? (int)(logical / a.ntiles) : (int)(logical % ngrp);
-  const int tile = a.cols_outer ? (int)(logical % a.ntiles) : (int)(logical / ngrp);
+  int grp, tile;
+  split_units(logical, a.ntiles, ngrp, a.cols_outer, grp, tile);
   const int col0 = grp * K;
   const ColState* st = a.red.cols;
   bool on[K];
@@ -312,8 +312,8 @@ template <int P, int NC, int RECON, bool DAG, bool DIAG, bool G5, int EPI>
 __global__ __launch_bounds__(NT) void k_dslash(DslashArgs a) {
   using T = typename PT<P>::acc;
   const unsigned logical = xcd_tile(blockIdx.x, gridDim.x, a.xcd_remap);
-  const int col = a.cols_outer ? (int)(logical / a.ntiles) : (int)(logical % a.ncol);
-  const int tile = a.cols_outer ? (int)(logical % a.ntiles) : (int)(logical / a.ncol);
+  int col, tile;
+  split_units(logical, a.ntiles, a.ncol, a.cols_outer, col, tile);
   const ColState* st = a.red.cols;
   if (a.gate && gate_skip_col(a.gate, a.red, col)) return;
 
@@ -417,6 +417,412 @@ __global__ __launch_bounds__(NT) void k_dslash(DslashArgs a) {
     part += site_epilogue<P, NC, EPI>(a, col, V, s, valid, res);
 #pragma unroll
     for (int q = 0; q < 4 * NC; ++q) cur[q] = nxt[q];
+  }
+  if (EPI != EPI_STORE) write_partial(a.red, col, tile, a.ntiles, part);
+}
+
+// Multi-RHS march (FP16 storage): K right-hand sides of one configuration per
+// thread, one (x1, x2, x3) column per thread marching over M consecutive x0
+// sites.  Links are loaded once per site and applied to the K spinors, and the
+// x0 neighbours come from registers: psi(x+0), loaded for the forward hop,
+// becomes the next site's centre, and U_0(x)^dag P psi(x), formed with the link
+// of the forward hop, is carried as the next site's backward hop.  So a
+// site's input is fetched once for direction 0 and its centre, instead of
+// three times through a cache that a multi-RHS working set overflows.
+// Directions 1..3 use the half-precision hop of the one-site kernels.
+// Workgroups are ordered column groups fastest, then x1/x2/x3 chunks, then x0
+// segments, so that the chunks holding a chunk's x1 and x2 neighbours run at
+// the same time (on the same XCD with the XCD-contiguous order).
+template <int NC, int RECON, bool DAG, bool DIAG, bool G5, int EPI, int K>
+__global__ __launch_bounds__(NT) void k_dslash_mrhs_march(DslashArgs a) {
+  using T = float;
+  const int ngrp = a.ncol / K;
+  const unsigned logical = xcd_tile(blockIdx.x, gridDim.x, a.xcd_remap);
+  int grp, tile;
+  split_units(logical, a.ntiles, ngrp, a.cols_outer, grp, tile);
+  const int col0 = grp * K;
+  const ColState* st = a.red.cols;
+  bool on[K];
+  bool any = false;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    on[k] = !(a.gate && gate_skip_col(a.gate, a.red, col0 + k));
+    any |= on[k];
+  }
+  if (!any) return;
+
+  const size_t V = (size_t)a.g.Vp;          // plane stride
+  const size_t S0 = (size_t)a.g.stride[0];  // sites per x0 slice
+  const int nchunk = a.ntiles / a.nseg;
+  const int chunk = tile % nchunk, seg = tile / nchunk;
+  const size_t c = (size_t)chunk * NT + threadIdx.x;
+  const bool valid = c < S0;
+  const size_t cc = valid ? c : 0;
+  const int L0 = a.g.L[0];
+  const int M = a.march;
+  const int x0s = seg * M;
+
+  const int cfg = col0 / a.cols_per_cfg;
+  const char* Ucfg = reinterpret_cast<const char*>(a.U) + (size_t)cfg * a.cfg_bytes;
+  const char* inb[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) inb[k] = reinterpret_cast<const char*>(a.in) + (size_t)(col0 + k) * a.col_bytes;
+
+  int x[4];
+  {
+    size_t rem = cc;
+    x[3] = rem % a.g.L[3];
+    rem /= a.g.L[3];
+    x[2] = rem % a.g.L[2];
+    x[1] = rem / a.g.L[2];
+  }
+  long long off_f[4], off_b[4];
+  bool top[4], bot[4];
+#pragma unroll
+  for (int mu = 1; mu < 4; ++mu) {
+    const int L = a.g.L[mu];
+    const long long str = a.g.stride[mu];
+    top[mu] = x[mu] == L - 1;
+    bot[mu] = x[mu] == 0;
+    off_f[mu] = top[mu] ? -(long long)(L - 1) * str : str;
+    off_b[mu] = bot[mu] ? (long long)(L - 1) * str : -str;
+  }
+
+  constexpr int SF = DAG ? +1 : -1;
+  const cx<T> ph0 = {T(a.bc_re[0]), T(a.bc_im[0])};
+  const bool real0 = ph0.im == T(0);
+  const char* U0 = Ucfg;  // direction 0
+
+  HalfSpinor<NC> cur[K];
+  cx<T> carry[K][2][NC];
+  float carry_f[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    carry_f[k] = 0.0f;
+    if (on[k]) load_half_spinor<NC>(inb[k], V, (size_t)x0s * S0 + cc, cur[k]);
+  }
+  {
+    // backward x0 hop of the first site, from x0s - 1
+    const int xm = (x0s == 0) ? L0 - 1 : x0s - 1;
+    const size_t sm = (size_t)xm * S0 + cc;
+    const bool bnd = xm == L0 - 1;
+    hf2 u[NC][NC];
+    load_half_link<NC, RECON>(U0, V, sm, u);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (!on[k]) continue;
+      HalfSpinor<NC> r;
+      load_half_spinor<NC>(inb[k], V, sm, r);
+      hf2 hA[2][NC], hB[2][NC];
+      project_forms<0, -SF, true>(r, hA, hB);
+      color_mul_half<true>(u, hA, hB, carry[k]);
+      if (!real0 && bnd) half_phase(carry[k], cconj(ph0));
+      carry_f[k] = -0.5f * r.s * ((real0 && bnd) ? ph0.re : 1.0f);
+    }
+  }
+
+  double part[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) part[k] = 0.0;
+  for (int j = 0; j < M; ++j) {
+    const int x0 = x0s + j;
+    const size_t s = (size_t)x0 * S0 + cc;
+    const bool top0 = x0 == L0 - 1;
+    // the accumulator starts from the diagonal term m0 psi(x), so that the
+    // centre spinor is dead once the x0 hops have used it
+    cx<T> acc[K][4 * NC];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (DIAG && on[k]) {
+        const T m0 = a.m0_from_state ? T(st[col0 + k].m0) : T(a.m0);
+        const float ms = m0 * cur[k].s;
+#pragma unroll
+        for (int q = 0; q < 4 * NC; ++q) {
+          const float2 p = h2f(cur[k].w[q]);
+          acc[k][q] = {p.x * ms, p.y * ms};
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4 * NC; ++q) acc[k][q] = {T(0), T(0)};
+      }
+      if (on[k]) spin_reconstruct_fma<0, -SF>(acc[k], carry[k], carry_f[k]);
+    }
+    // direction 0: forward hop, and the backward hop carried to the next site
+    {
+      HalfSpinor<NC> nxt[K];
+      const size_t sn = top0 ? cc : s + S0;
+      const float fb = (real0 && top0) ? ph0.re : 1.0f;
+      hf2 u[NC][NC];
+      load_half_link<NC, RECON>(U0, V, s, u);
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        if (!on[k]) continue;
+        load_half_spinor<NC>(inb[k], V, sn, nxt[k]);
+        hf2 hA[2][NC], hB[2][NC];
+        cx<T> o[2][NC];
+        project_forms<0, SF, false>(nxt[k], hA, hB);
+        color_mul_half<false>(u, hA, hB, o);
+        if (!real0 && top0) half_phase(o, ph0);
+        spin_reconstruct_fma<0, SF>(acc[k], o, -0.5f * nxt[k].s * fb);
+        project_forms<0, -SF, true>(cur[k], hA, hB);
+        color_mul_half<true>(u, hA, hB, carry[k]);
+        if (!real0 && top0) half_phase(carry[k], cconj(ph0));
+        carry_f[k] = -0.5f * cur[k].s * fb;
+        cur[k] = nxt[k];
+      }
+    }
+#define WL_MMHOP(MU)                                                                                          \
+    {                                                                                                         \
+      const cx<T> ph = {T(a.bc_re[MU]), T(a.bc_im[MU])};                                                      \
+      const char* Umu = Ucfg + MU * a.dir_bytes;                                                              \
+      const size_t sf = (size_t)((long long)s + off_f[MU]), sb = (size_t)((long long)s + off_b[MU]);          \
+      {                                                                                                       \
+        hf2 u[NC][NC];                                                                                        \
+        load_half_link<NC, RECON>(Umu, V, s, u);                                                              \
+        _Pragma("unroll") for (int k = 0; k < K; ++k) {                                                       \
+          if (!on[k]) continue;                                                                               \
+          hop_one<F16, NC, MU, SF, false, true>(u, inb[k], V, sf, top[MU], ph, acc[k]);                       \
+        }                                                                                                     \
+      }                                                                                                       \
+      {                                                                                                       \
+        hf2 u[NC][NC];                                                                                        \
+        load_half_link<NC, RECON>(Umu, V, sb, u);                                                             \
+        _Pragma("unroll") for (int k = 0; k < K; ++k) {                                                       \
+          if (!on[k]) continue;                                                                               \
+          hop_one<F16, NC, MU, -SF, true, true>(u, inb[k], V, sb, bot[MU], ph, acc[k]);                       \
+        }                                                                                                     \
+      }                                                                                                       \
+    }
+    WL_MMHOP(1)
+    WL_MMHOP(2)
+    WL_MMHOP(3)
+#undef WL_MMHOP
+
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (!on[k]) continue;
+      cx<T> res[4 * NC];
+#pragma unroll
+      for (int q = 0; q < 4 * NC; ++q) res[q] = acc[k][q];
+      if (G5) apply_g5<T, NC>(res);
+      part[k] += site_epilogue<F16, NC, EPI>(a, col0 + k, V, s, valid, res);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    if (EPI != EPI_STORE && on[k]) write_partial(a.red, col0 + k, tile, a.ntiles, part[k]);
+}
+
+// LDS-staged plane kernel (FP16 storage, one column per thread).  A workgroup
+// owns a tile of prow full x3 rows of the (x2, x3) plane at one x0 and marches
+// over `march` consecutive x1 values.  Each step's plane tile, with one halo
+// row on each side in x2 (periodic), sits in LDS (double-buffered: the next
+// plane is staged while the current one is read), so the x2 and x3 neighbours
+// of every site are LDS reads; the x1 neighbours come from registers (psi(x+1)
+// loaded for the forward hop is the next step's centre, and U_1(x)^dag P psi(x)
+// is carried as the next step's backward hop); only the x0 neighbours and the
+// links are read from memory.  Workgroups are ordered x0 fastest, so the tiles
+// whose planes are a tile's x0 neighbours run beside it.  One barrier per step.
+#ifndef WL_PLANE_WAVES
+#define WL_PLANE_WAVES 3
+#endif
+template <int NC, int RECON, bool DAG, bool DIAG, bool G5, int EPI>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WL_PLANE_WAVES))) void k_dslash_plane(
+    DslashArgs a) {
+  using T = float;
+  extern __shared__ __align__(16) unsigned char plane_lds[];
+  const unsigned logical = xcd_tile(blockIdx.x, gridDim.x, a.xcd_remap);
+  int col, tile;
+  split_units(logical, a.ntiles, a.ncol, a.cols_outer, col, tile);
+  if (a.gate && gate_skip_col(a.gate, a.red, col)) return;
+  const ColState* st = a.red.cols;
+
+  const int L0 = a.g.L[0], L1 = a.g.L[1], L2 = a.g.L[2], L3 = a.g.L[3];
+  const int prow = a.prow;
+  const int ntr = L2 / prow;
+  const int x0 = tile % L0;
+  const int rest = tile / L0;
+  const int x2t = rest % ntr;
+  const int seg = rest / ntr;
+  const int M = a.march;
+  const int x1s = seg * M;
+  const int nthr = prow * L3;  // threads owning a site; the rest only stage halo sites
+  const bool own = (int)threadIdx.x < nthr;
+  const int tid = own ? (int)threadIdx.x : 0;
+  const int x3 = tid % L3;
+  const int rr = tid / L3;
+  const int x2 = x2t * prow + rr;
+  const size_t V = (size_t)a.g.Vp;
+  const size_t S0 = (size_t)a.g.stride[0], S1 = (size_t)a.g.stride[1];
+  const size_t col_base = (size_t)x0 * S0 + (size_t)x2 * L3 + x3;  // site index without the x1 term
+  const int R = prow + 2;
+  const int RL = R * L3;
+  uint4* lw = reinterpret_cast<uint4*>(plane_lds);         // [2][NC][RL]
+  float* ls = reinterpret_cast<float*>(lw + 2 * NC * RL);  // [2][RL]
+  const int li = (rr + 1) * L3 + x3;                        // this site's LDS slot
+
+  const int cfg = col / a.cols_per_cfg;
+  const char* Ucfg = reinterpret_cast<const char*>(a.U) + (size_t)cfg * a.cfg_bytes;
+  const char* inb = reinterpret_cast<const char*>(a.in) + (size_t)col * a.col_bytes;
+
+  // halo slots: threads 0 .. 2 L3 - 1 stage x2 rows x2t*prow - 1 and x2t*prow + prow
+  const int hx = (int)threadIdx.x;
+  const bool halo = hx < 2 * L3;
+  const int hrow = halo ? hx / L3 : 0;
+  const int hx3 = hx - hrow * L3;
+  const int hx2 = hrow == 0 ? (x2t * prow + L2 - 1) % L2 : (x2t * prow + prow) % L2;
+  const int hli = (hrow == 0 ? 0 : R - 1) * L3 + hx3;
+  const size_t hcol_base = (size_t)x0 * S0 + (size_t)hx2 * L3 + hx3;
+
+  auto put = [&](int buf, int slot, const HalfSpinor<NC>& h) {
+#pragma unroll
+    for (int j = 0; j < NC; ++j)
+      lw[(buf * NC + j) * RL + slot] = make_uint4(h.w[4 * j], h.w[4 * j + 1], h.w[4 * j + 2], h.w[4 * j + 3]);
+    ls[buf * RL + slot] = h.s;
+  };
+  auto get = [&](int buf, int slot, HalfSpinor<NC>& h) {
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+      const uint4 t = lw[(buf * NC + j) * RL + slot];
+      h.w[4 * j] = t.x;
+      h.w[4 * j + 1] = t.y;
+      h.w[4 * j + 2] = t.z;
+      h.w[4 * j + 3] = t.w;
+    }
+    h.s = ls[buf * RL + slot];
+  };
+  auto stage_halo = [&](int buf, int x1p) {
+    if (halo) {
+      HalfSpinor<NC> h;
+      load_half_spinor<NC>(inb, V, hcol_base + (size_t)x1p * S1, h);
+      put(buf, hli, h);
+    }
+  };
+
+  // boundary flags and neighbour offsets of directions 0, 2, 3 (independent of x1)
+  const bool top0 = x0 == L0 - 1, bot0 = x0 == 0;
+  const long long off0f = top0 ? -(long long)(L0 - 1) * (long long)S0 : (long long)S0;
+  const long long off0b = bot0 ? (long long)(L0 - 1) * (long long)S0 : -(long long)S0;
+  const bool top2 = x2 == L2 - 1, bot2 = x2 == 0;
+  const bool top3 = x3 == L3 - 1, bot3 = x3 == 0;
+  const long long off2b = bot2 ? (long long)(L2 - 1) * L3 : -(long long)L3;
+  const long long off3b = bot3 ? (long long)(L3 - 1) : -1LL;
+  // LDS slots of the in-plane neighbours
+  const int li2f = (rr + 2) * L3 + x3, li2b = rr * L3 + x3;
+  const int li3f = (rr + 1) * L3 + (top3 ? 0 : x3 + 1), li3b = (rr + 1) * L3 + (bot3 ? L3 - 1 : x3 - 1);
+
+  constexpr int SF = DAG ? +1 : -1;
+  const cx<T> ph0 = {T(a.bc_re[0]), T(a.bc_im[0])}, ph1 = {T(a.bc_re[1]), T(a.bc_im[1])};
+  const cx<T> ph2 = {T(a.bc_re[2]), T(a.bc_im[2])}, ph3 = {T(a.bc_re[3]), T(a.bc_im[3])};
+  const bool real1 = ph1.im == T(0);
+  const char* U1 = Ucfg + a.dir_bytes;
+
+  {
+    HalfSpinor<NC> first;
+    load_half_spinor<NC>(inb, V, col_base + (size_t)x1s * S1, first);
+    if (own) put(0, li, first);
+    stage_halo(0, x1s);
+  }
+  cx<T> carry[2][NC];
+  float carry_f;
+  {
+    const int xm = (x1s == 0) ? L1 - 1 : x1s - 1;
+    const size_t sm = col_base + (size_t)xm * S1;
+    const bool bnd = xm == L1 - 1;
+    hf2 u[NC][NC];
+    load_half_link<NC, RECON>(U1, V, sm, u);
+    HalfSpinor<NC> r;
+    load_half_spinor<NC>(inb, V, sm, r);
+    hf2 hA[2][NC], hB[2][NC];
+    project_forms<1, -SF, true>(r, hA, hB);
+    color_mul_half<true>(u, hA, hB, carry);
+    if (!real1 && bnd) half_phase(carry, cconj(ph1));
+    carry_f = -0.5f * r.s * ((real1 && bnd) ? ph1.re : 1.0f);
+  }
+
+  double part = 0.0;
+  for (int j = 0; j < M; ++j) {
+    const int x1 = x1s + j;
+    const size_t s = col_base + (size_t)x1 * S1;
+    const bool top1 = x1 == L1 - 1;
+    const int cb = j & 1, nb = cb ^ 1;
+    __syncthreads();  // plane x1 staged in buffer cb; buffer nb free
+    HalfSpinor<NC> cur;  // the centre, from this site's own slot
+    get(cb, li, cur);
+    cx<T> acc[4 * NC];
+    if (DIAG) {
+      const T m0 = a.m0_from_state ? T(st[col].m0) : T(a.m0);
+      const float ms = m0 * cur.s;
+#pragma unroll
+      for (int q = 0; q < 4 * NC; ++q) {
+        const float2 p = h2f(cur.w[q]);
+        acc[q] = {p.x * ms, p.y * ms};
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4 * NC; ++q) acc[q] = {T(0), T(0)};
+    }
+    spin_reconstruct_fma<1, -SF>(acc, carry, carry_f);
+    // direction 1: forward hop; stage the next plane; carry the backward hop
+    HalfSpinor<NC> nxt;
+    {
+      const int x1n = top1 ? 0 : x1 + 1;
+      load_half_spinor<NC>(inb, V, col_base + (size_t)x1n * S1, nxt);
+      if (j + 1 < M) {
+        if (own) put(nb, li, nxt);
+        stage_halo(nb, x1n);
+      }
+      const float fb = (real1 && top1) ? ph1.re : 1.0f;
+      hf2 u[NC][NC];
+      load_half_link<NC, RECON>(U1, V, s, u);
+      hf2 hA[2][NC], hB[2][NC];
+      cx<T> o[2][NC];
+      project_forms<1, SF, false>(nxt, hA, hB);
+      color_mul_half<false>(u, hA, hB, o);
+      if (!real1 && top1) half_phase(o, ph1);
+      spin_reconstruct_fma<1, SF>(acc, o, -0.5f * nxt.s * fb);
+      project_forms<1, -SF, true>(cur, hA, hB);
+      color_mul_half<true>(u, hA, hB, carry);
+      if (!real1 && top1) half_phase(carry, cconj(ph1));
+      carry_f = -0.5f * cur.s * fb;
+    }
+    // direction 0: neighbours from memory
+    {
+      const char* U0 = Ucfg;
+      hf2 u[NC][NC];
+      load_half_link<NC, RECON>(U0, V, s, u);
+      hop_one<F16, NC, 0, SF, false, true>(u, inb, V, (size_t)((long long)s + off0f), top0, ph0, acc);
+      load_half_link<NC, RECON>(U0, V, (size_t)((long long)s + off0b), u);
+      hop_one<F16, NC, 0, -SF, true, true>(u, inb, V, (size_t)((long long)s + off0b), bot0, ph0, acc);
+    }
+    // directions 2 and 3: neighbours from the staged plane
+    {
+      const char* U2 = Ucfg + 2 * a.dir_bytes;
+      hf2 u[NC][NC];
+      HalfSpinor<NC> r;
+      load_half_link<NC, RECON>(U2, V, s, u);
+      get(cb, li2f, r);
+      hop_half<NC, 2, SF, false, true>(u, r, top2, ph2, acc);
+      load_half_link<NC, RECON>(U2, V, (size_t)((long long)s + off2b), u);
+      get(cb, li2b, r);
+      hop_half<NC, 2, -SF, true, true>(u, r, bot2, ph2, acc);
+    }
+    {
+      const char* U3 = Ucfg + 3 * a.dir_bytes;
+      hf2 u[NC][NC];
+      HalfSpinor<NC> r;
+      load_half_link<NC, RECON>(U3, V, s, u);
+      get(cb, li3f, r);
+      hop_half<NC, 3, SF, false, true>(u, r, top3, ph3, acc);
+      load_half_link<NC, RECON>(U3, V, (size_t)((long long)s + off3b), u);
+      get(cb, li3b, r);
+      hop_half<NC, 3, -SF, true, true>(u, r, bot3, ph3, acc);
+    }
+    cx<T> res[4 * NC];
+#pragma unroll
+    for (int q = 0; q < 4 * NC; ++q) res[q] = acc[q];
+    if (G5) apply_g5<T, NC>(res);
+    part += site_epilogue<F16, NC, EPI>(a, col, V, s, own, res);
   }
   if (EPI != EPI_STORE) write_partial(a.red, col, tile, a.ntiles, part);
 }

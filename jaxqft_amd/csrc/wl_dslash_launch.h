@@ -8,6 +8,19 @@ namespace wl {
 
 template <int P, int NC, int RECON, bool DAG, bool DIAG, bool G5, int EPI>
 static void launch_k(const DslashArgs& a, dim3 grid, hipStream_t stream) {
+  if constexpr (P == F16) {
+    if (a.plane) {
+      hipLaunchKernelGGL((k_dslash_plane<NC, RECON, DAG, DIAG, G5, EPI>), grid, dim3(NT), a.lds_bytes, stream, a);
+      return;
+    }
+  }
+  if constexpr (P == F16 && NC == 3) {
+    if (a.mrhs == 2 && a.march > 1) {
+      hipLaunchKernelGGL((k_dslash_mrhs_march<NC, RECON, DAG, DIAG, G5, EPI, 2>), dim3(a.ntiles * (a.ncol / 2)),
+                         dim3(NT), a.lds_bytes, stream, a);
+      return;
+    }
+  }
   if (NC == 3 && a.mrhs == 4)
     hipLaunchKernelGGL((k_dslash_mrhs<P, NC, RECON, DAG, DIAG, G5, EPI, 4>), dim3(a.ntiles * (a.ncol / 4)), dim3(NT),
                        a.lds_bytes, stream, a);

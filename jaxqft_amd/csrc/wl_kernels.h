@@ -100,7 +100,9 @@ struct DslashArgs {
   int box;                // 1: workgroups cover compact 4D boxes (bsh, bn below); overrides slab
   int bsh[4];             // log2 of the box extent per direction
   int bn[4];              // boxes per direction (L_mu >> bsh_mu)
-  int cols_outer;         // workgroup index = column (group) * ntiles + tile instead of tile * ncol + column
+  int cols_outer;         // column order: 0 columns (groups) fastest; n >= 1 blocks of n sweeping the lattice in turn
+  int plane;              // 1: LDS-staged plane kernel (F16, one column per thread, marching x1; march = x1 sites)
+  int prow;               // plane kernel: x2 rows per workgroup tile (prow * L3 <= NT, prow divides L2)
   int lds_bytes;          // dynamic LDS reserved per workgroup (caps resident workgroups; unused by the code)
   double bc_re[4], bc_im[4];
   double m0;              // diagonal coefficient (used when m0_from_state == 0)

@@ -293,6 +293,21 @@ __device__ __forceinline__ void load_link(const void* __restrict__ base, size_t 
   else LinkIO<P, NC, RECON>::load(base, V, site, u);
 }
 
+// The F16 hop on a neighbour spinor already in registers (from memory or LDS).
+template <int NC, int MU, int S, bool ADJ, bool FOLD>
+__device__ __forceinline__ void hop_half(const hf2 (&u)[NC][NC], const HalfSpinor<NC>& r, bool at_bnd, cx<float> ph,
+                                         cx<float> (&acc)[4 * NC], float fk = -0.5f) {
+  cx<float> o[2][NC];
+  const bool realbc = ph.im == 0.0f;
+  float f = (FOLD ? fk : 1.0f) * r.s;
+  if (realbc && at_bnd) f *= ph.re;
+  hf2 hA[2][NC], hB[2][NC];
+  project_forms<MU, S, ADJ>(r, hA, hB);
+  color_mul_half<ADJ>(u, hA, hB, o);
+  if (!realbc && at_bnd) half_phase(o, ADJ ? cconj(ph) : ph);  // complex boundary phase, in FP32
+  spin_reconstruct_fma<MU, S>(acc, o, f);
+}
+
 template <int P, int NC, int MU, int S, bool ADJ, bool FOLD>
 __device__ __forceinline__ void hop_one(const typename LinkRegs<P, NC>::type& u, const void* __restrict__ in,
                                         size_t V, size_t s_nb, bool at_bnd, cx<typename PT<P>::acc> ph,
@@ -303,14 +318,7 @@ __device__ __forceinline__ void hop_one(const typename LinkRegs<P, NC>::type& u,
   if constexpr (P == F16) {
     HalfSpinor<NC> r;
     load_half_spinor<NC>(in, V, s_nb, r);
-    const bool realbc = ph.im == T(0);
-    float f = (FOLD ? fk : 1.0f) * r.s;
-    if (realbc && at_bnd) f *= ph.re;
-    hf2 hA[2][NC], hB[2][NC];
-    project_forms<MU, S, ADJ>(r, hA, hB);
-    color_mul_half<ADJ>(u, hA, hB, o);
-    if (!realbc && at_bnd) half_phase(o, phe);  // complex boundary phase, in FP32
-    spin_reconstruct_fma<MU, S>(acc, o, f);
+    hop_half<NC, MU, S, ADJ, FOLD>(u, r, at_bnd, ph, acc, fk);
     return;
   } else {
     cx<T> p[4 * NC];

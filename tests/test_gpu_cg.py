@@ -187,3 +187,49 @@ def test_small_delta_still_converges(delta):
     assert st["iters"] < 1500, st
     assert true_resid(L, 3, 0.02, (1, 1, 1, -1), U, b, x) <= 1e-10
 
+
+
+@pytest.mark.parametrize("march", [2, 4])
+def test_multi_rhs_cg_with_marching_f16_stencil(march):
+    """The fused CG epilogues of the FP16 two-RHS marching kernel (norm on D,
+    the CG step on D^dag) drive every column of a batched mixed solve to the
+    FP64 true residual."""
+    print(f"[gpu] multi-rhs cg march={march}", flush=True)
+    from jaxqft_amd import _native as nat
+    from jaxqft_amd import fields
+    from jaxqft_amd.su3_wilson_nf2 import SU3WilsonNf2
+
+    L = (8, 4, 6, 4)
+    rng = fields.rng_for(1700)
+    U = fields.su3_hot_start(rng, 1, L, 0.5)
+    k = 4
+    rhs = fields.random_fermion(rng, (1, k, *L, 4, 3))
+    th = SU3WilsonNf2(lattice_shape=L, mass=0.05, fermion_bc=(1, 1, 1, -1), dtype=np.complex128, cg_tol=1e-10,
+                      cg_maxiter=4000, solver_precision="f16", solver_recon=12, reliable_delta=0.01)
+    th._context().set_tuning(nat.WL_F16, True, march, 2, True, 0)
+    X = th.solve_normal_multi(U, rhs)
+    assert all(s["status"] == "converged" for s in th.last_solve_stats), th.last_solve_stats
+    for j in range(k):
+        assert true_resid(L, 3, 0.05, (1, 1, 1, -1), U, rhs[:, j], X[:, j]) <= 1e-10
+
+
+@pytest.mark.parametrize("plane,k", [(2, 1), (4, 3)])
+def test_cg_with_plane_kernel(plane, k):
+    """The fused CG epilogues of the LDS-staged plane kernel drive a mixed
+    FP16 solve (one or several right-hand sides) to the FP64 true residual."""
+    print(f"[gpu] cg plane={plane} k={k}", flush=True)
+    from jaxqft_amd import _native as nat
+    from jaxqft_amd import fields
+    from jaxqft_amd.su3_wilson_nf2 import SU3WilsonNf2
+
+    L = (4, 8, 8, 16)
+    rng = fields.rng_for(1800)
+    U = fields.su3_hot_start(rng, 1, L, 0.5)
+    rhs = fields.random_fermion(rng, (1, k, *L, 4, 3))
+    th = SU3WilsonNf2(lattice_shape=L, mass=0.05, fermion_bc=(1, 1, 1, -1), dtype=np.complex128, cg_tol=1e-10,
+                      cg_maxiter=4000, solver_precision="f16", solver_recon=12, reliable_delta=0.01)
+    th._context().set_tuning(nat.WL_F16, k > 1, 1, 1 if k > 1 else 0, True, 0, 0, 0, 0, plane)
+    X = th.solve_normal_multi(U, rhs)
+    assert all(s["status"] == "converged" for s in th.last_solve_stats), th.last_solve_stats
+    for j in range(k):
+        assert true_resid(L, 3, 0.05, (1, 1, 1, -1), U, rhs[:, j], X[:, j]) <= 1e-10

@@ -295,12 +295,14 @@ def test_low_precision_complex_bc(nc, recon, prec, tol, nrhs, per_thread):
 
 
 @pytest.mark.parametrize("prec,recon,tol", [("f64", 18, 1e-13), ("f32", 12, 2e-6), ("f16", 12, 3e-3)])
-@pytest.mark.parametrize("per_thread,slab,cols_outer", [(1, 0, False), (2, 0, False), (4, 0, False), (2, 2, False),
-                                                      (4, 4, False), (1, 2, True), (2, 0, True)])
+@pytest.mark.parametrize("per_thread,slab,cols_outer", [(1, 0, 0), (2, 0, 0), (4, 0, 0), (2, 2, 0), (4, 4, 0),
+                                                      (1, 2, 1), (2, 0, 1), (1, 4, 2), (1, 0, 4), (2, 0, 3),
+                                                      (2, 4, 2)])
 def test_multi_rhs_stencil(prec, recon, tol, per_thread, slab, cols_outer):
     """k right-hand sides sharing each link load (1, 2 or 4 per thread; flat or
-    x1-slab site order; column-fastest or column-slowest workgroup order), two
-    gauge configurations, against the oracle column by column."""
+    x1-slab site order; the columns of a tile back to back, or blocks of 1, 2,
+    3 (lowered to 2) or 4 columns sweeping the lattice in turn), two gauge
+    configurations, against the oracle column by column."""
     print(f"[gpu] multi-rhs stencil {prec} k/thread={per_thread} slab={slab} cols_outer={cols_outer}", flush=True)
     import torch
 
@@ -333,3 +335,98 @@ def test_multi_rhs_stencil(prec, recon, tol, per_thread, slab, cols_outer):
             cfg = c // k
             ref = fn(U[cfg:cfg + 1], psi[c:c + 1])
             assert rel(got[c:c + 1], ref) < tol, (op, c)
+
+
+@pytest.mark.parametrize("recon", [12, 18])
+@pytest.mark.parametrize("L,march,cols_outer", [((8, 4, 6, 4), 2, False), ((8, 4, 6, 4), 8, True),
+                                                ((4, 6, 10, 6), 4, False), ((6, 6, 10, 6), 3, False)])
+def test_multi_rhs_march_stencil_f16(recon, L, march, cols_outer):
+    """The FP16 two-RHS marching kernel (x0 neighbours from registers, the
+    backward x0 hop carried between sites) on lattices whose x1*x2*x3 columns
+    fill a partial workgroup, with a complex x0 boundary phase, every operator,
+    against the oracle column by column."""
+    print(f"[gpu] multi-rhs march f16 r{recon} L={L} march={march} cols_outer={cols_outer}", flush=True)
+    import torch
+
+    from jaxqft_amd import _native as nat
+    from jaxqft_amd import fields
+    from oracle.wilson_oracle import WilsonOracle
+
+    bc = (np.exp(0.3j), -1, 1, -1)
+    k, B = 4, 2
+    rng = fields.rng_for(1200)
+    U = fields.su3_hot_start(rng, B, L, 0.5)
+    psi = fields.random_fermion(rng, (B * k, *L, 4, 3))
+    ctx = nat.Context(L, 3, B, 0.04, 1.0, bc)
+    pc = nat.WL_F16
+    ctx.set_tuning(pc, True, march, 2, True, 0, cols_outer)
+    t = ctx.get_tuning(pc, k)
+    assert (t["march"], t["rhs_per_thread"], t["cols_outer"]) == (march, 2, cols_outer)
+    g = nat.Gauge(ctx, torch.from_numpy(U).cuda(), nat.WL_LAYOUT_BMXYIJ)
+    fi, fo = nat.Field(ctx, pc, B * k), nat.Field(ctx, pc, B * k)
+    fi.import_(torch.from_numpy(psi).cuda())
+    out = torch.empty((B * k, *L, 4, 3), dtype=torch.complex128, device="cuda")
+    ora = WilsonOracle(L, 3, 0.04, fermion_bc=bc)
+    for op, fn in ((nat.WL_OP_D, ora.apply_D), (nat.WL_OP_DDAG, ora.apply_Ddag), (nat.WL_OP_G5D, ora.apply_g5D),
+                   (nat.WL_OP_K, ora.apply_offdiag), (nat.WL_OP_KDAG, ora.apply_offdiag_dagger)):
+        nat.check(ctx._lib.wl_dslash(ctx.h, g.h, op, recon, fi.h, fo.h, k, nat.c_void_p(nat.current_stream())))
+        fo.export(out)
+        got = out.cpu().numpy()
+        for c in range(B * k):
+            cfg = c // k
+            ref = fn(U[cfg:cfg + 1], psi[c:c + 1])
+            assert rel(got[c:c + 1], ref) < 3e-3, (op, c, rel(got[c:c + 1], ref))
+
+
+@pytest.mark.parametrize("L,plane,nrhs,cols_outer", [((4, 8, 6, 16), 2, 1, 0), ((6, 4, 8, 32), 4, 1, 0),
+                                                     ((4, 6, 10, 12), 1, 1, 0), ((4, 6, 10, 12), 6, 3, 0),
+                                                     ((8, 4, 4, 32), 2, 4, 2)])
+def test_plane_kernel_f16(L, plane, nrhs, cols_outer):
+    """The LDS-staged plane kernel (x2/x3 neighbours from a double-buffered LDS
+    tile with periodic halo rows, x1 neighbours carried in registers): tiles
+    with idle threads, tiles covering all of x2, complex boundary phases in
+    every direction, one or several right-hand sides, every operator, against
+    the oracle."""
+    print(f"[gpu] plane f16 L={L} plane={plane} nrhs={nrhs} cols_outer={cols_outer}", flush=True)
+    import torch
+
+    from jaxqft_amd import _native as nat
+    from jaxqft_amd import fields
+    from oracle.wilson_oracle import WilsonOracle
+
+    bc = (np.exp(0.4j), np.exp(-0.7j), -1, np.exp(1.1j))
+    B = 2
+    rng = fields.rng_for(1300)
+    U = fields.su3_hot_start(rng, B, L, 0.5)
+    psi = fields.random_fermion(rng, (B * nrhs, *L, 4, 3))
+    ctx = nat.Context(L, 3, B, 0.04, 1.0, bc)
+    pc = nat.WL_F16
+    ctx.set_tuning(pc, nrhs > 1, 1, 1 if nrhs > 1 else 0, True, 0, cols_outer, 0, 0, plane)
+    assert ctx.get_tuning(pc, nrhs)["plane"] == plane
+    g = nat.Gauge(ctx, torch.from_numpy(U).cuda(), nat.WL_LAYOUT_BMXYIJ)
+    fi, fo = nat.Field(ctx, pc, B * nrhs), nat.Field(ctx, pc, B * nrhs)
+    fi.import_(torch.from_numpy(psi).cuda())
+    out = torch.empty((B * nrhs, *L, 4, 3), dtype=torch.complex128, device="cuda")
+    ora = WilsonOracle(L, 3, 0.04, fermion_bc=bc)
+    for recon in (12, 18):
+        for op, fn in ((nat.WL_OP_D, ora.apply_D), (nat.WL_OP_DDAG, ora.apply_Ddag), (nat.WL_OP_G5D, ora.apply_g5D),
+                       (nat.WL_OP_K, ora.apply_offdiag), (nat.WL_OP_KDAG, ora.apply_offdiag_dagger)):
+            nat.check(ctx._lib.wl_dslash(ctx.h, g.h, op, recon, fi.h, fo.h, nrhs, nat.c_void_p(nat.current_stream())))
+            fo.export(out)
+            got = out.cpu().numpy()
+            for c in range(B * nrhs):
+                cfg = c // nrhs
+                ref = fn(U[cfg:cfg + 1], psi[c:c + 1])
+                assert rel(got[c:c + 1], ref) < 3e-3, (recon, op, c, rel(got[c:c + 1], ref))
+
+
+def test_plane_kernel_rejected_shapes_fall_back():
+    """A plane march that does not divide L1 is rejected; a lattice whose x3
+    rows do not fit a workgroup keeps the other kernels."""
+    import torch
+
+    from jaxqft_amd import _native as nat
+
+    ctx = nat.Context((4, 6, 4, 8), 3, 1, 0.1, 1.0, (1, 1, 1, -1))
+    with pytest.raises(nat.NativeError, match="plane"):
+        ctx.set_tuning(nat.WL_F16, False, 1, 0, True, 0, 0, 0, 0, 4)
