@@ -90,12 +90,23 @@ def tune(ctx: nat.Context, gauge: nat.Gauge, prec: int, recon: int, fin: nat.Fie
             for m, k, o in shapes:
                 install(m, k, r, o, co)
                 times[(m, k, r, o, co)] = time_dslash(ctx, gauge, prec, recon, fin, fout, nrhs, reps, stream)
-    best = min(times, key=times.get)
+    def as_dict(m, k, r, o, co):
+        return {"march": m, "rhs_per_thread": k, "xcd_remap": r, "slab": o[1] if o[0] == "s" else 0,
+                "block_shape": o[1] if o[0] == "b" else 0, "plane": o[1] if o[0] == "p" else 0, "cols_outer": co}
+
+    ranked = sorted(times, key=times.get)
+    best = ranked[0]
     install(*best)
     m, k, r, o, co = best
-    return {"march": m, "rhs_per_thread": k, "xcd_remap": r, "slab": o[1] if o[0] == "s" else 0,
-            "block_shape": o[1] if o[0] == "b" else 0, "plane": o[1] if o[0] == "p" else 0, "cols_outer": co,
+    return {**as_dict(*best),
+            "ranked": [as_dict(*c) for c in ranked],
             "us": {f"m{m}k{k}x{r}{o[0]}{o[1]}" + (f"c{co}" if co else ""): t for (m, k, r, o, co), t in times.items()}}
+
+
+def install_tuning(ctx: nat.Context, prec: int, multi: bool, t: Dict[str, object]) -> None:
+    """Install a tuning dict as returned in `tune(...)["ranked"]`."""
+    ctx.set_tuning(prec, multi, int(t["march"]), int(t["rhs_per_thread"]) if multi else 0, bool(t["xcd_remap"]),
+                   int(t["slab"]), int(t["cols_outer"]), 0, int(t["block_shape"]), int(t["plane"]))
 
 
 # 4D workgroup box presets (log2 extents per direction), as in the library
