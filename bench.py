@@ -62,6 +62,7 @@ def parse():
     ap.add_argument("--dslash-reps", type=int, default=50)
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--no-config4", action="store_true", help="skip the 24^3x48 multi-RHS case")
+    ap.add_argument("--no-config2", action="store_true", help="skip the 16^4 FP64 single-RHS case")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving scenario")
     ap.add_argument("--serving-requests", type=int, default=60)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
@@ -214,8 +215,11 @@ def main():
                 "us": round(us, 2), "GB/s": round(gbs, 1), "frac_hbm_peak": round(gbs / PEAK_HBM_GBS, 4),
                 "GFLOP/s": round(FLOP_PER_SITE * V / (us * 1e-6) / 1e9, 1), "bytes_per_site": bps,
                 "march": t1["best"], "xcd_remap": t1["xcd_remap"], "slab": t1["slab"], "block_shape": t1["block_shape"],
-                "tune_us": {m: round(t, 2) for m, t in t1["us"].items()},
+                "plane": t1["plane"], "tune_us": {m: round(t, 2) for m, t in t1["us"].items()},
             }
+            if prec == "f64" and recon == 18:
+                # context only: the no-reuse convention (every neighbour spinor and link read, 360 reals/site)
+                dslash[f"{prec}_r{recon}"]["GB/s_no_reuse_2880B"] = round(2880 * V / (us * 1e-6) / 1e9, 1)
             log(f"dslash {prec} r{recon}: {us:.1f} us {gbs:.0f} GB/s march {t1['best']} slab {t1['slab']} "
                 f"box {t1['block_shape']}")
             del fi, fo
@@ -410,7 +414,13 @@ def main():
             "algorithmic_bytes": dom_bytes,
         },
         "solve": {
-            "fp64_single_rhs": {**summarize(st64), "s_per_solve": round(fp64_s, 4), "solves_per_s": round(1 / fp64_s, 3)},
+            "fp64_single_rhs": {**summarize(st64), "s_per_solve": round(fp64_s, 4), "solves_per_s": round(1 / fp64_s, 3),
+                                # whole solve / iterations (set-up and the final true residual included)
+                                "us_per_iteration": round(fp64_s / st64[0]["iters"] * 1e6, 1),
+                                "cg_iteration_GB/s": round(3840 * V / (fp64_s / st64[0]["iters"]) / 1e9, 1),
+                                "cg_iteration_frac_hbm_peak": round(3840 * V / (fp64_s / st64[0]["iters"]) / 1e9
+                                                                    / PEAK_HBM_GBS, 4),
+                                "cg_iteration_bytes_per_site": 3840},
             "mixed_single_rhs": {**summarize(stm1), "s_per_solve": round(mix1_s, 4),
                                  "solves_per_s": round(1 / mix1_s, 3), "rel_error_vs_fp64": err1},
             "mixed_multi_rhs": {**summarize(stmk), "nrhs": k, "s_per_batch": round(mixk_s, 4),
@@ -424,6 +434,8 @@ def main():
         "dslash_single_rhs": dslash,
         "dslash_multi_rhs": dslash_k,
     }
+    if not args.no_config2:
+        out["fp64_16x16x16x16"] = config2(B, args, not args.no_cpu)
     if not args.no_config4:
         out["multi_rhs_24x24x24x48"] = config4(B, args)
     if not args.no_serving:
@@ -434,6 +446,61 @@ def main():
         out["cpu_baseline"] = cpu_baseline(U, b_host[:1], L, args, st64[0]["iters"])
     out["bench_wall_s"] = round(time.perf_counter() - t_start, 1)
     print(json.dumps(out))
+
+
+def config2(B, args, with_cpu):
+    """16^4, eps=0.5, m=0.05, one complex Gaussian source: the plain FP64
+    single-RHS path.  The FP64 Dslash (tuned launch shape) in GB/s of
+    algorithmic bytes (960 B/site) and GFLOP/s (1320 flop/site), the FP64 CG
+    solve rate, and the C/OpenMP oracle solving the same system on the host
+    cores (the whole solve, iterated-residual rule at 1e-10).  The 16^4
+    working set (FP64 links 38 MB) fits the 256 MB Infinity Cache, so its
+    GB/s is cache-resident, not an HBM roofline figure."""
+    torch, nat = B.torch, B.nat
+    from jaxqft_amd import fields
+    from jaxqft_amd.autotune import time_dslash, tune_march
+
+    L = (16, 16, 16, 16)
+    V = int(np.prod(L))
+    bc = (1, 1, 1, -1)
+    rng = fields.rng_for(fields.SEED_BASE + 1)
+    U = fields.su3_hot_start(rng, 1, L, 0.5)
+    b = fields.random_fermion(rng, (1, *L, 4, 3))
+    with torch.cuda.stream(B.stream):
+        ctx = nat.Context(L, 3, 1, 0.05, 1.0, bc)
+        gauge = nat.Gauge(ctx, torch.from_numpy(U).cuda(), nat.WL_LAYOUT_BMXYIJ)
+        bt = torch.from_numpy(b).cuda()
+        fi, fo = nat.Field(ctx, nat.WL_F64, 1), nat.Field(ctx, nat.WL_F64, 1)
+        fi.import_(bt)
+        t1 = tune_march(ctx, gauge, nat.WL_F64, 18, fi, fo, reps=10, stream=B.stream)
+        us = time_dslash(ctx, gauge, nat.WL_F64, 18, fi, fo, 1, args.dslash_reps, B.stream)
+        gbs = 960 * V / (us * 1e-6) / 1e9
+        bf, xf = nat.Field(ctx, nat.WL_F64, 1), nat.Field(ctx, nat.WL_F64, 1)
+        bf.import_(bt)
+        B.solve(ctx, gauge, bf, xf, 1, "f64", 18)
+        t, st = B.timed_solves(3, ctx, gauge, bf, xf, 1, "f64", 18)
+    res = {
+        "lattice": list(L), "eps": 0.5, "mass": 0.05, "cache_resident": True,
+        "dslash_f64": {"us": round(us, 2), "GB/s": round(gbs, 1), "frac_hbm_peak": round(gbs / PEAK_HBM_GBS, 4),
+                       "GFLOP/s": round(FLOP_PER_SITE * V / (us * 1e-6) / 1e9, 1), "bytes_per_site": 960,
+                       "march": t1["best"], "slab": t1["slab"], "block_shape": t1["block_shape"]},
+        "fp64_solve": {**summarize(st), "s_per_solve": round(t, 5), "solves_per_s": round(1 / t, 2)},
+    }
+    log(f"16^4: dslash {us:.1f} us {gbs:.0f} GB/s, FP64 solve {t*1e3:.2f} ms {summarize(st)}")
+    if with_cpu:
+        from oracle import oracle_c
+
+        threads = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+        oracle_c.cg(U[0], b[0], L, 3, 0.05, bc, 1e-10, 5)  # warm-up
+        t0 = time.perf_counter()
+        _, it = oracle_c.cg(U[0], b[0], L, 3, 0.05, bc, 1e-10, 20000)
+        cs = time.perf_counter() - t0
+        res["cpu_baseline"] = {"s_per_solve": round(cs, 3), "solves_per_s": round(1 / cs, 4), "iters": it,
+                               "cores": threads, "kind": "port",
+                               "sample": "the whole FP64 CG solve with the C/OpenMP oracle (iterated residual 1e-10)"}
+        res["gpu_over_cpu"] = round(cs / t, 1)
+        log(f"16^4 CPU oracle solve {cs:.2f} s ({it} iterations, {threads} threads)")
+    return res
 
 
 def config4(B, args):
