@@ -26,14 +26,24 @@ __device__ __forceinline__ void store_as(void* base, size_t V, size_t site, cons
   SpinorIO<P, NC>::store(base, V, site, t);
 }
 
+// A workgroup loops over its (column, tile) units: one unit per workgroup
+// normally, several when a launch gated on pending reliable updates is capped
+// (blas_grid), so that a mostly idle launch dispatches few workgroups; a
+// capped launch first checks, with one round of loads, whether the gate
+// admits any column at all.
 #define WL_BLAS_PROLOGUE                                            \
-  const int col = blockIdx.x % a.ncol;                              \
-  const int tile = blockIdx.x / a.ncol;                             \
-  if (gate_skip_col(a.gate, a.red, col)) return;                    \
+  const unsigned total_ = (unsigned)a.ntiles * (unsigned)a.ncol;    \
+  if (gridDim.x < total_ && !gate_any_col(a.gate, a.red, a.ncol))   \
+    return;                                                         \
+  for (unsigned u_ = blockIdx.x; u_ < total_; u_ += gridDim.x) {    \
+  const int col = (int)(u_ % (unsigned)a.ncol);                     \
+  const int tile = (int)(u_ / (unsigned)a.ncol);                    \
+  if (gate_skip_col(a.gate, a.red, col)) continue;                  \
   const size_t V = (size_t)a.Vp;                                    \
   const size_t site = (size_t)tile * NT + threadIdx.x;              \
   const bool valid = site < (size_t)a.V;                            \
   const size_t s = valid ? site : 0;
+#define WL_BLAS_EPILOGUE }
 
 // |a|^2 -> action
 template <int P, int NC>
@@ -46,6 +56,7 @@ __global__ __launch_bounds__(NT) void k_norm2(BlasArgs a) {
 #pragma unroll
   for (int c = 0; c < 4 * NC; ++c) part += (double)v[c].re * v[c].re + (double)v[c].im * v[c].im;
   write_partial(a.red, col, tile, a.ntiles, valid ? part : 0.0);
+  WL_BLAS_EPILOGUE
 }
 
 // p = r + beta p.   a=p, b=r
@@ -63,6 +74,7 @@ __global__ __launch_bounds__(NT) void k_xpay(BlasArgs a) {
     p[c].im = r[c].im + beta * p[c].im;
   }
   if (valid) store_as<P, NC>((char*)a.a + col * a.col_bytes_a, V, s, p);
+  WL_BLAS_EPILOGUE
 }
 
 // x += y ; y = 0.   a=x (PX), b=y (PY)
@@ -83,6 +95,7 @@ __global__ __launch_bounds__(NT) void k_accumulate(BlasArgs a) {
     store_as<PX, NC>((char*)a.a + col * a.col_bytes_a, V, s, x);
     store_as<PY, NC>((char*)a.b + col * a.col_bytes_b, V, s, z);
   }
+  WL_BLAS_EPILOGUE
 }
 
 // out = in (precision conversion).   a=out (PO), b=in (PI)
@@ -92,6 +105,7 @@ __global__ __launch_bounds__(NT) void k_convert(BlasArgs a) {
   cx<double> v[4 * NC];
   load_as<PI, NC>((char*)a.b + col * a.col_bytes_b, V, s, v);
   if (valid) store_as<PO, NC>((char*)a.a + col * a.col_bytes_a, V, s, v);
+  WL_BLAS_EPILOGUE
 }
 
 __global__ __launch_bounds__(NT) void k_reduce_finish(Reduction red, int nblk, int gate) {
@@ -150,7 +164,11 @@ __global__ void k_finalize_update(ColState* cols, int ncol) {
             : KERNEL<__VA_ARGS__, 1><<<grid, block, 0, stream>>>(a)),               \
    hipGetLastError())
 
-static inline dim3 blas_grid(const BlasArgs& a) { return dim3((unsigned)(a.ntiles * a.ncol)); }
+static inline dim3 blas_grid(const BlasArgs& a) {
+  const unsigned units = (unsigned)(a.ntiles * a.ncol);
+  // gated on pending reliable updates: about the resident workgroups (8 per CU)
+  return dim3((a.gate == 2 && units > 2048u) ? 2048u : units);
+}
 
 static hipError_t norm2_main(int p, int nc, BlasArgs a, hipStream_t stream) {
   dim3 grid = blas_grid(a), block(NT);

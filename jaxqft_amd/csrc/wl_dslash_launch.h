@@ -8,6 +8,17 @@ namespace wl {
 
 template <int P, int NC, int RECON, bool DAG, bool DIAG, bool G5, int EPI>
 static void launch_k(const DslashArgs& a, dim3 grid, hipStream_t stream) {
+  // A launch gated on pending reliable updates (the FP64 true residual) is
+  // idle in most iterations, and an idle launch still dispatches every
+  // workgroup: with an even number of columns it takes the two-RHS kernel,
+  // which halves the workgroups at about the one-RHS kernel's active speed.
+  if constexpr (P == F64 && NC == 3) {
+    if (a.gate == 2 && a.mrhs == 1 && a.march == 1 && a.cols_per_cfg % 2 == 0) {
+      hipLaunchKernelGGL((k_dslash_mrhs<P, NC, RECON, DAG, DIAG, G5, EPI, 2>), dim3(a.ntiles * (a.ncol / 2)), dim3(NT),
+                         a.lds_bytes, stream, a);
+      return;
+    }
+  }
   if constexpr (P == F16) {
     if (a.plane) {
       hipLaunchKernelGGL((k_dslash_plane<NC, RECON, DAG, DIAG, G5, EPI>), grid, dim3(NT), a.lds_bytes, stream, a);

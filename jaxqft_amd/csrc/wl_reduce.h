@@ -31,6 +31,15 @@ __device__ __forceinline__ bool gate_skip_col(int gate, const Reduction& red, in
   return gate_skip(gate, red.cols[col]);
 }
 
+// Whether the gate admits any of the ncol columns: every lane reads the state
+// of columns lane, lane + 64, ... and the wavefront votes, so each wavefront
+// of a workgroup reaches the same answer with one round of loads.
+__device__ __forceinline__ bool gate_any_col(int gate, const Reduction& red, int ncol) {
+  bool any = false;
+  for (int c = (int)(threadIdx.x & 63u); c < ncol; c += 64) any |= !gate_skip_col(gate, red, c);
+  return __ballot(any) != 0ull;
+}
+
 __device__ __forceinline__ void col_finish(ColState& s, int status) {
   s.status = status;
   s.done = 1;
