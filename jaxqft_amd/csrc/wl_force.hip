@@ -128,11 +128,8 @@ __device__ __forceinline__ void force_link(const ForceArgs& a, int cfg, size_t s
 // path); the four wavefronts read the same 64 sites' X and Y (L1/L2 reuse).
 constexpr int kForceSites = NT / 4;
 
-#ifndef WL_FORCE_WAVES
-#define WL_FORCE_WAVES 1
-#endif
 template <int NC, class S>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WL_FORCE_WAVES))) void k_pf_force(ForceArgs a) {
+__global__ __launch_bounds__(NT) void k_pf_force(ForceArgs a) {
   const int mu = (int)(threadIdx.x / kForceSites);
   const size_t site = (size_t)blockIdx.x * kForceSites + (threadIdx.x % kForceSites);
   const int cfg = blockIdx.y;
