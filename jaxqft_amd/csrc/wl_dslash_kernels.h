@@ -623,12 +623,10 @@ __global__ __launch_bounds__(NT) void k_dslash_mrhs_march(DslashArgs a) {
 // is carried as the next step's backward hop); only the x0 neighbours and the
 // links are read from memory.  Workgroups are ordered x0 fastest, so the tiles
 // whose planes are a tile's x0 neighbours run beside it.  One barrier per step.
-#ifndef WL_PLANE_WAVES
-#define WL_PLANE_WAVES 3
-#endif
+// Three wavefronts per SIMD (168 VGPRs with a few spilled) measured faster
+// for 12 right-hand sides than the unconstrained 192 VGPRs at two.
 template <int NC, int RECON, bool DAG, bool DIAG, bool G5, int EPI>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WL_PLANE_WAVES))) void k_dslash_plane(
-    DslashArgs a) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_dslash_plane(DslashArgs a) {
   using T = float;
   extern __shared__ __align__(16) unsigned char plane_lds[];
   const unsigned logical = xcd_tile(blockIdx.x, gridDim.x, a.xcd_remap);
