@@ -176,7 +176,7 @@ def main():
 
     from jaxqft_amd import _native as nat
     from jaxqft_amd import fields
-    from jaxqft_amd.autotune import install_tuning, time_dslash, tune, tune_march
+    from jaxqft_amd.autotune import install_tuning, time_dslash, tune, tune_eo, tune_march
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
@@ -331,6 +331,11 @@ def main():
         with torch.cuda.stream(stream):
             xe1 = torch.empty_like(b1_t)
             xek = torch.empty_like(bk_t)
+        # x1 slab order of the inner-precision hop, timed on capped solves
+        eo_tune = tune_eo(ctx, gauge, bk_t, xek, k, nat.PREC[args.inner], args.recon, stream=stream)
+        eo["inner_hop_eo_slab"] = eo_tune["eo_slab"]
+        eo["inner_hop_tuning_ms"] = eo_tune["ms"]
+        log(f"even-odd inner hop: eo_slab {eo_tune['eo_slab']} {eo_tune['ms']}")
         for name, bt, xt, nr, prec, recon, n in (("fp64_single_rhs", b1_t, xe1, 1, "f64", 18, args.fp64_steps),
                                                 ("mixed_single_rhs", b1_t, xe1, 1, args.inner, args.recon,
                                                  args.fp64_steps),

@@ -181,10 +181,17 @@ typedef struct wl_tuning {
                          values over a tile of full x3 rows with its x2 and x3 neighbours
                          read from LDS (needs 2 L3 <= 256); takes precedence over march,
                          slab and block_shape */
+  int eo_slab;        /* even-odd hop (wl_eo_apply, wl_cg_eo_normal): 0 = flat order of the
+                         compact sites; s > 0 (must divide L1) = slabs of s consecutive x1
+                         planes walked x0-fastest */
 } wl_tuning;
 /* wl_ctx_get_tuning reports the values in effect for nrhs right-hand sides. */
 int wl_ctx_set_tuning(wl_ctx* ctx, int prec, int multi_rhs, const wl_tuning* t);
 int wl_ctx_get_tuning(wl_ctx* ctx, int prec, int nrhs, wl_tuning* t);
+/* Sets only the eo_slab field (no reference counterpart): the even-odd hop is
+ * tuned apart from the full-lattice stencil, whose settings stay as they are
+ * (automatic ones included). */
+int wl_ctx_set_eo_slab(wl_ctx* ctx, int prec, int multi_rhs, int eo_slab);
 
 /* where: 0 = device pointer, 1 = host pointer */
 int wl_gauge_upload(wl_ctx* ctx, const void* U, int layout, int dtype, int where, wl_gauge** out);

@@ -12,7 +12,7 @@ hardware order).
 
 from __future__ import annotations
 
-from ctypes import c_void_p
+from ctypes import byref, c_void_p
 from typing import Dict, Iterable, Optional
 
 import torch
@@ -80,9 +80,11 @@ def tune(ctx: nat.Context, gauge: nat.Gauge, prec: int, recon: int, fin: nat.Fie
             shapes += [(m, 2, ("s", 0)) for m in mrhs_marches if m > 1 and L0 % m == 0]
     corders = [int(c) for c in cols_outer] if multi else [0]
 
+    eo_slab = ctx.get_tuning(prec, nrhs)["eo_slab"]  # kept: the even-odd hop is tuned on its own
+
     def install(m, k, r, o, co):
         ctx.set_tuning(prec, multi, m, k if multi else 0, bool(r), o[1] if o[0] == "s" else 0, co, 0,
-                       o[1] if o[0] == "b" else 0, o[1] if o[0] == "p" else 0)
+                       o[1] if o[0] == "b" else 0, o[1] if o[0] == "p" else 0, eo_slab)
 
     times = {}
     for co in corders:
@@ -104,9 +106,64 @@ def tune(ctx: nat.Context, gauge: nat.Gauge, prec: int, recon: int, fin: nat.Fie
 
 
 def install_tuning(ctx: nat.Context, prec: int, multi: bool, t: Dict[str, object]) -> None:
-    """Install a tuning dict as returned in `tune(...)["ranked"]`."""
+    """Install a tuning dict as returned in `tune(...)["ranked"]` (the even-odd slab is kept)."""
+    eo = ctx.get_tuning(prec, 2 if multi else 1)["eo_slab"]
     ctx.set_tuning(prec, multi, int(t["march"]), int(t["rhs_per_thread"]) if multi else 0, bool(t["xcd_remap"]),
-                   int(t["slab"]), int(t["cols_outer"]), 0, int(t["block_shape"]), int(t["plane"]))
+                   int(t["slab"]), int(t["cols_outer"]), 0, int(t["block_shape"]), int(t["plane"]), eo)
+
+
+def set_eo_slab(ctx: nat.Context, prec: int, multi: bool, eo_slab: int) -> None:
+    """Set the x1 slab of the even-odd hop, keeping the other launch-shape settings."""
+    ctx.set_eo_slab(prec, multi, eo_slab)
+
+
+def tune_eo(ctx: nat.Context, gauge: nat.Gauge, b: torch.Tensor, x: torch.Tensor, nrhs: int, prec: int, recon: int,
+            slabs: Iterable[int] = (0, 2, 4, 8), iters: int = 24, delta: float = 0.01,
+            stream: Optional[torch.cuda.Stream] = None) -> Dict[str, object]:
+    """Pick the x1 slab order of the even-odd hop for inner precision prec at
+    nrhs right-hand sides per configuration and install it.  Each candidate
+    (flat, or slab extents dividing L1) is timed on a solve of the even-odd
+    normal equations (wl_cg_eo_normal) capped at `iters` iterations, with
+    b and x the caller's complex128 reference-layout source and solution
+    arrays; the fused hop, Schur combine and CG-step launches it times are
+    the ones a full solve repeats; the solves capture HIP graphs, so they run
+    on `stream` or, by default, on a side stream ordered after the current
+    one.  Returns {"eo_slab", "ms": {slab: ms}}."""
+    if stream is None:
+        stream = torch.cuda.Stream()
+        stream.wait_stream(torch.cuda.current_stream())
+    L1 = ctx.lattice_shape[1]
+    multi = nrhs > 1
+    h = c_void_p(stream.cuda_stream)
+    sp, keep = nat.solve_params(1e-30, iters, nat.WL_STOP_TRUE_RESIDUAL, prec, recon, delta, 16, True)
+    st = (nat.WlSolveStats * (b.numel() // (ctx.volume * 4 * ctx.nc)))()
+
+    def run():
+        nat.check(ctx._lib.wl_cg_eo_normal(ctx.h, gauge.h, c_void_p(b.data_ptr()), c_void_p(x.data_ptr()), c_void_p(0),
+                                           nrhs, nat.WL_C128, byref(sp), st, h))
+
+    times = {}
+    for s in slabs:
+        if s and L1 % s:
+            continue
+        set_eo_slab(ctx, prec, multi, s)
+        run()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        best = None
+        for _ in range(2):
+            e0.record(stream)
+            run()
+            e1.record(stream)
+            e1.synchronize()
+            t = e0.elapsed_time(e1)
+            best = t if best is None else min(best, t)
+        times[s] = best
+    del keep
+    torch.cuda.current_stream().wait_stream(stream)
+    pick = min(times, key=times.get)
+    set_eo_slab(ctx, prec, multi, pick)
+    return {"eo_slab": pick, "ms": {str(s): round(t, 3) for s, t in times.items()}}
 
 
 # 4D workgroup box presets (log2 extents per direction), as in the library

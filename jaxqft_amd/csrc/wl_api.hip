@@ -125,6 +125,7 @@ struct wl_ctx {
   int blocks_per_cu[2][3] = {{0, 0, 0}, {0, 0, 0}};  // resident workgroup cap (0 = none)
   int block_shape[2][3] = {{0, 0, 0}, {0, 0, 0}};    // 4D box preset of one-site kernels (0 = none)
   int plane[2][3] = {{0, 0, 0}, {0, 0, 0}};          // x1 march of the LDS plane kernel (0 = off)
+  int eo_slab[2][3] = {{0, 0, 0}, {0, 0, 0}};        // x1 slab of the even-odd hop (0 = flat)
   int lds_per_cu = 0;     // bytes of LDS per CU
   int lds_per_block = 0;  // largest LDS allocation of one workgroup
 };
@@ -315,6 +316,8 @@ int wl_ctx_set_tuning(wl_ctx* c, int prec, int multi_rhs, const wl_tuning* t) {
   if (t->cols_outer < 0 || t->cols_outer > 64) return fail(WL_ERR_BAD_ARG, "cols_outer must be 0 .. 64");
   if (t->plane < 0 || (t->plane > 0 && c->geom.L[1] % t->plane != 0))
     return fail(WL_ERR_BAD_ARG, "plane %d must be 0 (off) or divide L1=%d", t->plane, c->geom.L[1]);
+  if (t->eo_slab < 0 || (t->eo_slab > 0 && c->geom.L[1] % t->eo_slab != 0))
+    return fail(WL_ERR_BAD_ARG, "eo_slab %d must be 0 (flat) or divide L1=%d", t->eo_slab, c->geom.L[1]);
   if (t->slab < 0 || (t->slab > 0 && c->geom.L[1] % t->slab != 0))
     return fail(WL_ERR_BAD_ARG, "slab %d must be 0 (flat) or divide L1=%d", t->slab, c->geom.L[1]);
   std::lock_guard<std::mutex> lk(c->mu);
@@ -326,7 +329,18 @@ int wl_ctx_set_tuning(wl_ctx* c, int prec, int multi_rhs, const wl_tuning* t) {
   c->blocks_per_cu[m][prec] = t->blocks_per_cu;
   c->block_shape[m][prec] = t->block_shape;
   c->plane[m][prec] = t->plane;
+  c->eo_slab[m][prec] = t->eo_slab;
   if (multi_rhs) c->mrhs[prec] = t->rhs_per_thread;
+  return WL_OK;
+}
+
+int wl_ctx_set_eo_slab(wl_ctx* c, int prec, int multi_rhs, int eo_slab) {
+  CHK(check_ctx(c));
+  if (prec < F64 || prec > F16) return fail(WL_ERR_BAD_ARG, "bad precision %d", prec);
+  if (eo_slab < 0 || (eo_slab > 0 && c->geom.L[1] % eo_slab != 0))
+    return fail(WL_ERR_BAD_ARG, "eo_slab %d must be 0 (flat) or divide L1=%d", eo_slab, c->geom.L[1]);
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->eo_slab[multi_rhs ? 1 : 0][prec] = eo_slab;
   return WL_OK;
 }
 
@@ -344,6 +358,7 @@ int wl_ctx_get_tuning(wl_ctx* c, int prec, int nrhs, wl_tuning* t) {
   t->blocks_per_cu = c->blocks_per_cu[m][prec];
   t->block_shape = t->march == 1 ? c->block_shape[m][prec] : 0;
   t->plane = c->plane[m][prec];
+  t->eo_slab = c->eo_slab[m][prec];
   return WL_OK;
 }
 
@@ -543,9 +558,20 @@ static void fill_eo_args(const wl_ctx* c, DslashArgs& a, int prec, const void* U
   a.cols_per_cfg = nrhs;
   a.g.Vp = c->Vhp;
   a.Vh = c->Vh;
-  a.ntiles = (int)((c->Vh + NT - 1) / NT);
   a.march = 1;
   a.nseg = c->geom.L[0];
+  a.box = 0;
+  a.cols_outer = 0;
+  a.plane = 0;
+  a.slab = c->eo_slab[nrhs > 1 ? 1 : 0][prec];
+  if (a.slab > 0) {
+    const long long plane = (long long)c->geom.L[2] * c->geom.L[3] / 2;
+    a.slab_tiles = (int)((a.slab * plane + NT - 1) / NT);
+    a.ntiles = c->geom.L[0] * (c->geom.L[1] / a.slab) * a.slab_tiles;
+  } else {
+    a.slab_tiles = 0;
+    a.ntiles = (int)((c->Vh + NT - 1) / NT);
+  }
   // two right-hand sides per thread share each link load whenever the
   // right-hand sides per configuration pair up (SU(3))
   a.mrhs = (nrhs > 1 && c->geom.nc == 3 && nrhs % 2 == 0) ? 2 : 1;

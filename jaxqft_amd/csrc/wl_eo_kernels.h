@@ -46,8 +46,21 @@ __global__ __launch_bounds__(NT) void k_hop_eo(DslashArgs a) {
   if (!any) return;
 
   const size_t V = (size_t)a.g.Vp;  // plane stride of the compact fields
-  const size_t i0 = (size_t)tile * NT + threadIdx.x;
-  const bool valid = i0 < (size_t)a.Vh;
+  // compact target index: flat, or slabs of a.slab consecutive x1 planes walked
+  // x0-fastest (each (x0, x1) plane holds L2 L3 / 2 compact sites, contiguous)
+  size_t i0;
+  bool valid;
+  if (a.slab == 0) {
+    i0 = (size_t)tile * NT + threadIdx.x;
+    valid = i0 < (size_t)a.Vh;
+  } else {
+    const int sl = tile / a.slab_tiles;
+    const int sub = tile - sl * a.slab_tiles;
+    const unsigned plane = (unsigned)a.g.stride[1] >> 1;  // L2 * L3 / 2
+    const unsigned ii = (unsigned)sub * NT + threadIdx.x;
+    valid = ii < (unsigned)a.slab * plane;
+    i0 = ((size_t)(sl % a.g.L[0]) * a.g.L[1] + (size_t)(sl / a.g.L[0]) * a.slab) * plane + ii;
+  }
   const size_t i = valid ? i0 : 0;
 
   int x[4];

@@ -217,3 +217,111 @@ def test_eo_reconstructs_full_solution_16x4(prec):
     it_full = th.last_solve_stats[0]["iters"]
     print(f"[gpu] iterations eo {it_eo} full {it_full}", flush=True)
     assert it_eo < it_full
+
+
+def _set_eo_slab_all(ctx, eo_slab):
+    from jaxqft_amd import _native as nat
+    from jaxqft_amd.autotune import set_eo_slab
+
+    for pc in (nat.WL_F64, nat.WL_F32, nat.WL_F16):
+        for multi in (False, True):
+            set_eo_slab(ctx, pc, multi, eo_slab)
+            assert ctx.get_tuning(pc, 2 if multi else 1)["eo_slab"] == eo_slab
+
+
+@pytest.mark.parametrize("eo_slab", [1, 2, 3, 6])
+def test_eo_slab_order_operators(eo_slab):
+    """The even-odd hop walking slabs of x1 planes x0-fastest (slabs of 1, 2,
+    3 and all 6 planes; 16 compact sites a plane, so most threads of a
+    workgroup idle) gives the oracle's blocks and Schur operators."""
+    print(f"[gpu] eo slab operators eo_slab={eo_slab}", flush=True)
+    from jaxqft_amd import fields
+    from oracle.wilson_oracle import WilsonOracle
+
+    L = (4, 6, 4, 8)
+    bc = (np.exp(0.4j), 1, -1, -1)
+    rng = fields.rng_for(1560 + eo_slab)
+    U = fields.su3_hot_start(rng, 2, L, 0.6)
+    psi = fields.random_fermion(rng, (2, *L, 4, 3))
+    th = theory(L, 3, 0.07, bc, batch=2)
+    _set_eo_slab_all(th._context(), eo_slab)
+    ora = WilsonOracle(L, 3, 0.07, fermion_bc=bc)
+    for name in ("apply_K_oe", "apply_K_eo", "apply_Kdag_oe", "apply_Kdag_eo"):
+        assert rel(getattr(th, name)(U, psi), getattr(ora, name)(U, psi)) < TOL, name
+    for name in ("apply_eo_schur_even", "apply_eo_schur_even_dagger", "apply_eo_schur_odd"):
+        assert rel(getattr(th, name)(U, psi), getattr(ora, name)(U, psi)) < TOL, name
+
+
+@pytest.mark.parametrize("prec", ["f64", "f16"])
+def test_eo_slab_order_multi_rhs_cg(prec):
+    """Batched even-odd CG with the slab order of the hop: each column meets
+    the true-residual target of its Schur normal equations and equals the
+    flat-order solve to solver precision."""
+    print(f"[gpu] eo slab cg {prec}", flush=True)
+    from jaxqft_amd import fields
+    from oracle.wilson_oracle import WilsonOracle
+
+    L = (4, 8, 4, 8)
+    bc = (1, 1, 1, -1)
+    rng = fields.rng_for(1570)
+    U = fields.su3_hot_start(rng, 1, L, 0.5)
+    rhs = fields.random_fermion(rng, (1, 4, *L, 4, 3))
+    masses = [0.05, 0.1, 0.2, 0.3]
+    kw = dict(solver_precision=prec, solver_recon=12) if prec != "f64" else dict(solver_precision="f64")
+    th0 = theory(L, 3, 0.1, bc, cg_tol=1e-10, cg_maxiter=3000, **kw)
+    X0 = np.asarray(th0.solve_eop_even_normal_multi(U, rhs, masses=masses))
+    th = theory(L, 3, 0.1, bc, cg_tol=1e-10, cg_maxiter=3000, **kw)
+    _set_eo_slab_all(th._context(), 4)
+    X = np.asarray(th.solve_eop_even_normal_multi(U, rhs, masses=masses))
+    for j, m in enumerate(masses):
+        st = th.last_solve_stats[j]
+        assert st["status"] == "converged", st
+        ora = WilsonOracle(L, 3, m, fermion_bc=bc)
+        res = _schur_normal_resid(ora, U, rhs[:, j], X[:, j])
+        assert res <= 1e-10, (j, res, st)
+        assert rel(X[:, j], X0[:, j]) < 1e-8, j
+
+
+def test_eo_slab_must_divide_l1():
+    print("[gpu] eo slab rejected", flush=True)
+    from jaxqft_amd import _native as nat
+    from jaxqft_amd.autotune import set_eo_slab
+
+    th = theory((4, 6, 4, 8), 3, 0.1, (1, 1, 1, -1))
+    ctx = th._context()
+    with pytest.raises(nat.NativeError, match="eo_slab"):
+        set_eo_slab(ctx, nat.WL_F64, False, 4)
+    assert ctx.get_tuning(nat.WL_F64, 1)["eo_slab"] == 0
+
+
+def test_tune_eo_installs_a_valid_slab():
+    """The even-odd tuner times each slab that divides L1 on capped solves,
+    installs the fastest, leaves the full-lattice launch shape alone, and a
+    solve after it still meets the true-residual target."""
+    print("[gpu] tune_eo", flush=True)
+    import torch
+    from jaxqft_amd import _native as nat, fields
+    from jaxqft_amd.autotune import tune_eo
+    from oracle.wilson_oracle import WilsonOracle
+
+    L = (4, 8, 4, 8)
+    bc = (1, 1, 1, -1)
+    rng = fields.rng_for(1580)
+    U = fields.su3_hot_start(rng, 1, L, 0.5)
+    b = fields.random_fermion(rng, (1, 4, *L, 4, 3))
+    th = theory(L, 3, 0.1, bc, cg_tol=1e-10, cg_maxiter=3000, solver_precision="f16", solver_recon=12)
+    ctx = th._context()
+    before = ctx.get_tuning(nat.WL_F16, 4)
+    g = nat.Gauge(ctx, torch.from_numpy(U).cuda(), nat.WL_LAYOUT_BMXYIJ)
+    bt = torch.from_numpy(b.reshape(4, *L, 4, 3)).cuda()
+    xt = torch.empty_like(bt)
+    t = tune_eo(ctx, g, bt, xt, 4, nat.WL_F16, 12, slabs=(0, 1, 2, 3, 4, 8), iters=6)
+    assert set(t["ms"]) == {"0", "1", "2", "4", "8"}
+    assert t["eo_slab"] in (0, 1, 2, 4, 8)
+    after = ctx.get_tuning(nat.WL_F16, 4)
+    assert after["eo_slab"] == t["eo_slab"]
+    assert {k: v for k, v in after.items() if k != "eo_slab"} == {k: v for k, v in before.items() if k != "eo_slab"}
+    X = np.asarray(th.solve_eop_even_normal_multi(U, b, masses=[0.1] * 4))
+    ora = WilsonOracle(L, 3, 0.1, fermion_bc=bc)
+    for j in range(4):
+        assert _schur_normal_resid(ora, U, b[:, j], X[:, j]) <= 1e-10, j
