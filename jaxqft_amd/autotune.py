@@ -134,9 +134,14 @@ def tune_eo(ctx: nat.Context, gauge: nat.Gauge, b: torch.Tensor, x: torch.Tensor
         stream.wait_stream(torch.cuda.current_stream())
     L1 = ctx.lattice_shape[1]
     multi = nrhs > 1
+    ncol = ctx.batch * nrhs
+    for name, t in (("b", b), ("x", x)):
+        if (t.dtype != torch.complex128 or not t.is_cuda or not t.is_contiguous()
+                or t.numel() != ncol * ctx.volume * 4 * ctx.nc):
+            raise ValueError(f"{name} must be a contiguous complex128 CUDA tensor of {ncol} spinor fields")
     h = c_void_p(stream.cuda_stream)
     sp, keep = nat.solve_params(1e-30, iters, nat.WL_STOP_TRUE_RESIDUAL, prec, recon, delta, 16, True)
-    st = (nat.WlSolveStats * (b.numel() // (ctx.volume * 4 * ctx.nc)))()
+    st = (nat.WlSolveStats * ncol)()
 
     def run():
         nat.check(ctx._lib.wl_cg_eo_normal(ctx.h, gauge.h, c_void_p(b.data_ptr()), c_void_p(x.data_ptr()), c_void_p(0),

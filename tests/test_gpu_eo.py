@@ -315,6 +315,8 @@ def test_tune_eo_installs_a_valid_slab():
     g = nat.Gauge(ctx, torch.from_numpy(U).cuda(), nat.WL_LAYOUT_BMXYIJ)
     bt = torch.from_numpy(b.reshape(4, *L, 4, 3)).cuda()
     xt = torch.empty_like(bt)
+    with pytest.raises(ValueError):
+        tune_eo(ctx, g, bt[:2], xt, 4, nat.WL_F16, 12)
     t = tune_eo(ctx, g, bt, xt, 4, nat.WL_F16, 12, slabs=(0, 1, 2, 3, 4, 8), iters=6)
     assert set(t["ms"]) == {"0", "1", "2", "4", "8"}
     assert t["eo_slab"] in (0, 1, 2, 4, 8)
