@@ -385,6 +385,10 @@ def main():
         dom_traffic, dom_traffic_src = measured_traffic(args, k, L, dom_tuning)
         del fi, fo, bk, xk, b1, x1, x1m
         torch.cuda.synchronize()
+        copy = device_copy_rate(stream)
+        log(f"device copy: {copy['GB/s']} GB/s ({copy['frac_hbm_peak']} of peak)")
+        for entry in list(dslash.values()) + list(dslash_k.values()):
+            entry["frac_of_device_copy"] = round(entry["GB/s"] / copy["GB/s"], 4)
 
     out = {
         "metric": METRIC,
@@ -417,7 +421,9 @@ def main():
             "traffic_source": dom_traffic_src,
             "kernel_us": round(dom_us, 2),
             "algorithmic_bytes": dom_bytes,
+            "frac_of_device_copy": round(dom_gbs / copy["GB/s"], 4),
         },
+        "device_copy": copy,
         "solve": {
             "fp64_single_rhs": {**summarize(st64), "s_per_solve": round(fp64_s, 4), "solves_per_s": round(1 / fp64_s, 3),
                                 # whole solve / iterations (set-up and the final true residual included)
@@ -662,6 +668,31 @@ def lime_decode(B, args, L, with_cpu):
     del U, pay, Uc, out, host, dev
     torch.cuda.synchronize()
     return res
+
+
+def device_copy_rate(stream, mib: int = 1024, reps: int = 10):
+    """Plain device-to-device copy of `mib` MiB on `stream` (HIP events):
+    read + write bytes per second, the practical HBM rate the stencils are
+    compared with."""
+    import torch
+
+    with torch.cuda.stream(stream):
+        src = torch.empty(mib << 17, dtype=torch.float64, device="cuda")
+        src.fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            dst.copy_(src)
+        e1.record(stream)
+        e1.synchronize()
+    us = e0.elapsed_time(e1) * 1e3 / reps
+    nbytes = 2 * src.numel() * 8
+    gbs = nbytes / (us * 1e-6) / 1e9
+    return {"GB/s": round(gbs, 1), "frac_hbm_peak": round(gbs / PEAK_HBM_GBS, 4), "us": round(us, 2),
+            "bytes_read_plus_written": nbytes}
 
 
 def cpu_baseline(U, b, L, args, gpu_iters):
