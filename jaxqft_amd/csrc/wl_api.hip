@@ -23,6 +23,7 @@ hipError_t launch_xpay(int p, int nc, BlasArgs a, hipStream_t stream);
 hipError_t launch_accumulate(int px, int py, int nc, BlasArgs a, hipStream_t stream);
 hipError_t launch_convert(int po, int pi, int nc, BlasArgs a, hipStream_t stream);
 hipError_t launch_finalize_update(ColState* cols, int ncol, hipStream_t stream);
+hipError_t launch_trueres_from_b2(ColState* cols, int ncol, hipStream_t stream);
 hipError_t launch_admit_finish(ColState* cols, int col, double tol2, hipStream_t stream);
 hipError_t pack_spinor(int prec, int nc, const void* src, int dtype, void* dst, long long V, long long Vp, int ncol,
                        size_t col_bytes, hipStream_t st);
@@ -835,6 +836,32 @@ struct Solver {
     return WL_OK;
   }
 
+  // The first update of a solve from x = 0: r_hi = b - A 0 = b exactly, so
+  // the FP64 operator applications are skipped: r = b (in the inner
+  // precision), |r_hi|^2 = |b|^2 decides convergence, p = r.
+  int initial_update_zero_guess() {
+    hipError_t e;
+    {
+      BlasArgs a = bargs(2);
+      a.a = r; a.col_bytes_a = mixed ? cbP : cb64;
+      a.b = b; a.col_bytes_b = cb64;
+      e = launch_convert(mixed ? P : F64, F64, nc, a, st);
+      if (e != hipSuccess) return fail(WL_ERR_HIP, "convert: %s", hipGetErrorString(e));
+    }
+    e = launch_trueres_from_b2(red.cols, ncol, st);
+    if (e != hipSuccess) return fail(WL_ERR_HIP, "true residual: %s", hipGetErrorString(e));
+    {
+      BlasArgs a = bargs(2);
+      a.a = p; a.col_bytes_a = cbP;
+      a.b = r; a.col_bytes_b = cbP;
+      e = launch_xpay(P, nc, a, st);
+      if (e != hipSuccess) return fail(WL_ERR_HIP, "xpay: %s", hipGetErrorString(e));
+    }
+    e = launch_finalize_update(red.cols, ncol, st);
+    if (e != hipSuccess) return fail(WL_ERR_HIP, "finalize: %s", hipGetErrorString(e));
+    return WL_OK;
+  }
+
   // One CG iteration on A = D^dag D, two stencil launches with fused vector
   // work: t = D p with |t|^2 = <p, A p> (alpha), then A p = D^dag t consumed
   // in the epilogue by y += alpha p, r -= alpha A p and |r|^2 (beta); then
@@ -993,7 +1020,7 @@ static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int n
     hc[i].stop2 = tol2 * hc[i].b2;
   }
   HIPCHK(hipMemcpyAsync(c->cols.p, hc, sizeof(ColState) * ncol, hipMemcpyHostToDevice, st));
-  CHK(s.reliable_update());
+  CHK(use_x0 ? s.reliable_update() : s.initial_update_zero_guess());
 
   // Iteration blocks of `poll`, 4 and 1 iterations (as HIP graphs when
   // requested); after each block the host reads the per-column state and

@@ -140,6 +140,17 @@ hipError_t launch_admit_finish(ColState* cols, int col, double tol2, hipStream_t
   return hipGetLastError();
 }
 
+// First update of a solve from a zero initial guess: the true residual is b
+// itself, so its norm is |b|^2 and the convergence decision of a true-residual
+// reduction is taken on that (a zero right-hand side converges here).
+__global__ void k_trueres_from_b2(ColState* cols, int ncol) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncol) return;
+  ColState& s = cols[c];
+  if (!(s.need_update && !s.done)) return;
+  apply_action(RA_TRUERES, s, s.b2);
+}
+
 // After a reliable update: restart the iterated residual from the true one.
 __global__ void k_finalize_update(ColState* cols, int ncol) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -207,6 +218,11 @@ hipError_t launch_convert(int po, int pi, int nc, BlasArgs a, hipStream_t stream
   if (po == F64 && pi == F16) return WL_DISPATCH_NC(k_convert, F64, F16);
   if (po == F32 && pi == F32) return WL_DISPATCH_NC(k_convert, F32, F32);
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_trueres_from_b2(ColState* cols, int ncol, hipStream_t stream) {
+  hipLaunchKernelGGL(k_trueres_from_b2, dim3((ncol + 63) / 64), dim3(64), 0, stream, cols, ncol);
+  return hipGetLastError();
 }
 
 hipError_t launch_finalize_update(ColState* cols, int ncol, hipStream_t stream) {

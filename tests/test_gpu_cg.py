@@ -116,6 +116,39 @@ def test_status_maxiter_and_nan():
     assert th2.last_solve_stats[0]["status"] == "converged" and np.all(x0 == 0)
 
 
+@pytest.mark.parametrize("prec", ["f64", "f16"])
+def test_zero_guess_start_matches_explicit_zero_guess(prec):
+    """A solve without a guess starts from r = b with |r|^2 = |b|^2 and no
+    operator application; the same solve given an explicit x0 = 0 (which
+    evaluates b - A x0 in FP64) takes the same number of iterations (within
+    one: |b|^2 from the norm kernel and from the residual kernel may differ
+    in the last bit) and reaches the same solution to solver precision, and
+    a zero column among them still converges at once."""
+    print(f"[gpu] zero-guess start {prec}", flush=True)
+    from jaxqft_amd import fields
+    from jaxqft_amd.su3_wilson_nf2 import SU3WilsonNf2
+
+    L = (4, 4, 4, 8)
+    bc = (1, 1, 1, -1)
+    rng = fields.rng_for(640)
+    U = fields.su3_hot_start(rng, 2, L, 0.5)
+    b = fields.random_fermion(rng, (2, *L, 4, 3))
+    b[1] = 0.0
+    kw = dict(solver_precision=prec, solver_recon=12) if prec != "f64" else dict(solver_precision="f64")
+    th = SU3WilsonNf2(lattice_shape=L, mass=0.08, fermion_bc=bc, batch_size=2, dtype=np.complex128, cg_tol=1e-10,
+                      cg_maxiter=3000, **kw)
+    x = np.asarray(th.solve_normal(U, b))
+    st = th.last_solve_stats
+    xg = np.asarray(th.solve_normal_with_guess(U, b, np.zeros_like(b)))
+    stg = th.last_solve_stats
+    assert st[1]["status"] == "converged" and st[1]["iters"] == 0 and np.all(x[1] == 0)
+    assert stg[1]["status"] == "converged" and stg[1]["iters"] == 0
+    assert st[0]["status"] == stg[0]["status"] == "converged"
+    assert abs(st[0]["iters"] - stg[0]["iters"]) <= 1
+    assert true_resid(L, 3, 0.08, bc, U[:1], b[:1], x[:1]) <= 1e-10
+    assert np.linalg.norm(x[0] - xg[0]) / np.linalg.norm(xg[0]) < 1e-8
+
+
 @pytest.mark.parametrize("prec", ["f64", "f32", "f16"])
 def test_multi_rhs_same_alone_and_in_the_mix(prec):
     """Per-column masses and convergence masking: every column of a batched
