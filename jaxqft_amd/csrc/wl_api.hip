@@ -1472,6 +1472,9 @@ int wl_batch_admit(wl_batch* bt, int slot, const void* b, int dtype, double mass
   e = launch_norm2(F64, nc, a, st);
   if (e == hipSuccess) e = launch_admit_finish((ColState*)bt->cols.p, slot, bt->tol2, st);
   if (e != hipSuccess) return fail(WL_ERR_HIP, "admit: %s", hipGetErrorString(e));
+  // x = 0: the column's first true residual is b (only this slot awaits an
+  // update between iteration blocks, and the update kernels are gated on it)
+  CHK(s.initial_update_zero_guess());
   bt->busy[slot] = 1;
   return WL_OK;
 }
