@@ -800,7 +800,8 @@ struct Solver {
     return WL_OK;
   }
 
-  // gated reliable update: x += y; r_hi = b - A x; r = r_hi; p = r + beta p
+  // gated reliable update: x += y; r = b - A x (FP64 arithmetic, stored in the
+  // inner precision); p = r + beta p
   int reliable_update() {
     hipError_t e;
     if (mixed) {
@@ -816,13 +817,13 @@ struct Solver {
     d.col_bytes = cb64;
     CHK(apply(d, F64, 18, false, x, thi, mid64, EPI_STORE, RA_NONE));
     d.w = b;
-    CHK(apply(d, F64, 18, true, thi, rhi, mid64, EPI_RESID, RA_TRUERES));
     if (mixed) {
-      BlasArgs a = bargs(2);
-      a.a = r; a.col_bytes_a = cbP;
-      a.b = rhi; a.col_bytes_b = cb64;
-      e = launch_convert(P, F64, nc, a, st);
-      if (e != hipSuccess) return fail(WL_ERR_HIP, "convert: %s", hipGetErrorString(e));
+      // r = b - A x straight into the inner precision (no FP64 copy of it)
+      d.res = r;
+      d.col_bytes_y = cbP;
+      CHK(apply(d, F64, 18, true, thi, nullptr, mid64, P == F32 ? EPI_RESID32 : EPI_RESID16, RA_TRUERES));
+    } else {
+      CHK(apply(d, F64, 18, true, thi, rhi, mid64, EPI_RESID, RA_TRUERES));
     }
     {
       BlasArgs a = bargs(2);
@@ -950,11 +951,11 @@ static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int n
   s.b = b;
   s.x = x;
   // workspace
-  CHK(c->ws[0].ensure(s.cb64 * ncol));  // r_hi
+  if (!s.mixed) CHK(c->ws[0].ensure(s.cb64 * ncol));  // r_hi (a mixed solve stores r straight in the inner precision)
   CHK(c->ws[1].ensure(s.cb64 * ncol));  // t_hi
   CHK(c->ws[2].ensure(s.cbP * ncol));   // p
   CHK(c->ws[4].ensure(s.cbP * ncol));   // t
-  s.rhi = c->ws[0].p;
+  s.rhi = s.mixed ? nullptr : c->ws[0].p;
   s.thi = c->ws[1].p;
   s.p = c->ws[2].p;
   s.t = c->ws[4].p;
@@ -1313,8 +1314,9 @@ struct wl_batch {
 
 static size_t batch_bytes_per_col(const wl_ctx* c, int prec) {
   const bool mixed = prec != F64;
-  size_t b = 4 * col_bytes_of(c, F64) + 2 * col_bytes_of(c, prec);  // b x r_hi t_hi; p t
+  size_t b = 3 * col_bytes_of(c, F64) + 2 * col_bytes_of(c, prec);  // b x t_hi; p t
   if (mixed) b += col_bytes_of(c, prec) + col_bytes_of(c, F64);  // r and an accumulator of at most FP64
+  else b += col_bytes_of(c, F64);                                 // r_hi (the FP64 iterated residual)
   return b + sizeof(double) * max_tiles(c) + sizeof(ColState);
 }
 
@@ -1378,7 +1380,7 @@ int wl_batch_create(wl_ctx* c, wl_gauge* g, int capacity, const wl_solve_params*
   };
   s.b = alloc(0, s.cb64 * capacity);
   s.x = alloc(1, s.cb64 * capacity);
-  s.rhi = alloc(2, s.cb64 * capacity);
+  s.rhi = s.mixed ? nullptr : alloc(2, s.cb64 * capacity);
   s.thi = alloc(3, s.cb64 * capacity);
   s.p = alloc(4, s.cbP * capacity);
   s.t = alloc(6, s.cbP * capacity);

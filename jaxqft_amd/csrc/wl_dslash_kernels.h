@@ -115,15 +115,15 @@ template <int P, int NC, int EPI>
 __device__ __forceinline__ double site_epilogue(const DslashArgs& a, int col, size_t V, size_t s, bool valid,
                                                 cx<typename PT<P>::acc> (&res)[4 * NC]) {
   using T = typename PT<P>::acc;
-  char* outb = reinterpret_cast<char*>(a.out) + (size_t)col * a.col_bytes;
+  auto outb = [&]() { return reinterpret_cast<char*>(a.out) + (size_t)col * a.col_bytes; };
   double part = 0.0;
   if (EPI == EPI_STORE || EPI == EPI_NORM) {
     if (EPI == EPI_NORM) {
 #pragma unroll
       for (int q = 0; q < 4 * NC; ++q) part += (double)res[q].re * (double)res[q].re + (double)res[q].im * (double)res[q].im;
     }
-    if (valid) store_nt<P, NC>(outb, V, s, res);
-  } else if (EPI == EPI_RESID) {
+    if (valid) store_nt<P, NC>(outb(), V, s, res);
+  } else if (EPI == EPI_RESID || EPI == EPI_RESID32 || EPI == EPI_RESID16) {
     const char* wb = reinterpret_cast<const char*>(a.w) + (size_t)col * a.col_bytes;
     cx<T> w[4 * NC];
     SpinorIO<P, NC>::load(wb, V, s, w);
@@ -132,7 +132,16 @@ __device__ __forceinline__ double site_epilogue(const DslashArgs& a, int col, si
       res[q] = csub(w[q], res[q]);
       part += (double)res[q].re * (double)res[q].re + (double)res[q].im * (double)res[q].im;
     }
-    if (valid) store_nt<P, NC>(outb, V, s, res);
+    if (EPI == EPI_RESID) {
+      if (valid) store_nt<P, NC>(outb(), V, s, res);
+    } else {
+      // the inner solver's residual, rounded as a separate conversion would
+      constexpr int PR = EPI == EPI_RESID32 ? F32 : F16;
+      cx<float> rr[4 * NC];
+#pragma unroll
+      for (int q = 0; q < 4 * NC; ++q) rr[q] = {(float)res[q].re, (float)res[q].im};
+      if (valid) SpinorIO<PR, NC>::store(reinterpret_cast<char*>(a.res) + (size_t)col * a.col_bytes_y, V, s, rr);
+    }
   } else {
     constexpr int PY = EpiAccum<EPI>::prec;
     using TY = typename PT<PY>::acc;

@@ -45,7 +45,7 @@ static void launch_k(const DslashArgs& a, dim3 grid, hipStream_t stream) {
 }
 
 // op: 0 D, 1 D^dag, 2 gamma5 D, 3 K, 4 K^dag.  The reducing epilogues exist for
-// the operator the solver uses them with: EPI_NORM on D, EPI_RESID (FP64 only)
+// the operator the solver uses them with: EPI_NORM on D, EPI_RESID[32|16] (FP64 only)
 // and the CG steps on D^dag (an FP64 solve accumulates into FP64).
 template <int P, int NC, int RECON>
 static hipError_t launch_p(int op, int epi, const DslashArgs& a, hipStream_t stream) {
@@ -68,6 +68,8 @@ static hipError_t launch_p(int op, int epi, const DslashArgs& a, hipStream_t str
   if (op != 1) return hipErrorInvalidValue;
   if constexpr (P == F64) {
     if (epi == EPI_RESID) launch_k<P, NC, RECON, true, true, false, EPI_RESID>(a, grid, stream);
+    else if (epi == EPI_RESID32) launch_k<P, NC, RECON, true, true, false, EPI_RESID32>(a, grid, stream);
+    else if (epi == EPI_RESID16) launch_k<P, NC, RECON, true, true, false, EPI_RESID16>(a, grid, stream);
     else if (epi == EPI_CG64) launch_k<P, NC, RECON, true, true, false, EPI_CG64>(a, grid, stream);
     else return hipErrorInvalidValue;
   } else {
@@ -113,6 +115,8 @@ static hipError_t launch_eo_p(int dag, int mode, int epi, const DslashArgs& a, h
   if (!dag) return hipErrorInvalidValue;
   if constexpr (P == F64) {
     if (epi == EPI_RESID) WL_EOL(true, 1, EPI_RESID);
+    else if (epi == EPI_RESID32) WL_EOL(true, 1, EPI_RESID32);
+    else if (epi == EPI_RESID16) WL_EOL(true, 1, EPI_RESID16);
     else if (epi == EPI_CG64) WL_EOL(true, 1, EPI_CG64);
     else return hipErrorInvalidValue;
   } else {

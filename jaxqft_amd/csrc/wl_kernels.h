@@ -74,6 +74,9 @@ enum Epilogue {
   EPI_CG64 = 3,   // CG step with A p = result (not stored): y += alpha w, r -= alpha A p; partial |r|^2; y in F64
   EPI_CG32 = 4,   //   ... y in F32
   EPI_CG16 = 5,   //   ... y in F16
+  EPI_RESID32 = 6,  // FP64 true residual stored in the inner precision: res = w - result in F32
+                    // (column stride col_bytes_y), partial |w - result|^2 from the FP64 values
+  EPI_RESID16 = 7,  //   ... in F16 with per-site scales
 };
 
 struct DslashArgs {
