@@ -70,6 +70,9 @@ def parse():
     ap.add_argument("--no-lime", action="store_true", help="skip the SciDAC payload decode")
     ap.add_argument("--profile-only", action="store_true",
                     help="run only the Dslash variants and a calibration copy (for rocprofv3 counter passes)")
+    ap.add_argument("--solve-trace", action="store_true",
+                    help="tune as the full run does, then run the headline solve once more after a 1 s pause "
+                         "(for a kernel trace of that solve alone) and stop")
     return ap.parse_args()
 
 
@@ -277,6 +280,19 @@ def main():
             }
             log(f"dslash {prec} r{recon} k={k}: {best[1]:.1f} us ({best[1]/k:.1f}/rhs) {gbs:.0f} GB/s {tried}")
             del fi, fo
+
+        if args.solve_trace:
+            # one warm headline solve, a pause that marks the trace, the traced solve
+            bk = nat.Field(ctx, nat.WL_F64, k)
+            bk.import_(bk_t)
+            xk = nat.Field(ctx, nat.WL_F64, k)
+            B.solve(ctx, gauge, bk, xk, k, args.inner, args.recon)
+            torch.cuda.synchronize()
+            time.sleep(1.0)
+            t, st = B.timed_solves(1, ctx, gauge, bk, xk, k, args.inner, args.recon)
+            log(f"traced mixed {k}-RHS solve {t*1e3:.1f} ms {summarize(st)}")
+            print(json.dumps({"solve_trace": True, "ms": round(t * 1e3, 2), **summarize(st)}))
+            return
 
         # ------------------------------------------------------------ solves
         b1 = nat.Field(ctx, nat.WL_F64, 1)
