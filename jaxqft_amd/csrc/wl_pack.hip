@@ -130,6 +130,39 @@ __global__ __launch_bounds__(NT) void k_convert_gauge(const void* __restrict__ s
   LinkStore<P, NC>::template store<RECON>(d, Vp, site, u);
 }
 
+// Largest |U_2j - conj(U_0 x U_1)_j| over the SU(3) links of the FP64 master:
+// how far the stored third row is from the one a 12-real encoding rebuilds.
+// A wavefront max (NaN kept), then one integer atomic max per wavefront on the
+// bits of the non-negative double (for those, bit order is value order).
+__device__ __forceinline__ double max_keep_nan(double a, double b) { return (a > b || a != a) ? a : b; }
+
+__global__ __launch_bounds__(NT) void k_row2_deviation(const void* __restrict__ src, long long V, long long Vp,
+                                                       size_t cfg_bytes, size_t dir_bytes,
+                                                       unsigned long long* __restrict__ out) {
+  const size_t site = (size_t)blockIdx.x * NT + threadIdx.x;
+  const int mu = blockIdx.y;
+  const int cfg = blockIdx.z;
+  double dev = 0.0;
+  if (site < (size_t)V) {
+    const double2* s =
+        reinterpret_cast<const double2*>(reinterpret_cast<const char*>(src) + cfg * cfg_bytes + mu * dir_bytes);
+    cx<double> u[3][3], r[3][3];
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) {
+        const double2 t = s[(a * 3 + b) * Vp + site];
+        u[a][b] = {t.x, t.y};
+        r[a][b] = u[a][b];
+      }
+    reconstruct_row2(r);
+    for (int b = 0; b < 3; ++b) {
+      const double dr = u[2][b].re - r[2][b].re, di = u[2][b].im - r[2][b].im;
+      dev = max_keep_nan(dev, sqrt(dr * dr + di * di));
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) dev = max_keep_nan(dev, __shfl_xor(dev, off));
+  if ((threadIdx.x & 63u) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(dev));
+}
+
 // Even-odd layouts.  Spinors: the sites of parity q of a full-lattice
 // reference array, compact (ascending full index).  Unpacking writes the
 // whole full-lattice array: the compact values at parity-q sites and zero at
@@ -268,6 +301,13 @@ hipError_t pack_gauge_f64(int nc, const void* src, int dtype, int layout, void* 
       hipLaunchKernelGGL((k_pack_gauge<F64, 1, 18, float>), grid, block, 0, st, (const float*)src, dst, V, Vp, layout,
                          cfg_bytes, dir_bytes);
   }
+  return hipGetLastError();
+}
+
+hipError_t row2_deviation(const void* src, long long V, long long Vp, int ncfg, size_t cfg_bytes, size_t dir_bytes,
+                          unsigned long long* out, hipStream_t st) {
+  dim3 grid((unsigned)((V + NT - 1) / NT), 4, (unsigned)ncfg), block(NT);
+  hipLaunchKernelGGL(k_row2_deviation, grid, block, 0, st, src, V, Vp, cfg_bytes, dir_bytes, out);
   return hipGetLastError();
 }
 

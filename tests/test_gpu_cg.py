@@ -116,6 +116,35 @@ def test_status_maxiter_and_nan():
     assert th2.last_solve_stats[0]["status"] == "converged" and np.all(x0 == 0)
 
 
+@pytest.mark.parametrize("perturb", [0.0, 1e-8])
+@pytest.mark.parametrize("prec", ["f64", "f16"])
+def test_true_residual_on_links_as_given(perturb, prec):
+    """The reliable updates and the final true residual use 12-real FP64
+    links only when those rebuild the stored third rows to 1e-14: on exact
+    SU(3) links and on links pushed 1e-8 off the group (where a rebuilt third
+    row would change the operator by ~1e-8), every solve meets 1e-10 by the
+    oracle applied to the links exactly as given."""
+    print(f"[gpu] true residual on given links perturb={perturb} {prec}", flush=True)
+    from jaxqft_amd import fields
+    from jaxqft_amd.su3_wilson_nf2 import SU3WilsonNf2
+
+    L = (4, 4, 4, 8)
+    bc = (1, 1, 1, -1)
+    rng = fields.rng_for(650)
+    U = fields.su3_hot_start(rng, 1, L, 0.5)
+    if perturb:
+        U = U + perturb * (rng.standard_normal(U.shape) + 1j * rng.standard_normal(U.shape))
+    b = fields.random_fermion(rng, (1, *L, 4, 3))
+    kw = dict(solver_precision=prec, solver_recon=12) if prec != "f64" else dict(solver_precision="f64")
+    th = SU3WilsonNf2(lattice_shape=L, mass=0.05, fermion_bc=bc, dtype=np.complex128, cg_tol=1e-10,
+                      cg_maxiter=3000, **kw)
+    x = np.asarray(th.solve_normal(U, b))
+    assert th.last_solve_stats[0]["status"] == "converged", th.last_solve_stats
+    res = true_resid(L, 3, 0.05, bc, U, b, x)
+    assert res <= 1e-10, res
+    assert abs(res - th.last_solve_stats[0]["true_resid"]) <= 1e-3 * res + 1e-15
+
+
 @pytest.mark.parametrize("prec", ["f64", "f16"])
 def test_zero_guess_start_matches_explicit_zero_guess(prec):
     """A solve without a guess starts from r = b with |r|^2 = |b|^2 and no
