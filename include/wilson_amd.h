@@ -193,7 +193,10 @@ int wl_ctx_get_tuning(wl_ctx* ctx, int prec, int nrhs, wl_tuning* t);
  * (automatic ones included). */
 int wl_ctx_set_eo_slab(wl_ctx* ctx, int prec, int multi_rhs, int eo_slab);
 
-/* where: 0 = device pointer, 1 = host pointer */
+/* where: 0 = device pointer, 1 = host pointer.  The upload also measures how
+ * far each SU(3) link's stored third row is from conj(row0 x row1); when every
+ * link agrees to 1e-14 the FP64 operator of the reliable updates and true
+ * residuals reads 12-real links (the same operator to rounding). */
 int wl_gauge_upload(wl_ctx* ctx, const void* U, int layout, int dtype, int where, wl_gauge** out);
 void wl_gauge_release(wl_gauge* g);
 /* prebuild the (prec, recon) encoding of a gauge field (otherwise built on first use) */
