@@ -20,7 +20,7 @@ namespace wl {
 hipError_t launch_dslash(int prec, int nc, int recon, int op, int epi, const DslashArgs& a, hipStream_t stream);
 hipError_t launch_norm2(int p, int nc, BlasArgs a, hipStream_t stream);
 hipError_t launch_xpay(int p, int nc, BlasArgs a, hipStream_t stream);
-hipError_t launch_accumulate(int px, int py, int nc, BlasArgs a, hipStream_t stream);
+hipError_t launch_accumulate(int px, int py, int pp, int nc, BlasArgs a, hipStream_t stream);
 hipError_t launch_convert(int po, int pi, int nc, BlasArgs a, hipStream_t stream);
 hipError_t launch_finalize_update(ColState* cols, int ncol, hipStream_t stream);
 hipError_t launch_trueres_from_b2(ColState* cols, int ncol, hipStream_t stream);
@@ -114,7 +114,7 @@ struct wl_ctx {
   std::mutex mu;
   // workspace
   DevBuf partials, cols;
-  DevBuf ws[11];
+  DevBuf ws[12];
   long long Vh = 0;   // sites of one parity (0: even-odd operators unavailable, some L odd)
   long long Vhp = 0;  // plane stride of compact (one-parity) fields
   ColState* host_cols = nullptr;  // pinned mirror
@@ -777,7 +777,8 @@ struct Solver {
   const void* UP;
   void *b, *x;    // F64 caller-side internal fields
   void *rhi, *thi;
-  void *r, *p, *t, *y;
+  void *r, *t, *y;
+  void* pb[2];    // search directions, used in turn: iteration k reads pb[k % 2] and writes pb[(k + 1) % 2]
   void *mid64 = nullptr, *midP = nullptr;  // odd-parity intermediates of S_e (even-odd only)
   Reduction red;
 
@@ -838,15 +839,22 @@ struct Solver {
     return WL_OK;
   }
 
-  // gated reliable update: x += y; r = b - A x (FP64 arithmetic, stored in the
-  // inner precision); p = r + beta p
-  int reliable_update() {
+  // x += y for the columns the gate admits, including an alpha p still held
+  // back by a pairwise column
+  hipError_t accumulate(BlasArgs a) {
+    a.a = x; a.col_bytes_a = cb64;
+    a.b = y; a.col_bytes_b = cbY;
+    a.c = pb[0]; a.d = pb[1]; a.col_bytes_c = cbP; a.col_bytes_d = cbP;
+    return launch_accumulate(F64, PY, P, nc, a, st);
+  }
+
+  // gated reliable update before an iteration that reads pb[par ^ 1]:
+  // x += y; r = b - A x (FP64 arithmetic, stored in the inner precision);
+  // pb[par ^ 1] = r + beta p (p: each column's latest search direction)
+  int reliable_update(int par) {
     hipError_t e;
     if (mixed) {
-      BlasArgs a = bargs(2);
-      a.a = x; a.col_bytes_a = cb64;
-      a.b = y; a.col_bytes_b = cbY;
-      e = launch_accumulate(F64, PY, nc, a, st);
+      e = accumulate(bargs(2));
       if (e != hipSuccess) return fail(WL_ERR_HIP, "accumulate: %s", hipGetErrorString(e));
     }
     DslashArgs d;
@@ -863,21 +871,29 @@ struct Solver {
     } else {
       CHK(apply(d, F64, R64, true, thi, rhi, mid64, EPI_RESID, RA_TRUERES));
     }
-    {
-      BlasArgs a = bargs(2);
-      a.a = p; a.col_bytes_a = cbP;
-      a.b = r; a.col_bytes_b = cbP;
-      e = launch_xpay(P, nc, a, st);
-      if (e != hipSuccess) return fail(WL_ERR_HIP, "xpay: %s", hipGetErrorString(e));
-    }
+    CHK(xpay(bargs(2), par, true));
     e = launch_finalize_update(red.cols, ncol, st);
     if (e != hipSuccess) return fail(WL_ERR_HIP, "finalize: %s", hipGetErrorString(e));
     return WL_OK;
   }
 
+  // pb[par ^ 1] = r + beta p for the columns the gate admits, p = pb[par]
+  // (an iteration) or each column's latest search direction (an update,
+  // whose column may have stopped iterating in an earlier iteration)
+  int xpay(BlasArgs a, int par, bool latest) {
+    a.a = latest ? pb[0] : pb[par]; a.col_bytes_a = cbP;
+    a.d = latest ? pb[1] : nullptr;
+    a.b = r; a.col_bytes_b = cbP;
+    a.c = pb[par ^ 1];
+    hipError_t e = launch_xpay(P, nc, a, st);
+    if (e != hipSuccess) return fail(WL_ERR_HIP, "xpay: %s", hipGetErrorString(e));
+    return WL_OK;
+  }
+
   // The first update of a solve from x = 0: r_hi = b - A 0 = b exactly, so
   // the FP64 operator applications are skipped: r = b (in the inner
-  // precision), |r_hi|^2 = |b|^2 decides convergence, p = r.
+  // precision), |r_hi|^2 = |b|^2 decides convergence, p = r (into pb[0],
+  // where the first iteration reads it; pb[1] is zero and beta is 0).
   int initial_update_zero_guess() {
     hipError_t e;
     {
@@ -889,13 +905,7 @@ struct Solver {
     }
     e = launch_trueres_from_b2(red.cols, ncol, st);
     if (e != hipSuccess) return fail(WL_ERR_HIP, "true residual: %s", hipGetErrorString(e));
-    {
-      BlasArgs a = bargs(2);
-      a.a = p; a.col_bytes_a = cbP;
-      a.b = r; a.col_bytes_b = cbP;
-      e = launch_xpay(P, nc, a, st);
-      if (e != hipSuccess) return fail(WL_ERR_HIP, "xpay: %s", hipGetErrorString(e));
-    }
+    CHK(xpay(bargs(2), 1, true));
     e = launch_finalize_update(red.cols, ncol, st);
     if (e != hipSuccess) return fail(WL_ERR_HIP, "finalize: %s", hipGetErrorString(e));
     return WL_OK;
@@ -903,29 +913,26 @@ struct Solver {
 
   // One CG iteration on A = D^dag D, two stencil launches with fused vector
   // work: t = D p with |t|^2 = <p, A p> (alpha), then A p = D^dag t consumed
-  // in the epilogue by y += alpha p, r -= alpha A p and |r|^2 (beta); then
-  // p = r + beta p.
-  int iteration() {
-    hipError_t e;
+  // in the epilogue by r -= alpha A p, |r|^2 (beta) and the y update; then
+  // p' = r + beta p into the other buffer.  The iteration of parity par reads
+  // p = pb[par]; the previous iteration's p, in pb[par ^ 1], lets a pairwise
+  // column add alpha p of two iterations to y at once (every second
+  // iteration), which halves the y traffic of the epilogue.
+  int iteration(int par) {
     DslashArgs d;
     dargs(d, P, UP, R);
     d.gate = 1;
     d.col_bytes = cbP;
-    CHK(apply(d, P, R, false, p, t, midP, EPI_NORM, RA_PAP));
-    d.w = p; d.res = r; d.y = y; d.col_bytes_y = cbY;
+    d.red.par = par;
+    CHK(apply(d, P, R, false, pb[par], t, midP, EPI_NORM, RA_PAP));
+    d.w = pb[par]; d.w2 = pb[par ^ 1]; d.res = r; d.y = y; d.col_bytes_y = cbY;
     const int epi = PY == F64 ? EPI_CG64 : (PY == F32 ? EPI_CG32 : EPI_CG16);
     CHK(apply(d, P, R, true, t, nullptr, midP, epi, RA_ITER));
-    {
-      BlasArgs a = bargs(1);
-      a.a = p; a.col_bytes_a = cbP;
-      a.b = r; a.col_bytes_b = cbP;
-      e = launch_xpay(P, nc, a, st);
-      if (e != hipSuccess) return fail(WL_ERR_HIP, "xpay: %s", hipGetErrorString(e));
-    }
+    CHK(xpay(bargs(1), par, false));
     // Mixed precision: the reliable update runs on the device right after the
     // iteration that requested it (all its kernels are gated on the column's
     // need_update flag), so no host round trip is needed.
-    if (fused) return reliable_update();
+    if (fused) return reliable_update(par);
     return WL_OK;
   }
 };
@@ -992,11 +999,13 @@ static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int n
   // workspace
   if (!s.mixed) CHK(c->ws[0].ensure(s.cb64 * ncol));  // r_hi (a mixed solve stores r straight in the inner precision)
   CHK(c->ws[1].ensure(s.cb64 * ncol));  // t_hi
-  CHK(c->ws[2].ensure(s.cbP * ncol));   // p
+  CHK(c->ws[2].ensure(s.cbP * ncol));   // p (buffer 0)
+  CHK(c->ws[11].ensure(s.cbP * ncol));  // p (buffer 1)
   CHK(c->ws[4].ensure(s.cbP * ncol));   // t
   s.rhi = s.mixed ? nullptr : c->ws[0].p;
   s.thi = c->ws[1].p;
-  s.p = c->ws[2].p;
+  s.pb[0] = c->ws[2].p;
+  s.pb[1] = c->ws[11].p;
   s.t = c->ws[4].p;
   if (eo) {
     CHK(c->ws[3].ensure(s.cbP * ncol));    // odd intermediate, inner precision
@@ -1026,7 +1035,8 @@ static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int n
   s.red.cols = (ColState*)c->cols.p;
   s.red.action = RA_NONE;
 
-  HIPCHK(hipMemsetAsync(s.p, 0, s.cbP * ncol, st));
+  HIPCHK(hipMemsetAsync(s.pb[0], 0, s.cbP * ncol, st));
+  HIPCHK(hipMemsetAsync(s.pb[1], 0, s.cbP * ncol, st));
   if (s.mixed) HIPCHK(hipMemsetAsync(s.y, 0, s.cbY * ncol, st));
   if (!use_x0) HIPCHK(hipMemsetAsync(x, 0, s.cb64 * ncol, st));
 
@@ -1039,6 +1049,8 @@ static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int n
     cs.maxiter = sp->maxiter;
     cs.mode = sp->stop_rule == WL_STOP_ITERATED ? 1 : 0;
     cs.delta = s.mixed ? effective_delta(sp->delta, s.P) : 0.0;
+    cs.pairwise = s.mixed ? 1 : 0;
+    cs.p_par = 1;  // pb[1] is zero: the first update sets p = r in pb[0]
     cs.need_update = 1;
     cs.active = 0;
     cs.status = WL_STATUS_RUNNING;
@@ -1060,16 +1072,17 @@ static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int n
     hc[i].stop2 = tol2 * hc[i].b2;
   }
   HIPCHK(hipMemcpyAsync(c->cols.p, hc, sizeof(ColState) * ncol, hipMemcpyHostToDevice, st));
-  CHK(use_x0 ? s.reliable_update() : s.initial_update_zero_guess());
+  CHK(use_x0 ? s.reliable_update(1) : s.initial_update_zero_guess());
 
-  // Iteration blocks of `poll`, 4 and 1 iterations (as HIP graphs when
+  // Iteration blocks of `poll`, 4 and 2 iterations (as HIP graphs when
   // requested); after each block the host reads the per-column state and
   // picks the next block from the predicted number of remaining iterations
   // (the observed residual decay rate), so little work is queued past
-  // convergence.
-  const int poll = sp->poll_every > 0 ? sp->poll_every : 8;
+  // convergence.  Every block has an even length, so each one starts on
+  // search-direction buffer 0 and ends having written it.
+  const int poll = sp->poll_every > 0 ? (sp->poll_every + 1) / 2 * 2 : 8;
   std::vector<int> sizes;
-  for (int n : {poll, 4, 1})
+  for (int n : {poll, 4, 2})
     if (n <= poll && (sizes.empty() || n < sizes.back())) sizes.push_back(n);
   std::vector<hipGraph_t> graphs(sizes.size(), nullptr);
   std::vector<hipGraphExec_t> execs(sizes.size(), nullptr);
@@ -1087,7 +1100,7 @@ static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int n
         return fail(WL_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
       }
       int rc = WL_OK;
-      for (int i = 0; i < sizes[k] && rc == WL_OK; ++i) rc = s.iteration();
+      for (int i = 0; i < sizes[k] && rc == WL_OK; ++i) rc = s.iteration(i & 1);
       e = hipStreamEndCapture(st, &graphs[k]);
       if (rc != WL_OK) {
         cleanup();
@@ -1135,7 +1148,7 @@ static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int n
       break;
     }
     if (any_update) {
-      result = s.reliable_update();
+      result = s.reliable_update(1);
       if (result != WL_OK) break;
       continue;
     }
@@ -1154,7 +1167,7 @@ static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int n
         break;
       }
     } else {
-      for (int i = 0; i < sizes[k] && result == WL_OK; ++i) result = s.iteration();
+      for (int i = 0; i < sizes[k] && result == WL_OK; ++i) result = s.iteration(i & 1);
       if (result != WL_OK) break;
     }
   }
@@ -1162,10 +1175,7 @@ static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int n
   if (result != WL_OK) return result;
   if (s.mixed) {
     // fold any remaining inner correction (columns stopped by NaN/breakdown)
-    BlasArgs a = s.bargs(0);
-    a.a = x; a.col_bytes_a = s.cb64;
-    a.b = s.y; a.col_bytes_b = s.cbY;
-    hipError_t e = launch_accumulate(F64, s.PY, nc, a, st);
+    hipError_t e = s.accumulate(s.bargs(0));
     if (e != hipSuccess) return fail(WL_ERR_HIP, "accumulate: %s", hipGetErrorString(e));
   }
   if (stats) {
@@ -1353,7 +1363,7 @@ struct wl_batch {
 
 static size_t batch_bytes_per_col(const wl_ctx* c, int prec) {
   const bool mixed = prec != F64;
-  size_t b = 3 * col_bytes_of(c, F64) + 2 * col_bytes_of(c, prec);  // b x t_hi; p t
+  size_t b = 3 * col_bytes_of(c, F64) + 3 * col_bytes_of(c, prec);  // b x t_hi; two p, t
   if (mixed) b += col_bytes_of(c, prec) + col_bytes_of(c, F64);  // r and an accumulator of at most FP64
   else b += col_bytes_of(c, F64);                                 // r_hi (the FP64 iterated residual)
   return b + sizeof(double) * max_tiles(c) + sizeof(ColState);
@@ -1392,7 +1402,7 @@ int wl_batch_create(wl_ctx* c, wl_gauge* g, int capacity, const wl_solve_params*
   bt->cap = capacity;
   bt->sp = *sp;
   bt->sp.mass = nullptr;
-  bt->poll = sp->poll_every > 0 ? sp->poll_every : 8;
+  bt->poll = sp->poll_every > 0 ? (sp->poll_every + 1) / 2 * 2 : 8;  // even: blocks start on search-direction buffer 0
   bt->tol2 = sp->tol * sp->tol;
   bt->busy.assign(capacity, 0);
   Solver& s = bt->s;
@@ -1421,7 +1431,8 @@ int wl_batch_create(wl_ctx* c, wl_gauge* g, int capacity, const wl_solve_params*
   s.x = alloc(1, s.cb64 * capacity);
   s.rhi = s.mixed ? nullptr : alloc(2, s.cb64 * capacity);
   s.thi = alloc(3, s.cb64 * capacity);
-  s.p = alloc(4, s.cbP * capacity);
+  s.pb[0] = alloc(4, s.cbP * capacity);
+  s.pb[1] = alloc(5, s.cbP * capacity);
   s.t = alloc(6, s.cbP * capacity);
   if (s.mixed) {
     s.r = alloc(7, s.cbP * capacity);
@@ -1463,7 +1474,7 @@ int wl_batch_create(wl_ctx* c, wl_gauge* g, int capacity, const wl_solve_params*
   if (sp->use_graph) {
     e = hipStreamBeginCapture(bt->st, hipStreamCaptureModeThreadLocal);
     int r2 = WL_OK;
-    for (int i = 0; i < bt->poll && r2 == WL_OK && e == hipSuccess; ++i) r2 = s.iteration();
+    for (int i = 0; i < bt->poll && r2 == WL_OK && e == hipSuccess; ++i) r2 = s.iteration(i & 1);
     hipError_t e2 = hipStreamEndCapture(bt->st, &bt->graph);
     if (e == hipSuccess) e = e2;
     if (e == hipSuccess) e = hipGraphInstantiate(&bt->exec, bt->graph, nullptr, nullptr, 0);
@@ -1494,7 +1505,8 @@ int wl_batch_admit(wl_batch* bt, int slot, const void* b, int dtype, double mass
   hipStream_t st = bt->st;
   hipError_t e = pack_spinor(F64, nc, b, dtype, (char*)s.b + slot * s.cb64, c->V, c->Vp, 1, s.cb64, st);
   if (e == hipSuccess) e = hipMemsetAsync((char*)s.x + slot * s.cb64, 0, s.cb64, st);
-  if (e == hipSuccess) e = hipMemsetAsync((char*)s.p + slot * s.cbP, 0, s.cbP, st);
+  if (e == hipSuccess) e = hipMemsetAsync((char*)s.pb[0] + slot * s.cbP, 0, s.cbP, st);
+  if (e == hipSuccess) e = hipMemsetAsync((char*)s.pb[1] + slot * s.cbP, 0, s.cbP, st);
   if (e == hipSuccess && s.mixed) e = hipMemsetAsync((char*)s.y + slot * s.cbY, 0, s.cbY, st);
   if (e != hipSuccess) return fail(WL_ERR_HIP, "admit: %s", hipGetErrorString(e));
   ColState& cs = bt->host[slot];
@@ -1503,6 +1515,8 @@ int wl_batch_admit(wl_batch* bt, int slot, const void* b, int dtype, double mass
   cs.maxiter = bt->sp.maxiter;
   cs.mode = bt->sp.stop_rule == WL_STOP_ITERATED ? 1 : 0;
   cs.delta = s.mixed ? effective_delta(bt->sp.delta, s.P) : 0.0;
+  cs.pairwise = s.mixed ? 1 : 0;
+  cs.p_par = 1;
   cs.status = WL_STATUS_RUNNING;
   e = hipMemcpyAsync((ColState*)bt->cols.p + slot, &cs, sizeof(ColState), hipMemcpyHostToDevice, st);
   if (e != hipSuccess) return fail(WL_ERR_HIP, "admit: %s", hipGetErrorString(e));
@@ -1529,7 +1543,7 @@ int wl_batch_step(wl_batch* bt) {
     if (e != hipSuccess) return fail(WL_ERR_HIP, "graph launch: %s", hipGetErrorString(e));
     return WL_OK;
   }
-  for (int i = 0; i < bt->poll; ++i) CHK(bt->s.iteration());
+  for (int i = 0; i < bt->poll; ++i) CHK(bt->s.iteration(i & 1));
   return WL_OK;
 }
 
@@ -1567,9 +1581,7 @@ int wl_batch_retire(wl_batch* bt, int slot, void* x, int dtype) {
   if (s.mixed) {
     BlasArgs a = s.bargs(3);
     a.red.only_col = slot;
-    a.a = s.x; a.col_bytes_a = s.cb64;
-    a.b = s.y; a.col_bytes_b = s.cbY;
-    hipError_t e = launch_accumulate(F64, s.PY, c->geom.nc, a, bt->st);
+    hipError_t e = s.accumulate(a);
     if (e != hipSuccess) return fail(WL_ERR_HIP, "retire: %s", hipGetErrorString(e));
   }
   hipError_t e = unpack_spinor(F64, c->geom.nc, (char*)s.x + slot * s.cb64, x, dtype, c->V, c->Vp, 1, s.cb64, bt->st);

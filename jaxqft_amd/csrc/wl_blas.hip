@@ -59,31 +59,47 @@ __global__ __launch_bounds__(NT) void k_norm2(BlasArgs a) {
   WL_BLAS_EPILOGUE
 }
 
-// p = r + beta p.   a=p, b=r
+// p' = r + beta p.   b=r; p from a, or, when d is set, from a or d as the
+// column's p_par says (buffer 0 or 1); p' goes to c (null: back to the source)
 template <int P, int NC>
 __global__ __launch_bounds__(NT) void k_xpay(BlasArgs a) {
   WL_BLAS_PROLOGUE
   using T = typename PT<P>::acc;
-  const T beta = (T)a.red.cols[col].beta;
+  const ColState& cs = a.red.cols[col];
+  const T beta = (T)cs.beta;
+  void* src = (a.d && cs.p_par) ? a.d : a.a;
   cx<T> p[4 * NC], r[4 * NC];
-  load_as<P, NC>((char*)a.a + col * a.col_bytes_a, V, s, p);
+  load_as<P, NC>((char*)src + col * a.col_bytes_a, V, s, p);
   load_as<P, NC>((char*)a.b + col * a.col_bytes_b, V, s, r);
 #pragma unroll
   for (int c = 0; c < 4 * NC; ++c) {
     p[c].re = r[c].re + beta * p[c].re;
     p[c].im = r[c].im + beta * p[c].im;
   }
-  if (valid) store_as<P, NC>((char*)a.a + col * a.col_bytes_a, V, s, p);
+  if (valid) store_as<P, NC>((char*)(a.c ? a.c : src) + col * a.col_bytes_a, V, s, p);
   WL_BLAS_EPILOGUE
 }
 
-// x += y ; y = 0.   a=x (PX), b=y (PY)
-template <int PX, int PY, int NC>
+// x += y (+ alpha_prev p when a pairwise column has alpha p pending); y = 0.
+// a=x (PX), b=y (PY), c and d = the two search-direction buffers (PP; the
+// column's p_par picks the one holding the pending p; null: none pending)
+template <int PX, int PY, int PP, int NC>
 __global__ __launch_bounds__(NT) void k_accumulate(BlasArgs a) {
   WL_BLAS_PROLOGUE
   cx<double> x[4 * NC], y[4 * NC];
   load_as<PX, NC>((char*)a.a + col * a.col_bytes_a, V, s, x);
   load_as<PY, NC>((char*)a.b + col * a.col_bytes_b, V, s, y);
+  const ColState& cs = a.red.cols[col];
+  if (a.c && cs.pairwise && cs.pend) {
+    cx<double> p[4 * NC];
+    load_as<PP, NC>((char*)(cs.p_par ? a.d : a.c) + col * a.col_bytes_c, V, s, p);
+    const double ap = cs.alpha_prev;
+#pragma unroll
+    for (int c = 0; c < 4 * NC; ++c) {
+      y[c].re += ap * p[c].re;
+      y[c].im += ap * p[c].im;
+    }
+  }
   cx<double> z[4 * NC];
 #pragma unroll
   for (int c = 0; c < 4 * NC; ++c) {
@@ -116,7 +132,7 @@ __global__ __launch_bounds__(NT) void k_reduce_finish(Reduction red, int nblk, i
   double s = 0.0;
   for (int i = threadIdx.x; i < nblk; i += NT) s += parts[i];
   s = block_sum<NT>(s, lds);
-  if (threadIdx.x == 0) apply_action(red.action, red.cols[col], s);
+  if (threadIdx.x == 0) apply_action(red.action, red.cols[col], s, red.par);
 }
 
 hipError_t launch_reduce_finish(const Reduction& red, int ncol, int nblk, int gate, hipStream_t stream) {
@@ -157,6 +173,7 @@ __global__ void k_finalize_update(ColState* cols, int ncol) {
   if (c >= ncol) return;
   ColState& s = cols[c];
   if (!s.need_update) return;
+  s.pend = 0;  // the update's x += y took any alpha p held back
   if (!s.done) {
     s.rr = s.rtrue2;
     s.r0norm = sqrt(s.rtrue2);
@@ -201,11 +218,19 @@ hipError_t launch_xpay(int p, int nc, BlasArgs a, hipStream_t stream) {
   return WL_DISPATCH_NC(k_xpay, F16);
 }
 
-hipError_t launch_accumulate(int px, int py, int nc, BlasArgs a, hipStream_t stream) {
+// pp: precision of the pending search direction a.c (inner precision)
+hipError_t launch_accumulate(int px, int py, int pp, int nc, BlasArgs a, hipStream_t stream) {
   dim3 grid = blas_grid(a), block(NT);
-  if (px == F64 && py == F32) return WL_DISPATCH_NC(k_accumulate, F64, F32);
-  if (px == F64 && py == F64) return WL_DISPATCH_NC(k_accumulate, F64, F64);
-  if (px == F64 && py == F16) return WL_DISPATCH_NC(k_accumulate, F64, F16);
+  if (px != F64) return hipErrorInvalidValue;
+  if (pp == F32) {
+    if (py == F32) return WL_DISPATCH_NC(k_accumulate, F64, F32, F32);
+    if (py == F64) return WL_DISPATCH_NC(k_accumulate, F64, F64, F32);
+    if (py == F16) return WL_DISPATCH_NC(k_accumulate, F64, F16, F32);
+  } else if (pp == F16) {
+    if (py == F32) return WL_DISPATCH_NC(k_accumulate, F64, F32, F16);
+    if (py == F64) return WL_DISPATCH_NC(k_accumulate, F64, F64, F16);
+    if (py == F16) return WL_DISPATCH_NC(k_accumulate, F64, F16, F16);
+  }
   return hipErrorInvalidValue;
 }
 

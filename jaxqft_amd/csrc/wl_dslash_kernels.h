@@ -145,28 +145,45 @@ __device__ __forceinline__ double site_epilogue(const DslashArgs& a, int col, si
   } else {
     constexpr int PY = EpiAccum<EPI>::prec;
     using TY = typename PT<PY>::acc;
-    const double alpha = a.red.cols[col].alpha;
-    const char* pb = reinterpret_cast<const char*>(a.w) + (size_t)col * a.col_bytes;
+    const ColState& cs = a.red.cols[col];
+    const double alpha = cs.alpha;
     char* rb = reinterpret_cast<char*>(a.res) + (size_t)col * a.col_bytes;
-    char* yb = reinterpret_cast<char*>(a.y) + (size_t)col * a.col_bytes_y;
-    cx<T> p[4 * NC], r[4 * NC];
-    cx<TY> y[4 * NC];
-    SpinorIO<P, NC>::load(pb, V, s, p);
+    cx<T> r[4 * NC];
     SpinorIO<P, NC>::load(rb, V, s, r);
-    SpinorIO<PY, NC>::load(yb, V, s, y);
     const T ar = (T)alpha;
-    const TY ay = (TY)alpha;
 #pragma unroll
     for (int q = 0; q < 4 * NC; ++q) {
-      y[q].re += ay * (TY)p[q].re;
-      y[q].im += ay * (TY)p[q].im;
       r[q].re -= ar * res[q].re;
       r[q].im -= ar * res[q].im;
       part += (double)r[q].re * (double)r[q].re + (double)r[q].im * (double)r[q].im;
     }
-    if (valid) {
-      SpinorIO<PY, NC>::store(yb, V, s, y);
-      SpinorIO<P, NC>::store(rb, V, s, r);
+    if (valid) SpinorIO<P, NC>::store(rb, V, s, r);
+    // y += alpha p every iteration, or (pairwise) nothing in the first
+    // iteration of a pair and alpha_prev p_prev + alpha p in the second
+    if (!(cs.pairwise && !cs.pend)) {
+      const char* pb = reinterpret_cast<const char*>(a.w) + (size_t)col * a.col_bytes;
+      char* yb = reinterpret_cast<char*>(a.y) + (size_t)col * a.col_bytes_y;
+      cx<T> p[4 * NC];
+      cx<TY> y[4 * NC];
+      SpinorIO<P, NC>::load(pb, V, s, p);
+      SpinorIO<PY, NC>::load(yb, V, s, y);
+      const TY ay = (TY)alpha;
+#pragma unroll
+      for (int q = 0; q < 4 * NC; ++q) {
+        y[q].re += ay * (TY)p[q].re;
+        y[q].im += ay * (TY)p[q].im;
+      }
+      if (cs.pairwise) {
+        const char* qb = reinterpret_cast<const char*>(a.w2) + (size_t)col * a.col_bytes;
+        SpinorIO<P, NC>::load(qb, V, s, p);
+        const TY aq = (TY)cs.alpha_prev;
+#pragma unroll
+        for (int q = 0; q < 4 * NC; ++q) {
+          y[q].re += aq * (TY)p[q].re;
+          y[q].im += aq * (TY)p[q].im;
+        }
+      }
+      if (valid) SpinorIO<PY, NC>::store(yb, V, s, y);
     }
   }
   return valid ? part : 0.0;

@@ -26,6 +26,7 @@ struct ColState {
   double rr_old;    // |r|^2 at the start of the current iteration
   double pAp;       // <p, A p> = |D p|^2
   double alpha, beta;
+  double alpha_prev;  // alpha of the first iteration of a pair whose alpha p is not yet in y (pend)
   double b2;        // |b|^2
   double stop2;     // inner stopping threshold on rr
   double tol2b2;    // true-residual target tol^2 |b|^2
@@ -42,7 +43,9 @@ struct ColState {
   int nupdates;
   int mode;         // 0: stop on true residual, 1: stop on iterated residual (reference rule)
   int last_update;  // iteration of the last reliable update
-  int pad_;
+  int pairwise;     // mixed precision: y takes alpha p of two iterations at a time
+  int pend;         // pairwise: alpha_prev p of the previous iteration is still to be added
+  int p_par;        // search-direction buffer (0 or 1) holding the column's latest p
 };
 
 // A reliable update is also forced after this many inner iterations without
@@ -63,6 +66,7 @@ struct Reduction {
   ColState* cols;        // [ncol]
   int action;
   int only_col;          // column of gate 3
+  int par;               // RA_ITER: search-direction buffer of the iteration (pairwise y updates)
 };
 
 // Stencil epilogues.  EPI_NORM and the CG steps feed the fused CG iteration
@@ -87,6 +91,7 @@ struct DslashArgs {
   const void* in;
   void* out;
   const void* w;          // epilogue operand: b of EPI_RESID, p of the CG steps (precision of out)
+  const void* w2;         // CG steps: the previous iteration's search direction (pairwise y updates)
   void* res;              // CG steps: iterated residual r (precision of out)
   void* y;                // CG steps: solution accumulator (precision given by the epilogue)
   size_t col_bytes;       // bytes of one spinor column (in, out, w and r share precision)

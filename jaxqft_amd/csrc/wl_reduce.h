@@ -46,7 +46,7 @@ __device__ __forceinline__ void col_finish(ColState& s, int status) {
   s.active = 0;
 }
 
-__device__ __forceinline__ void apply_action(int action, ColState& s, double sum) {
+__device__ __forceinline__ void apply_action(int action, ColState& s, double sum, int par = 0) {
   switch (action) {
     case RA_PAP: {
       s.pAp = sum;
@@ -59,6 +59,12 @@ __device__ __forceinline__ void apply_action(int action, ColState& s, double sum
       break;
     }
     case RA_ITER: {
+      s.p_par = par;
+      if (s.pairwise) {
+        // the launch just finished held alpha p back (first of a pair) or added it (second)
+        s.alpha_prev = s.alpha;
+        s.pend ^= 1;
+      }
       s.rr_old = s.rr;
       s.rr = sum;
       s.beta = sum / s.rr_old;
