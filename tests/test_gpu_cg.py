@@ -116,6 +116,41 @@ def test_status_maxiter_and_nan():
     assert th2.last_solve_stats[0]["status"] == "converged" and np.all(x0 == 0)
 
 
+@pytest.mark.parametrize("prec", ["f32", "f16"])
+def test_mixed_solve_independent_of_block_length(prec):
+    """A mixed solve decides its reliable updates on the device and alternates
+    its two search-direction buffers by iteration: the iterates, and so the
+    solution and the iteration count, are the same bit for bit whatever the
+    host's block length (odd requests round up to even blocks) and with or
+    without graphs; updates fall after iterations of both parities (delta 0.1
+    triggers many), and every solve meets 1e-10 by the oracle."""
+    print(f"[gpu] block-length independence {prec}", flush=True)
+    from jaxqft_amd import fields
+    from jaxqft_amd.su3_wilson_nf2 import SU3WilsonNf2
+
+    L = (4, 4, 4, 8)
+    bc = (1, 1, 1, -1)
+    rng = fields.rng_for(660)
+    U = fields.su3_hot_start(rng, 1, L, 0.5)
+    b = fields.random_fermion(rng, (1, 3, *L, 4, 3))
+    ref = None
+    for poll, graph in ((1, False), (3, True), (16, True), (16, False)):
+        th = SU3WilsonNf2(lattice_shape=L, mass=0.05, fermion_bc=bc, dtype=np.complex128, cg_tol=1e-10,
+                          cg_maxiter=3000, solver_precision=prec, solver_recon=12, reliable_delta=0.1,
+                          poll_every=poll, use_graph=graph)
+        x = np.asarray(th.solve_normal_multi(U, b))
+        st = th.last_solve_stats
+        assert all(c["status"] == "converged" for c in st), st
+        assert max(c["reliable_updates"] for c in st) >= 3
+        got = (x, [c["iters"] for c in st])
+        if ref is None:
+            ref = got
+            for j in range(3):
+                assert true_resid(L, 3, 0.05, bc, U, b[:, j], x[:, j]) <= 1e-10
+        else:
+            assert got[1] == ref[1] and np.array_equal(got[0], ref[0]), (poll, graph)
+
+
 @pytest.mark.parametrize("perturb", [0.0, 1e-8])
 @pytest.mark.parametrize("prec", ["f64", "f16"])
 def test_true_residual_on_links_as_given(perturb, prec):
