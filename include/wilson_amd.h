@@ -118,7 +118,8 @@ typedef struct {
   double delta;     /* reliable-update trigger on the residual drop (mixed precision), e.g. 0.1;
                        at least 3e-3 (FP16 inner) or 1e-5 (FP32 inner) is used, and an update
                        is also forced after 100 inner iterations without one */
-  int poll_every;   /* iterations queued between host polls of the convergence flags */
+  int poll_every;   /* iterations queued between host polls of the convergence flags (rounded up
+                       to an even number: iterations alternate two search-direction buffers) */
   int use_graph;    /* capture the poll_every-iteration body as a HIP graph */
   const double* mass; /* host array of per-column masses, or NULL for the context mass */
   int accum_prec;     /* WL_ACCUM_*: storage of the inner solution accumulator between reliable updates */

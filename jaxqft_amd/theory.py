@@ -93,7 +93,7 @@ class WilsonNf2Theory:
     solver_recon: int = 18          # 18 | 12: link reconstruction of the inner operator
     reliable_delta: float = 0.1     # reliable-update trigger on the residual drop
     solver_stop: str = "true"       # "true": FP64 true residual <= cg_tol; "iterated": reference rule
-    poll_every: int = 8
+    poll_every: int = 8             # iterations per host poll (rounded up to an even number)
     use_graph: bool = False
     solver_accum: str = "auto"      # inner solution accumulator storage: auto (f32) | f64 | f32 | f16
     Nc: int = field(default=3, init=False)
