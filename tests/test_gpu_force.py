@@ -46,6 +46,26 @@ def test_force_matches_oracle(nc, layout):
     assert rel(got64, ref) < 2e-6
 
 
+@pytest.mark.parametrize("layout", ["BMXYIJ", "BXYMIJ"])
+@pytest.mark.parametrize("L", [(3, 5, 2, 7), (2, 2, 2, 2), (4, 4, 4, 16)])
+def test_force_ragged_blocks_and_wraps(L, layout):
+    """The force over 64-site workgroups: a last workgroup with idle lanes
+    (210 sites), a lattice smaller than one workgroup, odd and tiny extents
+    and a 16-site x3 row, with twisted phases, all match the oracle."""
+    print(f"[gpu] force ragged {L} {layout}", flush=True)
+    from jaxqft_amd import fields
+    from oracle.wilson_oracle import WilsonOracle
+
+    bc = (1, np.exp(-0.7j), 1, -1)
+    rng = fields.rng_for(1810 + sum(L))
+    U = fields.su3_hot_start(rng, 2, L, 0.6, layout=layout)
+    X = fields.random_fermion(rng, (2, *L, 4, 3))
+    Y = fields.random_fermion(rng, (2, *L, 4, 3))
+    th = theory(L, 3, 0.05, bc, batch=2, layout=layout)
+    ora = WilsonOracle(L, 3, 0.05, fermion_bc=bc, layout=layout)
+    assert rel(th.pseudofermion_force_from_solution(U, X, Y), ora.pseudofermion_force_from_solution(U, X, Y)) < 1e-13
+
+
 @pytest.mark.parametrize("nc", [3, 1])
 def test_eo_force_matches_oracle(nc):
     print(f"[gpu] eo force nc={nc}", flush=True)
