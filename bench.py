@@ -67,6 +67,8 @@ def parse():
     ap.add_argument("--no-config4", action="store_true", help="skip the 24^3x48 multi-RHS case")
     ap.add_argument("--no-config2", action="store_true", help="skip the 16^4 FP64 single-RHS case")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving scenario")
+    ap.add_argument("--no-propagator", action="store_true",
+                    help="skip the point-source propagator and chronological-guess measurements")
     ap.add_argument("--serving-requests", type=int, default=60)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -470,6 +472,8 @@ def main():
         out["fp64_16x16x16x16"] = config2(B, args, not args.no_cpu)
     if not args.no_config4:
         out["multi_rhs_24x24x24x48"] = config4(B, args)
+    if not args.no_propagator:
+        out.update(propagator_chrono(B, args, U, L))
     if not args.no_serving:
         out["serving"] = serving(B, args)
     if not args.no_lime:
@@ -533,6 +537,97 @@ def config2(B, args, with_cpu):
         res["gpu_over_cpu"] = round(cs / t, 1)
         log(f"16^4 CPU oracle solve {cs:.2f} s ({it} iterations, {threads} threads)")
     return res
+
+
+def propagator_chrono(B, args, U32, L32):
+    """The point-source propagator driver (measurements.py) at 32^4 through the
+    theory object, tuned as a user would: D^dag on the 12 unit point sources
+    and one 12-column mixed solve.  Chronological guesses at 16^4 along a
+    smooth gauge trajectory U_k = exp(k eps H) U (eps = 0.01), phi_k =
+    D_k^dag eta: iterations and time of each solve from a cold start and from
+    the MRE guess over the last four solutions (chrono.py)."""
+    torch, nat = B.torch, B.nat
+    from jaxqft_amd import fields
+    from jaxqft_amd.autotune import tune
+    from jaxqft_amd.chrono import chrono_predict_mre
+    from jaxqft_amd.measurements import full_point_source_propagator
+    from jaxqft_amd.su3_wilson_nf2 import SU3WilsonNf2
+
+    pol = dict(cg_tol=args.tol, cg_maxiter=20000, solver_precision=args.inner, solver_recon=args.recon,
+               reliable_delta=args.delta, solver_accum=args.accum, use_graph=bool(args.graph), poll_every=args.poll)
+    out = {}
+    with torch.cuda.stream(B.stream):
+        th = SU3WilsonNf2(lattice_shape=L32, mass=args.mass, fermion_bc=(1, 1, 1, -1), dtype=np.complex128, **pol)
+        rq = th.resident_gauge(torch.from_numpy(U32).cuda())
+        ctx = th._context()
+        probe = torch.from_numpy(fields.random_fermion(fields.rng_for(fields.SEED_BASE + 7), (12, *L32, 4, 3))).cuda()
+        for prec, recon in ((args.inner, args.recon), ("f64", 18)):
+            pc = nat.PREC[prec]
+            fi, fo = nat.Field(ctx, pc, 12), nat.Field(ctx, pc, 12)
+            fi.import_(probe)
+            rq.gauge.prepare(pc, recon)
+            tune(ctx, rq.gauge, pc, recon, fi, fo, nrhs=12, reps=3, stream=B.stream)
+            del fi, fo
+        del probe
+        src = (3, 5, 7, 11)
+        full_point_source_propagator(rq, th, src, as_numpy=False)  # graphs, workspaces
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        prop, info = full_point_source_propagator(rq, th, src, as_numpy=False)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        st = th.last_solve_stats
+        out["propagator_32x32x32x32"] = {
+            "wall_s": round(wall, 4), "solve_s": round(info["inv_solve_total_sec_step"], 4), "columns": 12,
+            "iters_max": max(c["iters"] for c in st), "true_resid_max": max(c["true_resid"] for c in st),
+            "status": sorted({c["status"] for c in st}),
+            "workload": "D^dag on the 12 unit point sources at (3,5,7,11), one 12-column mixed solve of D^dag D, "
+                        "the propagator reshaped to (B, *L, 4, 3, 4, 3) on the device",
+        }
+        log(f"propagator 32^4: {wall*1e3:.1f} ms (solve {info['inv_solve_total_sec_step']*1e3:.1f} ms)")
+        del prop, rq, th
+
+        L = (16, 16, 16, 16)
+        rng = fields.rng_for(fields.SEED_BASE + 8)
+        U0 = torch.from_numpy(fields.su3_hot_start(rng, 1, L, 0.5)).cuda()
+        a = torch.from_numpy(rng.standard_normal((1, 4, *L, 3, 3)) + 1j * rng.standard_normal((1, 4, *L, 3, 3))).cuda()
+        H = 0.5 * (a - a.conj().transpose(-1, -2))
+        H = H - (torch.diagonal(H, dim1=-2, dim2=-1).sum(-1) / 3.0)[..., None, None] * torch.eye(
+            3, dtype=H.dtype, device="cuda")
+        eta = torch.from_numpy(fields.random_fermion(rng, (1, *L, 4, 3))).cuda()
+        th = SU3WilsonNf2(lattice_shape=L, mass=0.05, fermion_bc=(1, 1, 1, -1), dtype=np.complex128, **pol)
+        xs, ys, cold, warm = [], [], [], []
+        for kk in range(8):
+            Uk = torch.linalg.matrix_exp((0.01 * kk) * H.reshape(-1, 3, 3)).reshape(H.shape) @ U0
+            rq = th.resident_gauge(Uk)
+            phi = th.apply_Ddag(rq, eta)
+            th.solve_normal(rq, phi)  # warm-up of this gauge's encodings and graphs
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            x = th.solve_normal(rq, phi)
+            torch.cuda.synchronize()
+            cold.append((th.last_solve_stats[0]["iters"], time.perf_counter() - t0))
+            if xs:
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                guess = chrono_predict_mre(xs, ys, eta, 4)
+                x = th.solve_normal_with_guess(rq, phi, guess)
+                torch.cuda.synchronize()
+                warm.append((th.last_solve_stats[0]["iters"], time.perf_counter() - t0))
+            xs.append(x)
+            ys.append(th.apply_D(rq, x))
+            del rq
+        out["chrono_16x16x16x16"] = {
+            "solves": len(warm), "cold_iters_mean": round(float(np.mean([c[0] for c in cold[1:]])), 2),
+            "mre_iters_mean": round(float(np.mean([w[0] for w in warm])), 2),
+            "cold_ms_mean": round(1e3 * float(np.mean([c[1] for c in cold[1:]])), 3),
+            "mre_ms_mean_incl_prediction": round(1e3 * float(np.mean([w[1] for w in warm])), 3),
+            "workload": "U_k = exp(0.01 k H) U, phi_k = D_k^dag eta, mixed solves to 1e-10; MRE guess from the "
+                        "last 4 (x, D x) pairs with target eta",
+        }
+        log(f"chrono 16^4: cold {out['chrono_16x16x16x16']['cold_iters_mean']} iters, "
+            f"MRE {out['chrono_16x16x16x16']['mre_iters_mean']} iters")
+    return out
 
 
 def config4(B, args):

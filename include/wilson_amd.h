@@ -194,7 +194,9 @@ int wl_ctx_get_tuning(wl_ctx* ctx, int prec, int nrhs, wl_tuning* t);
  * (automatic ones included). */
 int wl_ctx_set_eo_slab(wl_ctx* ctx, int prec, int multi_rhs, int eo_slab);
 
-/* where: 0 = device pointer, 1 = host pointer.  The upload also measures how
+/* where: 0 = device pointer, 1 = host pointer.  The call is synchronous and
+ * takes no stream: a device pointer is read after all device work queued so far
+ * (on every stream) has finished.  The upload also measures how
  * far each SU(3) link's stored third row is from conj(row0 x row1); when every
  * link agrees to 1e-14 the FP64 operator of the reliable updates and true
  * residuals reads 12-real links (the same operator to rounding). */

@@ -399,6 +399,17 @@ int wl_gauge_upload(wl_ctx* c, const void* U, int layout, int dtype, int where, 
   }
   const void* src = U;
   DevBuf staging;
+  if (where == 0) {
+    // the upload takes no stream: the links may still be in flight on any
+    // stream of the caller (a non-blocking stream does not order against the
+    // null stream the pack runs on), so all device work finishes first
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+      g->master.release();
+      delete g;
+      return fail(WL_ERR_HIP, "gauge upload: %s", hipGetErrorString(e));
+    }
+  }
   if (where == 1) {
     rc = staging.ensure(src_bytes);
     if (rc) {

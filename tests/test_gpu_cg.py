@@ -116,6 +116,52 @@ def test_status_maxiter_and_nan():
     assert th2.last_solve_stats[0]["status"] == "converged" and np.all(x0 == 0)
 
 
+def test_gauge_upload_waits_for_the_producing_stream():
+    """Links computed on a non-default torch stream and uploaded at once: the
+    upload (which takes no stream) reads them only after that work finishes,
+    so D on the resident links matches the oracle on the same links and the
+    mixed solve converges (a pack racing the producer would read partial
+    links)."""
+    print("[gpu] gauge upload ordering", flush=True)
+    import torch
+
+    from jaxqft_amd import fields
+    from jaxqft_amd.su3_wilson_nf2 import SU3WilsonNf2
+    from oracle.wilson_oracle import WilsonOracle
+
+    L = (16, 16, 16, 16)
+    bc = (1, 1, 1, -1)
+    rng = fields.rng_for(670)
+    U0h = fields.su3_hot_start(rng, 1, L, 0.5)
+    ah = rng.standard_normal((1, 4, *L, 3, 3)) + 1j * rng.standard_normal((1, 4, *L, 3, 3))
+    psi = fields.random_fermion(rng, (1, *L, 4, 3))
+    th = SU3WilsonNf2(lattice_shape=L, mass=0.05, fermion_bc=bc, dtype=np.complex128, cg_tol=1e-10, cg_maxiter=3000,
+                      solver_precision="f16", solver_recon=12, reliable_delta=0.01, use_graph=True, poll_every=16)
+    ora = WilsonOracle(L, 3, 0.05, fermion_bc=bc)
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        U0 = torch.from_numpy(U0h).cuda()
+        a = torch.from_numpy(ah).cuda()
+        H = 0.5 * (a - a.conj().transpose(-1, -2))
+        H = H - (torch.diagonal(H, dim1=-2, dim2=-1).sum(-1) / 3.0)[..., None, None] * torch.eye(
+            3, dtype=H.dtype, device="cuda")
+        pt = torch.from_numpy(psi).cuda()
+        for k in (1, 2):
+            Uk = torch.linalg.matrix_exp((0.01 * k) * H.reshape(-1, 3, 3)).reshape(H.shape) @ U0
+            rq = th.resident_gauge(Uk)
+            got = th.apply_D(rq, pt)
+            x = th.solve_normal(rq, pt)
+            st = th.last_solve_stats[0]
+            side.synchronize()
+            Ukh = Uk.cpu().numpy()
+            ref = ora.apply_D(Ukh, psi)
+            got = got.cpu().numpy() if hasattr(got, "cpu") else np.asarray(got)
+            err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+            assert err < 1e-13, (k, err)
+            assert st["status"] == "converged", (k, st)
+            del rq
+
+
 @pytest.mark.parametrize("prec", ["f32", "f16"])
 def test_mixed_solve_independent_of_block_length(prec):
     """A mixed solve decides its reliable updates on the device and alternates
