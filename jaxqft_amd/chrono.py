@@ -26,6 +26,15 @@ def _norm(x) -> float:
     return math.sqrt(max(_vdot(x, x).real, 0.0))
 
 
+def _vdots(pairs) -> List[complex]:
+    """Many inner products; on the device they are reduced first and copied
+    to the host together (one synchronization instead of one per product)."""
+    if pairs and isinstance(pairs[0][0], torch.Tensor):
+        vals = torch.stack([torch.vdot(a.reshape(-1), b.reshape(-1)) for a, b in pairs]).cpu().numpy()
+        return [complex(v) for v in vals]
+    return [_vdot(a, b) for a, b in pairs]
+
+
 def chrono_predict_from_history(history: List, mode: str, max_hist: int):
     """'last' (the newest solution) or 'poly': x_{k+1} ~ sum_{j<p} (-1)^j C(p, j+1) x_{k-j}
     over the newest p = min(max_hist, len(history)) solutions (su3_wilson_nf2.py:85-111)."""
@@ -59,11 +68,13 @@ def chrono_predict_mre(history_x: List, history_y: List, target, max_hist: int, 
     w = np.array([math.exp(-0.35 * float((n - 1) - i)) for i in range(n)])
     G = np.zeros((n, n), dtype=np.complex128)
     g = np.zeros(n, dtype=np.complex128)
+    idx = [(i, j) for i in range(n) for j in range(i, n)]
+    prods = _vdots([(ys[i], target) for i in range(n)] + [(ys[i], ys[j]) for i, j in idx])
     for i in range(n):
-        g[i] = w[i] * _vdot(ys[i], target)
-        for j in range(i, n):
-            G[i, j] = w[i] * w[j] * _vdot(ys[i], ys[j])
-            G[j, i] = np.conj(G[i, j])
+        g[i] = w[i] * prods[i]
+    for (i, j), v in zip(idx, prods[n:]):
+        G[i, j] = w[i] * w[j] * v
+        G[j, i] = np.conj(G[i, j])
     lam = float(reg) * max(1.0, abs(float(np.trace(G).real)) / max(1, n))
     Gr = G + lam * np.eye(n)
     try:
@@ -81,13 +92,14 @@ def chrono_predict_mre(history_x: List, history_y: List, target, max_hist: int, 
         x0 = x0 + complex(ci) * xi
     beta = 1.0 / max(1.0, cl1)
     x0 = beta * x0 + (1.0 - beta) * xs[-1]
-    n_last, n_x0 = _norm(xs[-1]), _norm(x0)
-    if not math.isfinite(n_x0) or (n_last > 0.0 and n_x0 > 5.0 * n_last):
-        return xs[-1]
     yc = ys[-1] * 0
     for ci, yi in zip(c, ys):
         yc = yc + complex(ci) * yi
-    r_mre, r_last = _norm(target - yc), _norm(target - ys[-1])
+    dm, dl = target - yc, target - ys[-1]
+    sq = _vdots([(xs[-1], xs[-1]), (x0, x0), (dm, dm), (dl, dl)])
+    n_last, n_x0, r_mre, r_last = (math.sqrt(max(v.real, 0.0)) for v in sq)
+    if not math.isfinite(n_x0) or (n_last > 0.0 and n_x0 > 5.0 * n_last):
+        return xs[-1]
     if not (math.isfinite(r_mre) and math.isfinite(r_last)) or r_mre >= 0.97 * r_last:
         return xs[-1]
     return x0
