@@ -82,8 +82,9 @@ class Lane:
         dt = nat.WL_C128 if req.rhs.dtype == torch.complex128 else nat.WL_C64
         slots = [i for i, o in enumerate(self.owner) if o is None][: req.nrhs]
         assert len(slots) == req.nrhs
+        caller = torch.cuda.current_stream()  # the stream that produced req.rhs
         with torch.cuda.stream(self.stream):
-            self.stream.wait_stream(torch.cuda.default_stream())
+            self.stream.wait_stream(caller)
             for j, slot in enumerate(slots):
                 nat.check(lib.wl_batch_admit(self.h, slot, c_void_p(req.rhs[j].data_ptr()), dt, float(req.mass)))
                 self.owner[slot] = (req, j)
@@ -302,8 +303,9 @@ def fp64_queue(requests: Sequence[Request], gauges: Dict, ctxs: Dict, tol: float
         ctx.set_mass(r.mass)
         r.x = torch.empty_like(r.rhs)
         r.stats = []
+        caller = torch.cuda.current_stream()
         with torch.cuda.stream(stream):
-            stream.wait_stream(torch.cuda.default_stream())
+            stream.wait_stream(caller)
             for j in range(r.nrhs):
                 b.import_(r.rhs[j:j + 1])
                 st = (nat.WlSolveStats * 1)()
