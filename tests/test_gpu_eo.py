@@ -327,3 +327,52 @@ def test_tune_eo_installs_a_valid_slab():
     ora = WilsonOracle(L, 3, 0.1, fermion_bc=bc)
     for j in range(4):
         assert _schur_normal_resid(ora, U, b[:, j], X[:, j]) <= 1e-10, j
+
+
+@pytest.mark.parametrize("L", [(16, 16, 16, 16), (32, 32, 32, 32)])
+def test_eo_properties_at_benchmark_sizes(L):
+    """At full size (no oracle: these hold exactly for the true operators):
+    K_eo and K^dag_oe, K_oe and K^dag_eo are adjoint pairs between the
+    parities; S_e^dag is the adjoint of S_e and S_o^dag of S_o; the Schur
+    complements are linear and vanish off their parity; and the even-odd
+    factorisation D = [[m0, K_eo], [K_oe, m0]] reproduces D on the full
+    lattice (D psi = m0 psi + K_eo psi_o + K_oe psi_e)."""
+    print(f"[gpu] eo properties L={L}", flush=True)
+    import torch
+
+    from jaxqft_amd import _native as nat
+    from jaxqft_amd import fields
+
+    rng = fields.rng_for(1590)
+    th = theory(L, 3, 0.02, (1, 1, 1, -1))
+    G = th.resident_gauge(fields.su3_hot_start(rng, 1, L, 0.6))
+    psi = torch.from_numpy(fields.random_fermion(rng, (1, *L, 4, 3))).cuda()
+    chi = torch.from_numpy(fields.random_fermion(rng, (1, *L, 4, 3))).cuda()
+    x = torch.arange(L[0]).view(-1, 1, 1, 1) + torch.arange(L[1]).view(1, -1, 1, 1) + \
+        torch.arange(L[2]).view(1, 1, -1, 1) + torch.arange(L[3]).view(1, 1, 1, -1)
+    even = ((x % 2) == 0).to(torch.complex128).cuda().view(1, *L, 1, 1)
+    odd = 1.0 - even
+
+    def dot(a, b):
+        return torch.vdot(a.flatten(), b.flatten())
+
+    def close(a, b, tol):
+        return float(abs(a - b) / max(abs(a), abs(b))) < tol
+
+    pe, po, ce, co = even * psi, odd * psi, even * chi, odd * chi
+    # <chi_o, K_oe psi_e> = <K^dag_eo chi_o, psi_e> and <chi_e, K_eo psi_o> = <K^dag_oe chi_e, psi_o>
+    assert close(dot(co, th._eo(G, pe, nat.WL_EO_K_OE)), dot(th._eo(G, co, nat.WL_EO_KDAG_EO), pe), 1e-12)
+    assert close(dot(ce, th._eo(G, po, nat.WL_EO_K_EO)), dot(th._eo(G, ce, nat.WL_EO_KDAG_OE), po), 1e-12)
+    for norm in (False, True):
+        se = th._eo(G, pe, nat.WL_EO_SCHUR_E, norm)
+        assert close(dot(ce, se), dot(th._eo(G, ce, nat.WL_EO_SCHUR_E_DAG, norm), pe), 1e-12)
+        so = th._eo(G, po, nat.WL_EO_SCHUR_O, norm)
+        assert close(dot(co, so), dot(th._eo(G, co, nat.WL_EO_SCHUR_O_DAG, norm), po), 1e-12)
+        assert float(torch.abs(se * odd).max()) == 0.0 and float(torch.abs(so * even).max()) == 0.0
+        a, b = 0.3 - 1.2j, 2.1 + 0.4j
+        lin = th._eo(G, a * pe + b * ce, nat.WL_EO_SCHUR_E, norm) - (a * se + b * th._eo(G, ce, nat.WL_EO_SCHUR_E, norm))
+        assert float(torch.linalg.norm(lin) / torch.linalg.norm(se)) < 1e-14
+    m0 = 0.02 + 4.0
+    full = m0 * psi + th._eo(G, po, nat.WL_EO_K_EO) + th._eo(G, pe, nat.WL_EO_K_OE)
+    d = th.apply_D(G, psi)
+    assert float(torch.linalg.norm(full - d) / torch.linalg.norm(d)) < 1e-14
