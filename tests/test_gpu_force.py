@@ -133,3 +133,34 @@ def test_force_is_action_derivative_16x4():
     assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
     # the force is traceless anti-Hermitian
     assert float((F + F.conj().transpose(-1, -2)).abs().max()) < 1e-13 * float(F.abs().max())
+
+
+def test_force_properties_at_32x4():
+    """At 32^4 (no oracle): every force matrix is traceless anti-Hermitian,
+    the two output layouts hold the same matrices, and the force is additive
+    in Y (it is real-bilinear in X and Y)."""
+    print("[gpu] force properties 32^4", flush=True)
+    import torch
+
+    from jaxqft_amd import fields
+
+    L = (32, 32, 32, 32)
+    rng = fields.rng_for(1830)
+    th = theory(L, 3, 0.02, (1, 1, 1, -1))
+    thl = theory(L, 3, 0.02, (1, 1, 1, -1), layout="BXYMIJ")
+    U = fields.su3_hot_start(rng, 1, L, 0.6)
+    G = th.resident_gauge(U)
+    Gl = thl.resident_gauge(np.ascontiguousarray(np.moveaxis(U, 1, -3)))
+    X = torch.from_numpy(fields.random_fermion(rng, (1, *L, 4, 3))).cuda()
+    Y1 = torch.from_numpy(fields.random_fermion(rng, (1, *L, 4, 3))).cuda()
+    Y2 = torch.from_numpy(fields.random_fermion(rng, (1, *L, 4, 3))).cuda()
+    F1 = th.pseudofermion_force_from_solution(G, X, Y1)
+    F2 = th.pseudofermion_force_from_solution(G, X, Y2)
+    F12 = th.pseudofermion_force_from_solution(G, X, Y1 + Y2)
+    scale = float(torch.abs(F1).max())
+    assert float(torch.abs(F1 + F1.conj().transpose(-1, -2)).max()) <= 1e-14 * scale
+    tr = torch.diagonal(F1, dim1=-2, dim2=-1).sum(-1)
+    assert float(torch.abs(tr).max()) <= 1e-14 * scale
+    assert float(torch.linalg.norm(F12 - F1 - F2) / torch.linalg.norm(F12)) < 1e-13
+    Fl = thl.pseudofermion_force_from_solution(Gl, X, Y1)
+    assert torch.equal(torch.movedim(Fl, -3, 1), F1)
