@@ -171,7 +171,8 @@ class WilsonNf2Theory:
 
     def _gauge(self, U) -> Tuple[nat.Context, Any]:
         if isinstance(U, ResidentGauge):
-            if U.theory is not self and U.ctx is not self._context():
+            ctx = self._context()  # also brings the context's mass up to date with self.mass
+            if U.theory is not self and U.ctx is not ctx:
                 raise ValueError("resident gauge belongs to another theory")
             return U.ctx, U.gauge
         self._check_links(U)

@@ -430,3 +430,27 @@ def test_plane_kernel_rejected_shapes_fall_back():
     ctx = nat.Context((4, 6, 4, 8), 3, 1, 0.1, 1.0, (1, 1, 1, -1))
     with pytest.raises(nat.NativeError, match="plane"):
         ctx.set_tuning(nat.WL_F16, False, 1, 0, True, 0, 0, 0, 0, 4)
+
+
+def test_mass_change_reaches_raw_and_resident_links_alike():
+    """After theory.mass changes, D on a resident gauge and D on the raw link
+    array both use the new mass (and match the oracle at that mass)."""
+    print("[gpu] mass change", flush=True)
+    from jaxqft_amd import fields
+    from jaxqft_amd.su3_wilson_nf2 import SU3WilsonNf2
+    from oracle.wilson_oracle import WilsonOracle
+
+    L = (4, 4, 4, 8)
+    bc = (1, 1, 1, -1)
+    rng = fields.rng_for(690)
+    U = fields.su3_hot_start(rng, 1, L, 0.5)
+    psi = fields.random_fermion(rng, (1, *L, 4, 3))
+    th = SU3WilsonNf2(lattice_shape=L, mass=0.1, fermion_bc=bc, dtype=np.complex128)
+    G = th.resident_gauge(U)
+    th.apply_D(G, psi)
+    th.mass = 0.3
+    ref = WilsonOracle(L, 3, 0.3, fermion_bc=bc).apply_D(U, psi)
+    a = np.asarray(th.apply_D(G, psi))
+    b = np.asarray(th.apply_D(U, psi))
+    assert np.linalg.norm(a - ref) / np.linalg.norm(ref) < 1e-13
+    assert np.array_equal(a, b)
