@@ -414,3 +414,37 @@ def test_mixed_multi_rhs_at_32x4():
     th.mass = 0.02
     xa = th.solve_normal_multi(G, b[:, 3:4], masses=[masses[3]])
     assert float(torch.linalg.norm(xa[:, 0] - X[:, 3]) / torch.linalg.norm(X[:, 3])) < 1e-8
+
+
+@pytest.mark.parametrize("prec", ["f64", "f16"])
+def test_complex64_inputs_solve_in_fp64(prec):
+    """complex64 links and sources (the reference's default dtype): the solve
+    runs on the FP64 promotion of the given values and returns complex64 of
+    the FP64 solution, so it equals the oracle's FP64 solve of the same
+    (complex64) system to complex64 rounding, for the full and the even-odd
+    normal equations."""
+    print(f"[gpu] complex64 solve {prec}", flush=True)
+    from jaxqft_amd import fields
+    from jaxqft_amd.su3_wilson_nf2 import SU3WilsonNf2
+    from oracle.wilson_oracle import WilsonOracle
+
+    L = (4, 4, 4, 8)
+    bc = (1, 1, 1, -1)
+    rng = fields.rng_for(700)
+    U = fields.su3_hot_start(rng, 1, L, 0.5).astype(np.complex64)
+    b = fields.random_fermion(rng, (1, *L, 4, 3)).astype(np.complex64)
+    kw = dict(solver_precision=prec, solver_recon=12) if prec != "f64" else dict(solver_precision="f64")
+    th = SU3WilsonNf2(lattice_shape=L, mass=0.1, fermion_bc=bc, dtype=np.complex64, cg_tol=1e-10, cg_maxiter=3000,
+                      **kw)
+    ora = WilsonOracle(L, 3, 0.1, fermion_bc=bc)
+    U64, b64 = U.astype(np.complex128), b.astype(np.complex128)
+    x = np.asarray(th.solve_normal(U, b))
+    assert x.dtype == np.complex64 and th.last_solve_stats[0]["status"] == "converged"
+    from oracle.wilson_oracle import cg
+
+    xr, _ = cg(lambda v: ora.apply_normal(U64, v), b64, tol=1e-12, maxiter=3000)
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-6
+    xe = np.asarray(th.solve_eop_even_normal(U, b))
+    assert xe.dtype == np.complex64 and th.last_solve_stats[0]["status"] == "converged"
+    xer = ora.solve_eop_even_normal(U64, b64, tol=1e-12, maxiter=3000)
+    assert np.linalg.norm(xe - xer) / np.linalg.norm(xer) < 1e-6
