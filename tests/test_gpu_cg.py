@@ -52,16 +52,21 @@ def test_config1_cg_4x4x4x4(policy):
     assert np.linalg.norm(x - z["x"]) / np.linalg.norm(z["x"]) < 1e-8
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32", "f16"])
 @pytest.mark.parametrize("nc", [3, 1])
-def test_batch_and_guess(nc):
-    print(f"[gpu] cg batch/guess nc={nc}", flush=True)
+def test_batch_and_guess(nc, prec):
+    """Two configurations per call, cold and from a guess near the solution,
+    SU(3) and U(1), plain FP64 and mixed precision: every column meets 1e-10
+    by the oracle and the guess cuts the iterations."""
+    print(f"[gpu] cg batch/guess nc={nc} {prec}", flush=True)
     from jaxqft_amd import fields
     from jaxqft_amd.su3_wilson_nf2 import SU3WilsonNf2
     from jaxqft_amd.u1_wilson_nf2 import U1WilsonNf2
 
     L = (4, 4, 2, 8)
     cls = SU3WilsonNf2 if nc == 3 else U1WilsonNf2
-    th = cls(lattice_shape=L, mass=0.05, batch_size=2, dtype=np.complex128, cg_tol=1e-10, cg_maxiter=4000)
+    th = cls(lattice_shape=L, mass=0.05, batch_size=2, dtype=np.complex128, cg_tol=1e-10, cg_maxiter=4000,
+             solver_precision=prec, reliable_delta=0.01)
     rng = fields.rng_for(500 + nc)
     U = fields.su3_hot_start(rng, 2, L, 0.5) if nc == 3 else fields.u1_hot_start(rng, 2, L, 0.5)
     b = fields.random_fermion(rng, th.fermion_shape())
