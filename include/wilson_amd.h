@@ -202,13 +202,19 @@ int wl_ctx_set_eo_slab(wl_ctx* ctx, int prec, int multi_rhs, int eo_slab);
  * residuals reads 12-real links (the same operator to rounding). */
 int wl_gauge_upload(wl_ctx* ctx, const void* U, int layout, int dtype, int where, wl_gauge** out);
 void wl_gauge_release(wl_gauge* g);
-/* prebuild the (prec, recon) encoding of a gauge field (otherwise built on first use) */
+/* prebuild the (prec, recon) encoding of a gauge field (otherwise built on
+ * first use).  A new encoding is built on `stream` and the call waits for
+ * it, so any stream may use it afterwards; it cannot be built while
+ * `stream` is capturing a graph (prepare it before the capture). */
 int wl_gauge_prepare(wl_gauge* g, int prec, int recon, void* stream);
 
 /* Operator on reference-layout arrays of ncol = batch * nrhs spinors. */
 int wl_apply(wl_ctx* ctx, wl_gauge* g, int op, const void* in, void* out, int nrhs, int dtype, void* stream);
 
-/* Resident internal-layout fields (structure of arrays, see DESIGN). */
+/* Resident internal-layout fields: per column, structure-of-arrays planes of
+ * 16-byte chunks over the padded site count (F64: double2 per complex; F32:
+ * float4 per two complex; F16: eight halves per four complex, then one plane
+ * of float per-site scales). */
 int wl_field_create(wl_ctx* ctx, int prec, int ncol, wl_field** out);
 void wl_field_destroy(wl_field* f);
 int wl_field_import(wl_field* f, const void* src, int dtype, void* stream);

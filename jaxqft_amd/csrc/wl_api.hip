@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -77,7 +78,7 @@ static int fail(int code, const char* fmt, ...) {
 // Device buffer.  Each allocation starts at a staggered offset (a multiple
 // of 9 cache lines) so that the same site of different fields does not map
 // to the same L2 set.
-static unsigned g_alloc_color = 0;
+static std::atomic<unsigned> g_alloc_color{0};
 struct DevBuf {
   void* p = nullptr;    // usable base
   void* raw = nullptr;  // allocation
@@ -468,6 +469,29 @@ void wl_gauge_release(wl_gauge* g) {
 
 }  // extern "C"
 
+// A new link encoding is built on the caller's stream but later read from
+// whatever stream uses the gauge next, so it is complete before it is handed
+// out.  Under graph capture the build would only be recorded, not run:
+// refused there.
+static int encoding_build_check(hipStream_t st, int prec, int recon) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
+    return fail(WL_ERR_BAD_ARG,
+                "link encoding (precision %d, recon %d) is built on first use and cannot be built under graph "
+                "capture: call wl_gauge_prepare before capturing", prec, recon);
+  (void)hipGetLastError();
+  return WL_OK;
+}
+
+static int encoding_finish(hipStream_t st, hipError_t e, DevBuf& b, const char* what) {
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) {
+    b.release();
+    return fail(WL_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+  }
+  return WL_OK;
+}
+
 // returns device pointer of the (prec, recon) encoding, building it if needed
 static int gauge_encoding(wl_gauge* g, int prec, int recon, hipStream_t st, const void** out) {
   wl_ctx* c = g->ctx;
@@ -483,15 +507,13 @@ static int gauge_encoding(wl_gauge* g, int prec, int recon, hipStream_t st, cons
     *out = it->second.p;
     return WL_OK;
   }
+  CHK(encoding_build_check(st, prec, recon));
   DevBuf b;
   size_t cb = cfg_bytes(c, prec, recon);
   CHK(b.ensure(cb * g->ncfg));
   size_t mcb = cfg_bytes(c, F64, 18);
   hipError_t e = convert_gauge(prec, c->geom.nc, recon, g->master.p, b.p, c->V, c->Vp, g->ncfg, mcb, mcb / 4, cb, cb / 4, st);
-  if (e != hipSuccess) {
-    b.release();
-    return fail(WL_ERR_HIP, "gauge convert: %s", hipGetErrorString(e));
-  }
+  CHK(encoding_finish(st, e, b, "gauge convert"));
   g->enc[key] = b;
   *out = b.p;
   return WL_OK;
@@ -510,16 +532,14 @@ static int gauge_encoding_eo(wl_gauge* g, int prec, int recon, hipStream_t st, c
     *out = it->second.p;
     return WL_OK;
   }
+  CHK(encoding_build_check(st, prec, recon));
   DevBuf b;
   const size_t cb = cfg_bytes_eo(c, prec, recon);
   CHK(b.ensure(cb * g->ncfg));
   const size_t mcb = cfg_bytes(c, F64, 18);
   hipError_t e = convert_gauge_eo(prec, c->geom.nc, recon, g->master.p, b.p, c->g, c->Vh, c->Vhp, g->ncfg, mcb, mcb / 4,
                                   cb, cb / 8, st);
-  if (e != hipSuccess) {
-    b.release();
-    return fail(WL_ERR_HIP, "gauge convert (even-odd): %s", hipGetErrorString(e));
-  }
+  CHK(encoding_finish(st, e, b, "gauge convert (even-odd)"));
   g->enc[key] = b;
   *out = b.p;
   return WL_OK;

@@ -454,3 +454,76 @@ def test_mass_change_reaches_raw_and_resident_links_alike():
     b = np.asarray(th.apply_D(U, psi))
     assert np.linalg.norm(a - ref) / np.linalg.norm(ref) < 1e-13
     assert np.array_equal(a, b)
+
+
+def test_link_encoding_built_on_one_stream_is_ready_on_another():
+    """A low-precision link encoding built on a busy stream (queued behind
+    long work) and used at once from an unrelated non-blocking stream: the
+    build waits for its own kernels, so the stencil on the other stream reads
+    finished links and matches the oracle (one racing the build would read
+    unconverted memory)."""
+    print("[gpu] link encoding across streams", flush=True)
+    import torch
+
+    from jaxqft_amd import _native as nat
+    from jaxqft_amd import fields
+    from oracle.wilson_oracle import WilsonOracle
+
+    L = (16, 16, 16, 16)
+    bc = (1, 1, 1, -1)
+    rng = fields.rng_for(910)
+    U = fields.su3_hot_start(rng, 1, L, 0.5)
+    psi = fields.random_fermion(rng, (1, *L, 4, 3))
+    ctx = nat.Context(L, 3, 1, 0.05, 1.0, bc)
+    g = nat.Gauge(ctx, torch.from_numpy(U).cuda(), nat.WL_LAYOUT_BMXYIJ)
+    ref = WilsonOracle(L, 3, 0.05, fermion_bc=bc).apply_D(U, psi)
+    busy, other = torch.cuda.Stream(), torch.cuda.Stream()
+    m = torch.randn(4096, 4096, device="cuda")
+    torch.cuda.synchronize()
+    with torch.cuda.stream(busy):
+        for _ in range(12):
+            m = m @ m
+            m = m / m.norm()
+        g.prepare(nat.WL_F16, 12)
+    with torch.cuda.stream(other):
+        fi, fo = nat.Field(ctx, nat.WL_F16, 1), nat.Field(ctx, nat.WL_F16, 1)
+        fi.import_(torch.from_numpy(psi).cuda())
+        nat.check(ctx._lib.wl_dslash(ctx.h, g.h, nat.WL_OP_D, 12, fi.h, fo.h, 1, nat.c_void_p(nat.current_stream())))
+        out = torch.empty((1, *L, 4, 3), dtype=torch.complex128, device="cuda")
+        fo.export(out)
+        other.synchronize()
+    assert rel(out.cpu().numpy(), ref) < 3e-3
+
+
+@pytest.mark.filterwarnings("ignore:The CUDA Graph is empty")
+def test_link_encoding_not_built_under_graph_capture():
+    """Building a link encoding inside a graph capture would only record its
+    kernels: the library refuses and says to prepare it first; a prepared
+    encoding is then used without a build."""
+    print("[gpu] link encoding under capture", flush=True)
+    import torch
+
+    from jaxqft_amd import _native as nat
+    from jaxqft_amd import fields
+
+    L = (4, 4, 4, 8)
+    rng = fields.rng_for(911)
+    U = fields.su3_hot_start(rng, 1, L, 0.5)
+    ctx = nat.Context(L, 3, 1, 0.05, 1.0, (1, 1, 1, -1))
+    g = nat.Gauge(ctx, torch.from_numpy(U).cuda(), nat.WL_LAYOUT_BMXYIJ)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    err = None
+    with torch.cuda.stream(s):
+        graph.capture_begin()
+        try:
+            g.prepare(nat.WL_F32, 12)
+        except nat.NativeError as e:
+            err = str(e)
+        finally:
+            graph.capture_end()
+    torch.cuda.current_stream().wait_stream(s)
+    assert err is not None and "wl_gauge_prepare" in err, err
+    g.prepare(nat.WL_F32, 12)
+    torch.cuda.synchronize()
