@@ -1150,8 +1150,13 @@ static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int n
   long long guard = 0;
   const long long guard_max = 64LL + 8LL * sp->maxiter;
   while (true) {
-    HIPCHK(hipMemcpyAsync(hc, c->cols.p, sizeof(ColState) * ncol, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    // (errors break out of the loop so that the captured graphs are freed)
+    hipError_t pe = hipMemcpyAsync(hc, c->cols.p, sizeof(ColState) * ncol, hipMemcpyDeviceToHost, st);
+    if (pe == hipSuccess) pe = hipStreamSynchronize(st);
+    if (pe != hipSuccess) {
+      result = fail(WL_ERR_HIP, "solver state read: %s", hipGetErrorString(pe));
+      break;
+    }
     bool all_done = true, any_update = false, any_active = false;
     double remaining = 0.0;
     bool have_rate = false;
