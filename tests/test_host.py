@@ -58,6 +58,51 @@ def test_null_handles_rejected():
     lib.wl_field_destroy(None)
 
 
+def test_every_entry_point_rejects_null_handles():
+    """Each entry point that takes a context, gauge, field or batch handle
+    returns WL_ERR_BAD_ARG with a message for a null one (checked before any
+    device call, so this runs without a GPU); the size queries return 0 and
+    the destroy calls accept null."""
+    lib = nat.load()
+    B = nat.WL_ERR_BAD_ARG
+    sp = nat.WlSolveParams()
+    t = nat.WlTuning()
+    dims = (ctypes.c_int * 4)(4, 4, 4, 4)
+    out = ctypes.c_void_p()
+    calls = {
+        "wl_ctx_set_mass": lambda: lib.wl_ctx_set_mass(None, 0.1),
+        "wl_ctx_set_tuning": lambda: lib.wl_ctx_set_tuning(None, 0, 1, ctypes.byref(t)),
+        "wl_ctx_get_tuning": lambda: lib.wl_ctx_get_tuning(None, 0, 1, ctypes.byref(t)),
+        "wl_ctx_set_eo_slab": lambda: lib.wl_ctx_set_eo_slab(None, 0, 1, 0),
+        "wl_gauge_upload": lambda: lib.wl_gauge_upload(None, None, 0, 0, 0, ctypes.byref(out)),
+        "wl_gauge_prepare": lambda: lib.wl_gauge_prepare(None, 0, 18, None),
+        "wl_field_create": lambda: lib.wl_field_create(None, 0, 1, ctypes.byref(out)),
+        "wl_field_export": lambda: lib.wl_field_export(None, None, 0, None),
+        "wl_dslash": lambda: lib.wl_dslash(None, None, 0, 18, None, None, 1, None),
+        "wl_cg_normal_field": lambda: lib.wl_cg_normal_field(None, None, None, None, 0, 1, ctypes.byref(sp), None, None),
+        "wl_eo_apply": lambda: lib.wl_eo_apply(None, None, 0, 1, None, None, 1, 0, None),
+        "wl_pf_force": lambda: lib.wl_pf_force(None, None, None, None, None, 0, 0, None),
+        "wl_cg_eo_normal": lambda: lib.wl_cg_eo_normal(None, None, None, None, None, 1, 0, ctypes.byref(sp), None,
+                                                       None),
+        "wl_lime_decode": lambda: lib.wl_lime_decode(None, 0, 8, dims, 4, 1, 12, 0, None, 0, None),
+        "wl_batch_create": lambda: lib.wl_batch_create(None, None, 1, ctypes.byref(sp), None, ctypes.byref(out)),
+        "wl_batch_admit": lambda: lib.wl_batch_admit(None, 0, None, 0, 0.1),
+        "wl_batch_step": lambda: lib.wl_batch_step(None),
+        "wl_batch_query": lambda: lib.wl_batch_query(None, None, None),
+        "wl_batch_retire": lambda: lib.wl_batch_retire(None, 0, None, 0),
+    }
+    for name, call in calls.items():
+        assert call() == B, name
+        assert lib.wl_last_error(), name
+    assert lib.wl_field_bytes(None) == 0
+    assert lib.wl_batch_bytes(None, 4, 0) == 0
+    lib.wl_batch_destroy(None)
+    # every declared entry point that takes a handle is covered above or in the test before
+    covered = set(calls) | {"wl_apply", "wl_field_import", "wl_cg_normal", "wl_ctx_destroy", "wl_gauge_release",
+                            "wl_field_destroy", "wl_field_bytes", "wl_batch_bytes", "wl_batch_destroy"}
+    assert set(nat.declared_symbols()) - covered == {"wl_version", "wl_last_error", "wl_ctx_create"}
+
+
 def test_theory_validation_and_shapes():
     from jaxqft_amd.su3_wilson_nf2 import SU3WilsonNf2
     from jaxqft_amd.u1_wilson_nf2 import U1WilsonNf2
