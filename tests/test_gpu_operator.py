@@ -527,3 +527,55 @@ def test_link_encoding_not_built_under_graph_capture():
     assert err is not None and "wl_gauge_prepare" in err, err
     g.prepare(nat.WL_F32, 12)
     torch.cuda.synchronize()
+
+
+def test_foreign_and_mismatched_handles_rejected():
+    """Handles from another context, fields of the wrong precision or column
+    count and in == out are refused with WL_ERR_BAD_ARG before any launch;
+    the context stays usable afterwards."""
+    print("[gpu] foreign handles", flush=True)
+    import torch
+    from ctypes import byref, c_void_p
+
+    from jaxqft_amd import _native as nat
+    from jaxqft_amd import fields
+
+    L = (4, 4, 4, 8)
+    bc = (1, 1, 1, -1)
+    rng = fields.rng_for(920)
+    U = torch.from_numpy(fields.su3_hot_start(rng, 1, L, 0.5)).cuda()
+    psi = torch.from_numpy(fields.random_fermion(rng, (1, *L, 4, 3))).cuda()
+    out = torch.empty_like(psi)
+    c1, c2 = nat.Context(L, 3, 1, 0.1, 1.0, bc), nat.Context(L, 3, 1, 0.1, 1.0, bc)
+    g1, g2 = nat.Gauge(c1, U, nat.WL_LAYOUT_BMXYIJ), nat.Gauge(c2, U, nat.WL_LAYOUT_BMXYIJ)
+    lib = c1._lib
+    B = nat.WL_ERR_BAD_ARG
+    h = c_void_p(nat.current_stream())
+    p = lambda t: c_void_p(t.data_ptr())
+    sp, keep = nat.solve_params(1e-10, 100)
+    st = (nat.WlSolveStats * 1)()
+    assert lib.wl_apply(c1.h, g2.h, nat.WL_OP_D, p(psi), p(out), 1, nat.WL_C128, h) == B
+    assert lib.wl_cg_normal(c1.h, g2.h, p(psi), p(out), None, 1, nat.WL_C128, byref(sp), st, h) == B
+    assert lib.wl_eo_apply(c1.h, g2.h, nat.WL_EO_K_OE, 1, p(psi), p(out), 1, nat.WL_C128, h) == B
+    assert lib.wl_pf_force(c1.h, g2.h, p(psi), p(psi), p(out), nat.WL_LAYOUT_BMXYIJ, nat.WL_C128, h) == B
+    bt = c_void_p()
+    assert lib.wl_batch_create(c1.h, g2.h, 2, byref(sp), h, byref(bt)) == B
+    f64a, f64b = nat.Field(c1, nat.WL_F64, 1), nat.Field(c1, nat.WL_F64, 1)
+    f32 = nat.Field(c1, nat.WL_F32, 1)
+    f64x2 = nat.Field(c1, nat.WL_F64, 2)
+    foreign = nat.Field(c2, nat.WL_F64, 1)
+    assert lib.wl_dslash(c1.h, g1.h, nat.WL_OP_D, 18, f64a.h, f32.h, 1, h) == B      # precision
+    assert lib.wl_dslash(c1.h, g1.h, nat.WL_OP_D, 18, f64a.h, f64x2.h, 1, h) == B    # columns
+    assert lib.wl_dslash(c1.h, g1.h, nat.WL_OP_D, 18, f64a.h, f64a.h, 1, h) == B     # in == out
+    assert lib.wl_dslash(c1.h, g1.h, nat.WL_OP_D, 18, f64a.h, foreign.h, 1, h) == B  # foreign field
+    assert lib.wl_dslash(c1.h, g1.h, nat.WL_OP_D, 18, f64x2.h, f64x2.h, 2, h) == B
+    assert lib.wl_dslash(c1.h, g1.h, nat.WL_OP_D, 18, f64a.h, f64b.h, 2, h) == B     # batch * nrhs
+    assert lib.wl_dslash(c1.h, g1.h, 99, 18, f64a.h, f64b.h, 1, h) == B              # op
+    assert lib.wl_cg_normal_field(c1.h, g1.h, f32.h, f64a.h, 0, 1, byref(sp), st, h) == B
+    # the context still works
+    nat.check(lib.wl_apply(c1.h, g1.h, nat.WL_OP_D, p(psi), p(out), 1, nat.WL_C128, h))
+    ref = torch.empty_like(psi)
+    nat.check(lib.wl_apply(c2.h, g2.h, nat.WL_OP_D, p(psi), p(ref), 1, nat.WL_C128, h))
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref)
+    del keep
