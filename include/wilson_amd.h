@@ -283,7 +283,10 @@ int wl_lime_decode(const void* payload, size_t nbytes, int word_bytes, const int
  * stream and reports, per column, state (0 free, 1 running, 2 finished) and
  * stats; wl_batch_retire writes a column's solution x (device, reference
  * layout) and frees the column.  Columns are independent: a right-hand side
- * gives the same solution alone or batched. */
+ * gives the same solution alone or batched.  Admitting into an occupied slot,
+ * retiring a free one or naming a slot outside [0, capacity) returns
+ * WL_ERR_BAD_ARG; retiring a running column returns its current iterate.
+ * The batch keeps using `g`: release the gauge only after wl_batch_destroy. */
 typedef struct wl_batch wl_batch;
 size_t wl_batch_bytes(wl_ctx* ctx, int capacity, int inner_prec);
 int wl_batch_create(wl_ctx* ctx, wl_gauge* g, int capacity, const wl_solve_params* sp, void* stream, wl_batch** out);
