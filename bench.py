@@ -357,7 +357,10 @@ def main():
                           stream=stream)
         eo["inner_hop_eo_slab"] = eo_tune["eo_slab"]
         eo["inner_hop_tuning_ms"] = eo_tune["ms"]
-        log(f"even-odd inner hop: eo_slab {eo_tune['eo_slab']} {eo_tune['ms']}")
+        eo["update_hop_eo_slab"] = eo_tune.get("update_eo_slab")
+        eo["update_hop_tuning_ms"] = eo_tune.get("update_ms")
+        log(f"even-odd inner hop: eo_slab {eo_tune['eo_slab']} {eo_tune['ms']}; "
+            f"FP64 update hop: eo_slab {eo_tune.get('update_eo_slab')} {eo_tune.get('update_ms')}")
         for name, bt, xt, nr, prec, recon, n in (("fp64_single_rhs", b1_t, xe1, 1, "f64", 18, args.fp64_steps),
                                                 ("mixed_single_rhs", b1_t, xe1, 1, args.inner, args.recon,
                                                  args.fp64_steps),

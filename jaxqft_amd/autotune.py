@@ -119,7 +119,8 @@ def set_eo_slab(ctx: nat.Context, prec: int, multi: bool, eo_slab: int) -> None:
 
 def tune_eo(ctx: nat.Context, gauge: nat.Gauge, b: torch.Tensor, x: torch.Tensor, nrhs: int, prec: int, recon: int,
             slabs: Iterable[int] = (0, 2, 4, 8), iters: int = 24, delta: float = 0.01,
-            stream: Optional[torch.cuda.Stream] = None) -> Dict[str, object]:
+            stream: Optional[torch.cuda.Stream] = None,
+            update_slabs: Iterable[int] = (0, 2, 4, 8)) -> Dict[str, object]:
     """Pick the x1 slab order of the even-odd hop for inner precision prec at
     nrhs right-hand sides per configuration and install it.  Each candidate
     (flat, or slab extents dividing L1) is timed on a solve of the even-odd
@@ -128,7 +129,11 @@ def tune_eo(ctx: nat.Context, gauge: nat.Gauge, b: torch.Tensor, x: torch.Tensor
     arrays; the fused hop, Schur combine and CG-step launches it times are
     the ones a full solve repeats; the solves capture HIP graphs, so they run
     on `stream` or, by default, on a side stream ordered after the current
-    one.  Returns {"eo_slab", "ms": {slab: ms}}."""
+    one.  For a mixed-precision prec the slab of the FP64 hops (the reliable
+    updates, and FP64 even-odd operators at this nrhs) is then picked the
+    same way among `update_slabs`, with the inner slab installed (an empty
+    `update_slabs` leaves it alone).  Returns {"eo_slab", "ms": {slab: ms}}
+    and, when the FP64 slab was tuned, {"update_eo_slab", "update_ms"}."""
     if stream is None:
         stream = torch.cuda.Stream()
         stream.wait_stream(torch.cuda.current_stream())
@@ -147,28 +152,36 @@ def tune_eo(ctx: nat.Context, gauge: nat.Gauge, b: torch.Tensor, x: torch.Tensor
         nat.check(ctx._lib.wl_cg_eo_normal(ctx.h, gauge.h, c_void_p(b.data_ptr()), c_void_p(x.data_ptr()), c_void_p(0),
                                            nrhs, nat.WL_C128, byref(sp), st, h))
 
-    times = {}
-    for s in slabs:
-        if s and L1 % s:
-            continue
-        set_eo_slab(ctx, prec, multi, s)
-        run()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        best = None
-        for _ in range(2):
-            e0.record(stream)
+    def sweep(p, cands):
+        times = {}
+        for s in cands:
+            if s and L1 % s:
+                continue
+            set_eo_slab(ctx, p, multi, s)
             run()
-            e1.record(stream)
-            e1.synchronize()
-            t = e0.elapsed_time(e1)
-            best = t if best is None else min(best, t)
-        times[s] = best
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            best = None
+            for _ in range(2):
+                e0.record(stream)
+                run()
+                e1.record(stream)
+                e1.synchronize()
+                t = e0.elapsed_time(e1)
+                best = t if best is None else min(best, t)
+            times[s] = best
+        pick = min(times, key=times.get)
+        set_eo_slab(ctx, p, multi, pick)
+        return pick, {str(s): round(t, 3) for s, t in times.items()}
+
+    pick, ms = sweep(prec, slabs)
+    out = {"eo_slab": pick, "ms": ms}
+    update_slabs = tuple(update_slabs)
+    if prec != nat.WL_F64 and update_slabs:
+        out["update_eo_slab"], out["update_ms"] = sweep(nat.WL_F64, update_slabs)
     del keep
     torch.cuda.current_stream().wait_stream(stream)
-    pick = min(times, key=times.get)
-    set_eo_slab(ctx, prec, multi, pick)
-    return {"eo_slab": pick, "ms": {str(s): round(t, 3) for s, t in times.items()}}
+    return out
 
 
 # 4D workgroup box presets (log2 extents per direction), as in the library

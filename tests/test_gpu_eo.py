@@ -312,6 +312,7 @@ def test_tune_eo_installs_a_valid_slab():
     th = theory(L, 3, 0.1, bc, cg_tol=1e-10, cg_maxiter=3000, solver_precision="f16", solver_recon=12)
     ctx = th._context()
     before = ctx.get_tuning(nat.WL_F16, 4)
+    before64 = ctx.get_tuning(nat.WL_F64, 4)
     g = nat.Gauge(ctx, torch.from_numpy(U).cuda(), nat.WL_LAYOUT_BMXYIJ)
     bt = torch.from_numpy(b.reshape(4, *L, 4, 3)).cuda()
     xt = torch.empty_like(bt)
@@ -323,6 +324,13 @@ def test_tune_eo_installs_a_valid_slab():
     after = ctx.get_tuning(nat.WL_F16, 4)
     assert after["eo_slab"] == t["eo_slab"]
     assert {k: v for k, v in after.items() if k != "eo_slab"} == {k: v for k, v in before.items() if k != "eo_slab"}
+    # the FP64 hops of the reliable updates: tuned the same way, installed for FP64 at this nrhs
+    assert set(t["update_ms"]) == {"0", "2", "4", "8"} and t["update_eo_slab"] in (0, 2, 4, 8)
+    after64 = ctx.get_tuning(nat.WL_F64, 4)
+    assert after64["eo_slab"] == t["update_eo_slab"]
+    assert {k: v for k, v in after64.items() if k != "eo_slab"} == {k: v for k, v in before64.items() if k != "eo_slab"}
+    assert "update_eo_slab" not in tune_eo(ctx, g, bt, xt, 4, nat.WL_F16, 12, slabs=(0,), iters=6, update_slabs=())
+    assert ctx.get_tuning(nat.WL_F64, 4)["eo_slab"] == t["update_eo_slab"]
     X = np.asarray(th.solve_eop_even_normal_multi(U, b, masses=[0.1] * 4))
     ora = WilsonOracle(L, 3, 0.1, fermion_bc=bc)
     for j in range(4):
