@@ -361,6 +361,11 @@ def main():
         eo["update_hop_tuning_ms"] = eo_tune.get("update_ms")
         log(f"even-odd inner hop: eo_slab {eo_tune['eo_slab']} {eo_tune['ms']}; "
             f"FP64 update hop: eo_slab {eo_tune.get('update_eo_slab')} {eo_tune.get('update_ms')}")
+        # the single-RHS launches have settings of their own: tuned the same way on one column
+        eo_tune1 = tune_eo(ctx, gauge, b1_t, xe1, 1, nat.PREC[args.inner], args.recon, delta=args.eo_delta,
+                           stream=stream)
+        eo["single_rhs_tuning"] = eo_tune1
+        log(f"even-odd single-RHS: {eo_tune1}")
         for name, bt, xt, nr, prec, recon, n in (("fp64_single_rhs", b1_t, xe1, 1, "f64", 18, args.fp64_steps),
                                                 ("mixed_single_rhs", b1_t, xe1, 1, args.inner, args.recon,
                                                  args.fp64_steps),
