@@ -69,7 +69,15 @@ def parse():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving scenario")
     ap.add_argument("--no-propagator", action="store_true",
                     help="skip the point-source propagator and chronological-guess measurements")
-    ap.add_argument("--serving-requests", type=int, default=60)
+    ap.add_argument("--serving-requests", type=int, default=60, help="requests of the 200/s overload point")
+    ap.add_argument("--serving-sweep-requests", type=int, default=200,
+                    help="requests per point of the sustainable-load sweep (0 skips the sweep)")
+    ap.add_argument("--serving-sweep-loads", type=str, default="0.25,0.5,0.75",
+                    help="arrival rates of the sweep as fractions of the request rate served at overload")
+    ap.add_argument("--detail-out", type=str, default="gpurun_out/bench_detail.json",
+                    help="file for the full record (every variant, tuning table and sub-benchmark); the last "
+                         "stdout line is the compact record")
+    ap.add_argument("--no-stream", action="store_true", help="skip the HBM streaming probes")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lime", action="store_true", help="skip the SciDAC payload decode")
@@ -335,6 +343,16 @@ def main():
         mixk_s, stmk = B.timed_solves(args.steps, ctx, gauge, bk, xk, k, args.inner, args.recon)
         log(f"mixed {k}-RHS solve {mixk_s*1e3:.1f} ms {summarize(stmk)}")
 
+        # the same solve once more with kernel timing: host-driven iterations
+        # with HIP events on the bench stream around each iteration's launches
+        ctx.set_profiling(True)
+        xp = nat.Field(ctx, nat.WL_F64, k)
+        stp = B.solve(ctx, gauge, bk, xp, k, args.inner, args.recon)
+        prof = ctx.profile()
+        ctx.set_profiling(False)
+        del xp
+        log(f"profiled {k}-RHS solve {summarize(stp)}: {prof}")
+
         # error of the mixed solutions against the plain FP64 solve
         xa = torch.empty_like(b1_t)
         xb = torch.empty_like(b1_t)
@@ -421,37 +439,19 @@ def main():
         for entry in list(dslash.values()) + list(dslash_k.values()):
             entry["frac_of_device_copy"] = round(entry["GB/s"] / copy["GB/s"], 4)
 
-    out = {
+    cg_step = cg_step_roofline(args, V, k, prof, stp)
+    detail = {
         "metric": METRIC,
         "value": round(k / mixk_s, 4),
         "unit": "solves/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
         "ms_per_step": round(mixk_s * 1e3, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": f"{args.inner}-storage/f32-arith inner, f64 reliable updates",
-        "data": "synthetic (seeded PCG64 SU(3) links eps=%g, complex Gaussian sources)" % args.eps,
-        "config": {
-            "workload": "SU(3) Wilson D^dag D CG, %s, m=%g, bc=(1,1,1,-1), %d RHS per solve, FP64 true residual %g"
-                        % ("x".join(map(str, L)), args.mass, k, args.tol),
-            "lattice": list(L), "nrhs": k, "inner": args.inner, "recon": args.recon, "delta": args.delta,
-            "accumulator": args.accum,
-            "graph": bool(args.graph),
-        },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": f"stencil {args.inner} r{args.recon}, {k} RHS, tuning {dom_tuning} (links counted once)",
-            "achieved": round(dom_gbs, 1),
-            "peak": PEAK_HBM_GBS,
-            "unit": "GB/s",
-            "frac": round(dom_gbs / PEAK_HBM_GBS, 4),
-            "traffic": dom_traffic,
-            "traffic_source": dom_traffic_src,
-            "kernel_us": round(dom_us, 2),
-            "algorithmic_bytes": dom_bytes,
+        "config": {"lattice": list(L), "nrhs": k, "inner": args.inner, "recon": args.recon, "delta": args.delta,
+                   "accumulator": args.accum, "graph": bool(args.graph)},
+        "cg_step_kernels": cg_step,
+        "dslash_launch": {
+            "kernel": f"stencil {args.inner} r{args.recon}, {k} RHS, store epilogue, tuning {dom_tuning}",
+            "us": round(dom_us, 2), "GB/s": round(dom_gbs, 1), "frac_hbm_peak": round(dom_gbs / PEAK_HBM_GBS, 4),
+            "algorithmic_bytes": dom_bytes, "traffic": dom_traffic, "traffic_source": dom_traffic_src,
             "frac_of_device_copy": round(dom_gbs / copy["GB/s"], 4),
         },
         "device_copy": copy,
@@ -476,20 +476,170 @@ def main():
         "dslash_single_rhs": dslash,
         "dslash_multi_rhs": dslash_k,
     }
+    if not args.no_stream:
+        detail["device_stream"] = device_stream(B)
     if not args.no_config2:
-        out["fp64_16x16x16x16"] = config2(B, args, not args.no_cpu)
+        detail["fp64_16x16x16x16"] = config2(B, args, not args.no_cpu)
     if not args.no_config4:
-        out["multi_rhs_24x24x24x48"] = config4(B, args)
+        detail["multi_rhs_24x24x24x48"] = config4(B, args)
     if not args.no_propagator:
-        out.update(propagator_chrono(B, args, U, L))
+        detail.update(propagator_chrono(B, args, U, L))
     if not args.no_serving:
-        out["serving"] = serving(B, args)
+        detail["serving"] = serving(B, args)
     if not args.no_lime:
-        out["lime_decode"] = lime_decode(B, args, L, not args.no_cpu)
+        detail["lime_decode"] = lime_decode(B, args, L, not args.no_cpu)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(U, b_host[:1], L, args, st64[0]["iters"])
-    out["bench_wall_s"] = round(time.perf_counter() - t_start, 1)
-    print(json.dumps(out))
+        detail["cpu_baseline"] = cpu_baseline(U, b_host[:1], L, args, st64[0]["iters"])
+    detail["bench_wall_s"] = round(time.perf_counter() - t_start, 1)
+    try:
+        outp = Path(args.detail_out)
+        outp.parent.mkdir(parents=True, exist_ok=True)
+        outp.write_text(json.dumps(detail, indent=1))
+        log(f"full record written to {outp}")
+    except OSError as e:
+        log(f"could not write the full record: {e}")
+    line = compact_record(args, detail, L, k)
+    text = json.dumps(line, separators=(",", ":"))
+    log(f"compact record: {len(text)} bytes")
+    print(text, flush=True)
+
+
+def cg_step_roofline(args, V, k, prof, stats):
+    """Algorithmic bytes and kernel times of the two stencil launches of the
+    headline CG iteration, from the profiled solve (HIP events on the bench
+    stream around every launch of its iterations).
+
+    Per site, links counted once (12-real FP16: 96 B) and, per right-hand
+    side, each spinor field moved once in the storage precision (FP16: 48 B
+    of halves + 4 B scale):
+      D with |t|^2 (t = D p):   links + k (p read + t written)
+      D^dag with the CG step:   links + k (t read + r read + r written)
+                                plus, every second iteration (pairwise y),
+                                k (p read + p_prev read + y read + y written)
+    The D^dag launch is reported over one pair of iterations (both parities
+    timed), which is how its average over a solve is made up."""
+    spin = dslash_bytes_per_site(args.inner, args.recon, 1) - dslash_bytes_per_site(args.inner, args.recon, 0)
+    spin1 = spin // 2  # one field of the inner precision
+    links = dslash_bytes_per_site(args.inner, args.recon, 0)
+    acc = {"f16": "f16", "f32": "f32", "f64": "f64", "auto": "f32"}[args.accum]
+    spin_y = (dslash_bytes_per_site(acc, 18, 1) - dslash_bytes_per_site(acc, 18, 0)) // 2
+    op_b = (links + k * 2 * spin1) * V
+    dag_b0 = (links + k * 3 * spin1) * V
+    dag_b1 = (links + k * (5 * spin1 + 2 * spin_y)) * V
+    t_op = prof["us_op"]
+    t_dag = prof["us_op_dag"]
+    n = prof["iterations"]
+    if min(n) == 0:
+        return {"error": "no timed iterations", "profile": prof}
+    # which parity added to y: the slower launch of the pair
+    y_par = 1 if t_dag[1] >= t_dag[0] else 0
+    pair_b = dag_b0 + dag_b1
+    pair_us = t_dag[0] + t_dag[1]
+    op_us = (t_op[0] * n[0] + t_op[1] * n[1]) / (n[0] + n[1])
+    dag_gbs = pair_b / (pair_us * 1e-6) / 1e9
+    op_gbs = op_b / (op_us * 1e-6) / 1e9
+    return {
+        "profiled_solve": summarize(stats),
+        "timed_iterations": n,
+        "dag_cg_step": {
+            "kernel": f"D^dag {args.inner} r{args.recon}, {k} RHS, fused CG step (y in {acc}, pairwise)",
+            "us_mean": round(pair_us / 2, 2), "us_by_parity": [round(t, 2) for t in t_dag], "y_parity": y_par,
+            "bytes_per_site_without_y": dag_b0 // V, "bytes_per_site_with_y": dag_b1 // V,
+            "GB/s": round(dag_gbs, 1), "frac_hbm_peak": round(dag_gbs / PEAK_HBM_GBS, 4),
+        },
+        "d_norm": {
+            "kernel": f"D {args.inner} r{args.recon}, {k} RHS, |t|^2 partials",
+            "us_mean": round(op_us, 2), "bytes_per_site": op_b // V,
+            "GB/s": round(op_gbs, 1), "frac_hbm_peak": round(op_gbs / PEAK_HBM_GBS, 4),
+        },
+        "xpay_us": round(prof["us_xpay"], 2),
+    }
+
+
+def compact_record(args, d, L, k):
+    """The driver-facing record: the headline, the roofline of the launch
+    that dominates the timed step (D^dag with the fused CG step), the
+    standalone Dslash launch beside it, the streaming ceiling, the CPU
+    baseline and one summary figure per other configuration."""
+    cg = d["cg_step_kernels"]
+    dag = cg.get("dag_cg_step", {})
+    dn = cg.get("d_norm", {})
+    dl = d["dslash_launch"]
+    line = {
+        "metric": METRIC,
+        "value": d["value"],
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": d["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": f"{args.inner}-storage/f32-arith inner, f64 reliable updates",
+        "data": "synthetic (seeded PCG64 SU(3) links eps=%g, complex Gaussian sources)" % args.eps,
+        "config": {
+            "workload": "SU(3) Wilson D^dag D CG, %s, m=%g, bc=(1,1,1,-1), %d RHS per solve, %s inner (recon %d, "
+                        "per-site FP32 scales, FP32 arithmetic), FP64 reliable updates delta=%g, FP64 true "
+                        "residual %g" % ("x".join(map(str, L)), args.mass, k, args.inner, args.recon, args.delta,
+                                         args.tol),
+            "lattice": list(L), "nrhs": k,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dag.get("kernel"),
+            "achieved": dag.get("GB/s"),
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": dag.get("frac_hbm_peak"),
+            "traffic": None,
+            "kernel_us": dag.get("us_mean"),
+            "bytes_per_site": [dag.get("bytes_per_site_without_y"), dag.get("bytes_per_site_with_y")],
+        },
+        "roofline_d_norm": {"achieved": dn.get("GB/s"), "frac": dn.get("frac_hbm_peak"), "kernel_us": dn.get("us_mean"),
+                            "bytes_per_site": dn.get("bytes_per_site")},
+        "roofline_dslash": {"kernel": "D %s r%d, %d RHS (store)" % (args.inner, args.recon, k), "achieved": dl["GB/s"],
+                            "frac": dl["frac_hbm_peak"], "kernel_us": dl["us"], "traffic": dl["traffic"],
+                            "algorithmic_bytes": dl["algorithmic_bytes"]},
+        "device_copy_GB/s": d["device_copy"]["GB/s"],
+    }
+    if "device_stream" in d:
+        line["device_stream"] = {kk: {"GB/s": v["GB/s"], "frac": v["frac_hbm_peak"], "variant": v["variant"]}
+                                 for kk, v in d["device_stream"]["best"].items()}
+    sv = d["solve"]
+    line["fp64_single_rhs"] = {"solves_per_s": sv["fp64_single_rhs"]["solves_per_s"],
+                               "iters": sv["fp64_single_rhs"]["iters_max"]}
+    line["speedup_vs_fp64_single"] = sv["speedup_mixed_multi_vs_fp64_single"]
+    line["true_resid_max"] = sv["mixed_multi_rhs"]["true_resid_max"]
+    line["rel_error_vs_fp64"] = sv["mixed_multi_rhs"]["rel_error_vs_fp64_col0"]
+    eo = d.get("even_odd", {}).get("mixed_multi_rhs")
+    if eo:
+        line["even_odd_mixed_12rhs_solves_per_s"] = eo["solves_per_s"]
+    c3 = d.get("config3_single_rhs_delta0.1")
+    if c3:
+        line["config3"] = {kk: {"ms": round(v["s_per_solve"] * 1e3, 1), "iters": v["iters_max"],
+                                "err_vs_fp64": float("%.2e" % v["rel_error_vs_fp64"])} for kk, v in c3.items()}
+    c2 = d.get("fp64_16x16x16x16")
+    if c2:
+        line["config2_16^4"] = {"dslash_GB/s": c2["dslash_f64"]["GB/s"], "GFLOP/s": c2["dslash_f64"]["GFLOP/s"],
+                                "solves_per_s": c2["fp64_solve"]["solves_per_s"], "cache_resident": True}
+    c4 = d.get("multi_rhs_24x24x24x48")
+    if c4:
+        line["config4_24^3x48"] = {kk: v["solves_per_s"] for kk, v in c4.items()}
+    srv = d.get("serving")
+    if srv:
+        cb = srv["continuous_batching"]
+        line["serving"] = {"overload_200/s": {"rhs_per_s": cb["rhs_per_s"], "p50_ms": cb["latency_p50_ms"],
+                                              "p99_ms": cb["latency_p99_ms"]},
+                           "fp64_queue_rhs_per_s": srv["fp64_unbatched_queue"]["rhs_per_s"],
+                           "sweep": [{"req/s": p["offered_req_per_s"], "p50_ms": p["latency_p50_ms"],
+                                      "p99_ms": p["latency_p99_ms"], "rhs_per_s": p["rhs_per_s"]}
+                                     for p in srv.get("sustainable_load_sweep", [])]}
+    if "cpu_baseline" in d:
+        cbl = d["cpu_baseline"]
+        line["cpu_baseline"] = {kk: cbl[kk] for kk in ("value", "unit", "cores", "kind", "sample")}
+    line["detail"] = args.detail_out
+    return line
 
 
 def config2(B, args, with_cpu):
@@ -711,8 +861,29 @@ def serving(B, args):
     res64 = fp64_queue(base, srv.backend.gauges, srv.backend.ctxs, tol=args.tol, realtime=True)
     log(f"serving unbatched fp64 queue: {res64}")
     diff = max(float(torch.linalg.norm(a.x - b.x) / torch.linalg.norm(b.x)) for a, b in zip(reqs, base))
+    del reqs, base
+    torch.cuda.synchronize()
+    # sustainable loads: Poisson arrivals at fractions of the request rate
+    # served at overload, so that p50 / p99 are service latencies and not
+    # queue growth; right-hand sides are made at admission and dropped when
+    # a request finishes (stats kept)
+    sweep = []
+    if args.serving_sweep_requests > 0:
+        cap = res["requests_per_s"]
+        for frac in (float(v) for v in args.serving_sweep_loads.split(",")):
+            rate = frac * cap
+            rq = poisson_requests(rate, args.serving_sweep_requests, lattices, 4,
+                                  seed=fields.SEED_BASE + 50 + int(100 * frac), lazy=True)
+            out = srv.run(rq, realtime=True, release=True)
+            out["offered_req_per_s"] = round(rate, 3)
+            out["offered_fraction_of_overload_rate"] = frac
+            sweep.append(out)
+            log(f"serving at {rate:.2f} req/s ({frac} of the overload rate): {out}")
+            del rq
     return {
-        "workload": f"{len(reqs)} Poisson requests at 200/s over 8^4,16^4,24^3x48,32^4 (4 fields each, eps=0.5), "
+        "sustainable_load_sweep": sweep,
+        "workload": f"{args.serving_requests} Poisson requests at 200/s (overload: arrivals outpace service, "
+                    "so latencies include queue growth) over 8^4,16^4,24^3x48,32^4 (4 fields each, eps=0.5), "
                     "m~U[0.02,0.2], k~U{1..12}",
         "hbm_budget_GB": round(budget / 1e9, 1),
         "continuous_batching": res,
@@ -794,6 +965,53 @@ def lime_decode(B, args, L, with_cpu):
         del lat, Uh
     del U, pay, Uc, out, host, dev
     torch.cuda.synchronize()
+    return res
+
+
+def device_stream(B, mib: int = 1024, reps: int = 10):
+    """Hand-written HBM streaming kernels (wl_stream_probe), 1 GiB per
+    buffer: read-only, write-only and copy, 16 B per lane, plain or
+    non-temporal accesses, 1, 2 or 4 accesses in flight per lane, one pass
+    over the buffer or a 2048 / 8192-workgroup grid-stride loop.  HIP events
+    on the bench stream around `reps` launches; bytes counted are those read
+    (read), written (write) or read + written (copy).  The fastest variant of
+    each kind is the measured ceiling the stencils are read against."""
+    import torch
+
+    nat = B.nat
+    stream = B.stream
+    nbytes = mib << 20
+    res = {"bytes_per_buffer": nbytes, "variants": {}, "best": {}}
+    with torch.cuda.stream(stream):
+        src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda")
+        src.fill_(1.0)
+        dst = torch.empty_like(src)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        for kind, name, moved in ((nat.WL_STREAM_READ, "read", nbytes), (nat.WL_STREAM_WRITE, "write", nbytes),
+                                  (nat.WL_STREAM_COPY, "copy", 2 * nbytes)):
+            for ntm in (0, 1):
+                for unroll in (1, 2, 4):
+                    for blocks in (0, 2048, 8192):
+                        def run():
+                            nat.check(B.lib.wl_stream_probe(kind, ntm, unroll, c_void_p(src.data_ptr()),
+                                                            c_void_p(dst.data_ptr()), nbytes, blocks, B.h))
+                        run()
+                        run()
+                        e0.record(stream)
+                        for _ in range(reps):
+                            run()
+                        e1.record(stream)
+                        e1.synchronize()
+                        us = e0.elapsed_time(e1) * 1e3 / reps
+                        gbs = moved / (us * 1e-6) / 1e9
+                        tag = f"{name}/{'nt' if ntm else 'plain'}/u{unroll}/{'pass' if blocks == 0 else f'g{blocks}'}"
+                        res["variants"][tag] = round(gbs, 1)
+                        if name not in res["best"] or gbs > res["best"][name]["GB/s"]:
+                            res["best"][name] = {"GB/s": round(gbs, 1), "frac_hbm_peak": round(gbs / PEAK_HBM_GBS, 4),
+                                                 "us": round(us, 2), "variant": tag}
+        del src, dst
+    torch.cuda.synchronize()
+    log("device stream: " + ", ".join(f"{k} {v['GB/s']} GB/s ({v['variant']})" for k, v in res["best"].items()))
     return res
 
 

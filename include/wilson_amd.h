@@ -29,13 +29,21 @@
  *   - All array pointers are device pointers unless stated; buffers are
  *     caller-owned, gauge and field handles are library-owned.
  *   - Calls are ordered on the given HIP stream (void* = hipStream_t, NULL =
- *     default stream).  One context serializes its own calls; separate
- *     contexts may be used concurrently.
+ *     default stream).  Calls on one context share its device workspace: the
+ *     host side holds a per-context lock for the duration of a call, and a
+ *     call issued on a stream other than the previous call's first makes its
+ *     stream wait for the end of that call (an event recorded at the end of
+ *     every call), so calls on one context run in the order they were issued
+ *     whatever streams they use.  Under graph capture nothing is waited for or
+ *     recorded, and the caller orders the captured work against other calls
+ *     on the context.  Separate contexts may be used concurrently.
  *   - Every function returns 0 on success or a WL_ERR_* code; wl_last_error()
  *     gives the message of the calling thread's last failure.
  */
 #ifndef WILSON_AMD_H
 #define WILSON_AMD_H
+
+#include <stddef.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -272,6 +280,39 @@ int wl_cg_eo_normal(wl_ctx* ctx, wl_gauge* g, const void* b, void* x, const void
  * V * datacount * ncomplex * 2 * word_bytes. */
 int wl_lime_decode(const void* payload, size_t nbytes, int word_bytes, const int* dims, int nd, int datacount,
                    int ncomplex, int order, void* out, int dtype, void* stream);
+
+/* Kernel timing of CG iterations (no counterpart in the reference; a
+ * measurement aid).  While profiling is on, solves on the context run their
+ * iterations host-driven (no graphs) with HIP events on the solve's stream
+ * around the two stencil launches and the search-direction update of every
+ * iteration; wl_ctx_get_profile reports the averages since profiling was
+ * switched on, split by the parity of the iteration within its pair (a
+ * mixed-precision solve adds to its solution accumulator every second
+ * iteration, in the launches of parity 1).  Switching on (or off) resets the
+ * sums.  The operator launch M (D, or the two half-lattice launches of the
+ * even Schur complement) is timed with its |M p|^2 reduction, M^dag with its
+ * fused CG step. */
+typedef struct {
+  int iterations[2];   /* iterations timed, by parity */
+  double us_op[2];     /* t = M p with |t|^2 partials */
+  double us_op_dag[2]; /* M^dag t with r -= alpha M^dag t, |r|^2 partials and the y update */
+  double us_xpay;      /* p' = r + beta p */
+} wl_profile;
+int wl_ctx_set_profiling(wl_ctx* ctx, int on);
+int wl_ctx_get_profile(wl_ctx* ctx, wl_profile* out);
+
+/* HBM streaming probes (no counterpart in the reference; the bandwidth the
+ * stencils are compared with): 16 bytes per lane, `unroll` (1, 2, 4) loads or
+ * stores in flight per lane, non-temporal when `nontemporal`; blocks = 0 covers
+ * the buffer once with one workgroup per 4 KiB x unroll, else `blocks`
+ * workgroups stride over it.  bytes: a multiple of 16; src and dst 16-byte
+ * aligned device pointers (READ: dst receives at most one word, COPY: dst of
+ * `bytes`).  Returns WL_ERR_BAD_ARG for bad arguments. */
+#define WL_STREAM_READ 0
+#define WL_STREAM_WRITE 1
+#define WL_STREAM_COPY 2
+int wl_stream_probe(int kind, int nontemporal, int unroll, const void* src, void* dst, size_t bytes, int blocks,
+                    void* stream);
 
 /* Continuous-batching solve engine (no counterpart in the reference; it serves
  * the point-source propagator loop of core/measurements.py:277-371 and request

@@ -73,6 +73,13 @@ class WlTuning(ctypes.Structure):
                 ("eo_slab", c_int)]
 
 
+class WlProfile(ctypes.Structure):
+    _fields_ = [("iterations", c_int * 2), ("us_op", c_double * 2), ("us_op_dag", c_double * 2), ("us_xpay", c_double)]
+
+
+WL_STREAM_READ, WL_STREAM_WRITE, WL_STREAM_COPY = 0, 1, 2
+
+
 class NativeError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"libwilson_amd error {code}: {msg}")
@@ -125,6 +132,9 @@ def load() -> ctypes.CDLL:
         "wl_pf_force": (c_int, [vp, vp, vp, vp, vp, c_int, c_int, vp]),
         "wl_cg_eo_normal": (c_int, [vp, vp, vp, vp, vp, c_int, c_int, POINTER(WlSolveParams), POINTER(WlSolveStats), vp]),
         "wl_lime_decode": (c_int, [vp, c_size_t, c_int, POINTER(c_int), c_int, c_int, c_int, c_int, vp, c_int, vp]),
+        "wl_ctx_set_profiling": (c_int, [vp, c_int]),
+        "wl_ctx_get_profile": (c_int, [vp, POINTER(WlProfile)]),
+        "wl_stream_probe": (c_int, [c_int, c_int, c_int, vp, vp, c_size_t, c_int, vp]),
     }
     sig.update({
         "wl_batch_bytes": (c_size_t, [vp, c_int, c_int]),
@@ -211,6 +221,17 @@ class Context:
     def set_mass(self, mass: float) -> None:
         check(self._lib.wl_ctx_set_mass(self.h, float(mass)))
         self.mass = float(mass)
+
+    def set_profiling(self, on: bool) -> None:
+        """Kernel timing of CG iterations (host-driven iterations with HIP
+        events between launches); switching resets the sums."""
+        check(self._lib.wl_ctx_set_profiling(self.h, 1 if on else 0))
+
+    def profile(self) -> dict:
+        p = WlProfile()
+        check(self._lib.wl_ctx_get_profile(self.h, byref(p)))
+        return {"iterations": [p.iterations[0], p.iterations[1]], "us_op": [p.us_op[0], p.us_op[1]],
+                "us_op_dag": [p.us_op_dag[0], p.us_op_dag[1]], "us_xpay": p.us_xpay}
 
     def __del__(self):
         h = getattr(self, "h", None)

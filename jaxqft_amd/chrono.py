@@ -103,3 +103,50 @@ def chrono_predict_mre(history_x: List, history_y: List, target, max_hist: int, 
     if not (math.isfinite(r_mre) and math.isfinite(r_last)) or r_mre >= 0.97 * r_last:
         return xs[-1]
     return x0
+
+
+class SolverGuess:
+    """History of normal-equation solutions and their images y = D x for
+    chronological initial guesses, kept per pseudofermion as the reference
+    monomial keeps it (WilsonNf2PseudofermionMonomial._predict_solver_guess /
+    _update_solver_guess, su3_wilson_nf2.py:254-294): at most `history`
+    entries; mode 'last' or 'poly' extrapolates the solutions, 'mre' combines
+    them to fit the target eta (phi = D^dag eta) and falls back to 'poly'
+    without eta or images."""
+
+    def __init__(self, mode: str = "last", history: int = 1):
+        mode = str(mode).lower()
+        if mode not in ("last", "poly", "mre"):
+            raise ValueError("solver_guess_mode must be one of: last, poly, mre")
+        if int(history) <= 0:
+            raise ValueError("solver_guess_history must be positive")
+        self.mode = mode
+        self.history = int(history)
+        self.xs: List = []
+        self.ys: List = []
+
+    def clear(self) -> None:
+        self.xs = []
+        self.ys = []
+
+    def predict(self, phi, eta=None):
+        """A guess of the same shape as phi, or None with no usable history."""
+        shape = tuple(phi.shape)
+        keep = [(x, y) for x, y in zip(self.xs, self.ys) if tuple(x.shape) == shape][-self.history:]
+        self.xs = [x for x, _ in keep]
+        self.ys = [y for _, y in keep]
+        if not self.xs:
+            return None
+        if self.mode == "mre":
+            if eta is not None:
+                pairs = [(x, y) for x, y in zip(self.xs, self.ys) if y is not None and tuple(y.shape) == tuple(eta.shape)]
+                if pairs:
+                    x0 = chrono_predict_mre([p[0] for p in pairs], [p[1] for p in pairs], eta, self.history)
+                    if x0 is not None:
+                        return x0
+            return chrono_predict_from_history(self.xs, "poly", self.history)
+        return chrono_predict_from_history(self.xs, self.mode, self.history)
+
+    def update(self, x, y=None) -> None:
+        self.xs = (self.xs + [x])[-self.history:]
+        self.ys = (self.ys + [y])[-self.history:]

@@ -63,6 +63,12 @@ static int fail(int code, const char* fmt, ...) {
   return code;
 }
 
+namespace wl {
+// Records the message of a failure found in another translation unit (the
+// calling thread's wl_last_error) and returns code.
+int report_error(int code, const char* msg) { return fail(code, "%s", msg); }
+}  // namespace wl
+
 #define HIPCHK(expr)                                                                            \
   do {                                                                                          \
     hipError_t _e = (expr);                                                                     \
@@ -132,6 +138,51 @@ struct wl_ctx {
   int eo_slab[2][3] = {{0, 0, 0}, {0, 0, 0}};        // x1 slab of the even-odd hop (0 = flat)
   int lds_per_cu = 0;     // bytes of LDS per CU
   int lds_per_block = 0;  // largest LDS allocation of one workgroup
+  // stream order of calls sharing the workspace: the end of the last call,
+  // recorded on the stream it ran on
+  hipEvent_t order_ev = nullptr;
+  hipStream_t order_st = nullptr;
+  bool order_set = false;
+  // kernel timing of CG iterations (wl_ctx_set_profiling)
+  struct Prof {
+    static constexpr int SLOTS = 64;  // iterations timed per host-driven block
+    bool on = false;
+    hipEvent_t ev[SLOTS][4] = {};
+    int par[SLOTS] = {};
+    int n = 0;
+    double sum_op[2] = {0, 0}, sum_dag[2] = {0, 0}, sum_xpay = 0;
+    int cnt[2] = {0, 0};
+  } prof;
+};
+
+// Calls on one context share its workspace (ws, partials, cols).  A call on
+// a stream other than the previous call's first waits for the end of that
+// call, and every call records its own end, so calls issued from different
+// streams are ordered as issued.  Under graph capture the caller orders the
+// captured work itself (an event of the uncaptured stream cannot be waited
+// on inside a capture): nothing is waited for or recorded there.
+struct StreamOrder {
+  wl_ctx* c;
+  hipStream_t st;
+  bool on;
+  StreamOrder(wl_ctx* ctx, hipStream_t s) : c(ctx), st(s), on(true) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (!c->order_ev || (hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)) {
+      on = false;
+      return;
+    }
+    (void)hipGetLastError();
+    if (c->order_set && c->order_st != st) (void)hipStreamWaitEvent(st, c->order_ev, 0);
+  }
+  ~StreamOrder() {
+    if (!on) return;
+    if (hipEventRecord(c->order_ev, st) == hipSuccess) {
+      c->order_st = st;
+      c->order_set = true;
+    } else {
+      (void)hipGetLastError();
+    }
+  }
 };
 
 struct wl_gauge {
@@ -241,7 +292,6 @@ static int plane_rows(const wl_ctx* c) {
   return 0;
 }
 
-
 extern "C" {
 
 int wl_version(void) { return 1; }
@@ -286,17 +336,72 @@ int wl_ctx_create(const wl_geom* geom, const wl_params* params, wl_ctx** out) {
     delete c;
     return fail(WL_ERR_HIP, "no HIP device");
   }
+  if (hipEventCreateWithFlags(&c->order_ev, hipEventDisableTiming) != hipSuccess) {
+    delete c;
+    return fail(WL_ERR_HIP, "stream-order event");
+  }
   *out = c;
   return WL_OK;
 }
 
+static void prof_release(wl_ctx* c) {
+  for (auto& row : c->prof.ev)
+    for (auto& e : row)
+      if (e) {
+        (void)hipEventDestroy(e);
+        e = nullptr;
+      }
+  c->prof.on = false;
+}
+
 void wl_ctx_destroy(wl_ctx* c) {
   if (!c) return;
+  if (c->order_set) (void)hipEventSynchronize(c->order_ev);
   c->partials.release();
   c->cols.release();
   for (auto& b : c->ws) b.release();
   if (c->host_cols) (void)hipHostFree(c->host_cols);
+  if (c->order_ev) (void)hipEventDestroy(c->order_ev);
+  prof_release(c);
   delete c;
+}
+
+int wl_ctx_set_profiling(wl_ctx* c, int on) {
+  CHK(check_ctx(c));
+  std::lock_guard<std::mutex> lk(c->mu);
+  auto& p = c->prof;
+  if (on && !p.on) {
+    for (auto& row : p.ev)
+      for (auto& e : row)
+        if (hipEventCreate(&e) != hipSuccess) {
+          e = nullptr;
+          prof_release(c);
+          return fail(WL_ERR_HIP, "profiling events");
+        }
+    p.on = true;
+  } else if (!on && p.on) {
+    prof_release(c);
+  }
+  p.n = 0;
+  p.sum_op[0] = p.sum_op[1] = p.sum_dag[0] = p.sum_dag[1] = p.sum_xpay = 0.0;
+  p.cnt[0] = p.cnt[1] = 0;
+  return WL_OK;
+}
+
+int wl_ctx_get_profile(wl_ctx* c, wl_profile* out) {
+  CHK(check_ctx(c));
+  if (!out) return fail(WL_ERR_BAD_ARG, "null profile");
+  std::lock_guard<std::mutex> lk(c->mu);
+  const auto& p = c->prof;
+  memset(out, 0, sizeof(*out));
+  for (int q = 0; q < 2; ++q) {
+    out->iterations[q] = p.cnt[q];
+    out->us_op[q] = p.cnt[q] ? p.sum_op[q] / p.cnt[q] : 0.0;
+    out->us_op_dag[q] = p.cnt[q] ? p.sum_dag[q] / p.cnt[q] : 0.0;
+  }
+  const int n = p.cnt[0] + p.cnt[1];
+  out->us_xpay = n ? p.sum_xpay / n : 0.0;
+  return WL_OK;
 }
 
 int wl_ctx_set_mass(wl_ctx* c, double mass) {
@@ -684,29 +789,63 @@ static int op_internal(wl_ctx* c, wl_gauge* g, int op, int prec, int recon, cons
   return dslash_internal(c, g, op, prec, recon, in, out, ncol, nrhs, col_bytes, st);
 }
 
-extern "C" {
-
-int wl_apply(wl_ctx* c, wl_gauge* g, int op, const void* in, void* out, int nrhs, int dtype, void* stream) {
+static int check_apply_args(const wl_ctx* c, const wl_gauge* g, int op, const void* in, const void* out, int nrhs,
+                            int dtype) {
   CHK(check_ctx(c));
   if (!g || g->ctx != c) return fail(WL_ERR_BAD_ARG, "gauge handle does not belong to this context");
   if (!in || !out) return fail(WL_ERR_BAD_ARG, "null field pointer");
   if (nrhs < 1) return fail(WL_ERR_BAD_ARG, "nrhs must be >= 1");
   if (dtype != WL_C128 && dtype != WL_C64) return fail(WL_ERR_BAD_ARG, "bad dtype %d", dtype);
-  CHK(check_op(op));
-  std::lock_guard<std::mutex> lk(c->mu);
-  hipStream_t st = (hipStream_t)stream;
+  return check_op(op);
+}
+
+static int apply_checked_run(wl_ctx* c, wl_gauge* g, int op, const void* in, void* out, int nrhs, int dtype, void* b0,
+                             void* b1, void* b2, hipStream_t st) {
   const int nc = c->geom.nc;
   const int ncol = c->geom.batch * nrhs;
   const size_t cb = col_bytes_of(c, F64);
-  CHK(c->ws[0].ensure(cb * ncol));
-  CHK(c->ws[1].ensure(cb * ncol));
-  CHK(c->ws[2].ensure(cb * ncol));
-  hipError_t e = pack_spinor(F64, nc, in, dtype, c->ws[0].p, c->V, c->Vp, ncol, cb, st);
+  hipError_t e = pack_spinor(F64, nc, in, dtype, b0, c->V, c->Vp, ncol, cb, st);
   if (e != hipSuccess) return fail(WL_ERR_HIP, "pack: %s", hipGetErrorString(e));
-  CHK(op_internal(c, g, op, F64, 18, c->ws[0].p, c->ws[1].p, c->ws[2].p, ncol, nrhs, cb, st));
-  e = unpack_spinor(F64, nc, c->ws[1].p, out, dtype, c->V, c->Vp, ncol, cb, st);
+  CHK(op_internal(c, g, op, F64, 18, b0, b1, b2, ncol, nrhs, cb, st));
+  e = unpack_spinor(F64, nc, b1, out, dtype, c->V, c->Vp, ncol, cb, st);
   if (e != hipSuccess) return fail(WL_ERR_HIP, "unpack: %s", hipGetErrorString(e));
   return WL_OK;
+}
+
+namespace wl {
+
+// Bytes of one of the three FP64 buffers of wl_apply for nrhs right-hand
+// sides per configuration.
+size_t apply_buffer_bytes(const wl_ctx* c, int nrhs) { return col_bytes_of(c, F64) * (size_t)c->geom.batch * nrhs; }
+
+int ctx_device(const wl_ctx* c) { return c->device; }
+
+// wl_apply: the reference-layout input is packed into b0, op is applied into
+// b1 (b2 holds D psi of D^dag D) and b1 is unpacked into out; each buffer
+// holds apply_buffer_bytes(c, nrhs).  With b0 == nullptr the buffers are the
+// context workspace (allocated on first use, ordered against the context's
+// other calls); otherwise they are the caller's and nothing is allocated.
+int apply_checked(wl_ctx* c, wl_gauge* g, int op, const void* in, void* out, int nrhs, int dtype, void* b0, void* b1,
+                  void* b2, hipStream_t st) {
+  CHK(check_apply_args(c, g, op, in, out, nrhs, dtype));
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (!b0) {
+    StreamOrder order(c, st);
+    const size_t need = apply_buffer_bytes(c, nrhs);
+    CHK(c->ws[0].ensure(need));
+    CHK(c->ws[1].ensure(need));
+    CHK(c->ws[2].ensure(need));
+    return apply_checked_run(c, g, op, in, out, nrhs, dtype, c->ws[0].p, c->ws[1].p, c->ws[2].p, st);
+  }
+  return apply_checked_run(c, g, op, in, out, nrhs, dtype, b0, b1, b2, st);
+}
+
+}  // namespace wl
+
+extern "C" {
+
+int wl_apply(wl_ctx* c, wl_gauge* g, int op, const void* in, void* out, int nrhs, int dtype, void* stream) {
+  return wl::apply_checked(c, g, op, in, out, nrhs, dtype, nullptr, nullptr, nullptr, (hipStream_t)stream);
 }
 
 int wl_gauge_prepare(wl_gauge* g, int prec, int recon, void* stream) {
@@ -776,6 +915,7 @@ int wl_dslash(wl_ctx* c, wl_gauge* g, int op, int recon, const wl_field* in, wl_
   if (nrhs < 1 || in->ncol != c->geom.batch * nrhs) return fail(WL_ERR_BAD_ARG, "field has %d columns, expected batch*nrhs", in->ncol);
   CHK(check_op(op));
   std::lock_guard<std::mutex> lk(c->mu);
+  StreamOrder order(c, (hipStream_t)stream);
   void* tmp = nullptr;
   if (op == WL_OP_DDAGD) {
     CHK(c->ws[9].ensure(in->col_bytes * in->ncol));
@@ -812,6 +952,14 @@ struct Solver {
   void* pb[2];    // search directions, used in turn: iteration k reads pb[k % 2] and writes pb[(k + 1) % 2]
   void *mid64 = nullptr, *midP = nullptr;  // odd-parity intermediates of S_e (even-odd only)
   Reduction red;
+  wl_ctx::Prof* prof = nullptr;  // kernel timing of host-driven iterations (null: off)
+
+  // the four events of the next timed iteration, or null
+  hipEvent_t* prof_slot(int par) {
+    if (!prof || !prof->on || prof->n >= wl_ctx::Prof::SLOTS) return nullptr;
+    prof->par[prof->n] = par;
+    return prof->ev[prof->n++];
+  }
 
   void dargs(DslashArgs& a, int prec, const void* U, int recon) {
     if (eo) {
@@ -950,16 +1098,21 @@ struct Solver {
   // column add alpha p of two iterations to y at once (every second
   // iteration), which halves the y traffic of the epilogue.
   int iteration(int par) {
+    hipEvent_t* ev = prof_slot(par);
     DslashArgs d;
     dargs(d, P, UP, R);
     d.gate = 1;
     d.col_bytes = cbP;
     d.red.par = par;
+    if (ev) HIPCHK(hipEventRecord(ev[0], st));
     CHK(apply(d, P, R, false, pb[par], t, midP, EPI_NORM, RA_PAP));
+    if (ev) HIPCHK(hipEventRecord(ev[1], st));
     d.w = pb[par]; d.w2 = pb[par ^ 1]; d.res = r; d.y = y; d.col_bytes_y = cbY;
     const int epi = PY == F64 ? EPI_CG64 : (PY == F32 ? EPI_CG32 : EPI_CG16);
     CHK(apply(d, P, R, true, t, nullptr, midP, epi, RA_ITER));
+    if (ev) HIPCHK(hipEventRecord(ev[2], st));
     CHK(xpay(bargs(1), par, false));
+    if (ev) HIPCHK(hipEventRecord(ev[3], st));
     // Mixed precision: the reliable update runs on the device right after the
     // iteration that requested it (all its kernels are gated on the column's
     // need_update flag), so no host round trip is needed.
@@ -988,6 +1141,26 @@ static int accum_prec_of(const wl_solve_params* sp, int inner) {
     case WL_ACCUM_F16: return F16;
     default: return F32;
   }
+}
+
+// Adds the timed iterations recorded since the last call (the stream has
+// been synchronized) to the context's kernel-time sums.
+static void prof_harvest(wl_ctx::Prof& p) {
+  for (int k = 0; k < p.n; ++k) {
+    float t01 = 0.f, t12 = 0.f, t23 = 0.f;
+    if (hipEventElapsedTime(&t01, p.ev[k][0], p.ev[k][1]) != hipSuccess ||
+        hipEventElapsedTime(&t12, p.ev[k][1], p.ev[k][2]) != hipSuccess ||
+        hipEventElapsedTime(&t23, p.ev[k][2], p.ev[k][3]) != hipSuccess) {
+      (void)hipGetLastError();
+      continue;
+    }
+    const int q = p.par[k] & 1;
+    p.sum_op[q] += 1e3 * t01;
+    p.sum_dag[q] += 1e3 * t12;
+    p.sum_xpay += 1e3 * t23;
+    p.cnt[q] += 1;
+  }
+  p.n = 0;
 }
 
 // CG on M^dag M x = b for internal FP64 fields b, x: M = D (full lattice) or,
@@ -1065,6 +1238,9 @@ static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int n
   s.red.partials = (double*)c->partials.p;
   s.red.cols = (ColState*)c->cols.p;
   s.red.action = RA_NONE;
+  // kernel timing runs the iterations host-driven, with events between launches
+  s.prof = c->prof.on ? &c->prof : nullptr;
+  const bool use_graph = sp->use_graph && !s.prof;
 
   HIPCHK(hipMemsetAsync(s.pb[0], 0, s.cbP * ncol, st));
   HIPCHK(hipMemsetAsync(s.pb[1], 0, s.cbP * ncol, st));
@@ -1123,7 +1299,7 @@ static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int n
     for (auto& g : graphs)
       if (g) (void)hipGraphDestroy(g);
   };
-  if (sp->use_graph) {
+  if (use_graph) {
     for (size_t k = 0; k < sizes.size(); ++k) {
       hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
       if (e != hipSuccess) {
@@ -1157,6 +1333,7 @@ static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int n
       result = fail(WL_ERR_HIP, "solver state read: %s", hipGetErrorString(pe));
       break;
     }
+    if (s.prof) prof_harvest(*s.prof);
     bool all_done = true, any_update = false, any_active = false;
     double remaining = 0.0;
     bool have_rate = false;
@@ -1239,6 +1416,7 @@ int wl_cg_normal_field(wl_ctx* c, wl_gauge* g, const wl_field* b, wl_field* x, i
   if (nrhs < 1 || b->ncol != c->geom.batch * nrhs || x->ncol != b->ncol)
     return fail(WL_ERR_BAD_ARG, "fields must have batch*nrhs columns");
   std::lock_guard<std::mutex> lk(c->mu);
+  StreamOrder order(c, (hipStream_t)stream);
   return cg_fields(c, g, b->buf.p, x->buf.p, use_x0, nrhs, sp, stats, (hipStream_t)stream);
 }
 
@@ -1250,6 +1428,7 @@ int wl_cg_normal(wl_ctx* c, wl_gauge* g, const void* b, void* x, const void* x0,
   if (nrhs < 1) return fail(WL_ERR_BAD_ARG, "nrhs must be >= 1");
   std::lock_guard<std::mutex> lk(c->mu);
   hipStream_t st = (hipStream_t)stream;
+  StreamOrder order(c, st);
   const int nc = c->geom.nc;
   const int ncol = c->geom.batch * nrhs;
   const size_t cb = col_bytes_of(c, F64);
@@ -1278,6 +1457,7 @@ int wl_eo_apply(wl_ctx* c, wl_gauge* g, int op, int normalized, const void* in, 
   if (c->Vh == 0) return fail(WL_ERR_UNSUPPORTED, "even-odd operators need every lattice extent even");
   std::lock_guard<std::mutex> lk(c->mu);
   hipStream_t st = (hipStream_t)stream;
+  StreamOrder order(c, st);
   const int nc = c->geom.nc;
   const int ncol = c->geom.batch * nrhs;
   const size_t cb = col_bytes_eo(c, F64);
@@ -1353,6 +1533,7 @@ int wl_cg_eo_normal(wl_ctx* c, wl_gauge* g, const void* b, void* x, const void* 
   if (c->Vh == 0) return fail(WL_ERR_UNSUPPORTED, "even-odd operators need every lattice extent even");
   std::lock_guard<std::mutex> lk(c->mu);
   hipStream_t st = (hipStream_t)stream;
+  StreamOrder order(c, st);
   const int nc = c->geom.nc;
   const int ncol = c->geom.batch * nrhs;
   const size_t cb = col_bytes_eo(c, F64);

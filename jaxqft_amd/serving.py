@@ -21,7 +21,7 @@ import collections
 import time
 from ctypes import byref, c_int, c_void_p
 from dataclasses import dataclass, field
-from typing import Any, Dict, List, Optional, Sequence, Tuple
+from typing import Any, Callable, Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
 import torch
@@ -34,8 +34,10 @@ class Request:
     rid: int
     key: Tuple[Tuple[int, ...], int]  # (lattice shape, gauge field id)
     mass: float
-    rhs: torch.Tensor                  # (k, *L, 4, Nc) complex, on the device
+    rhs: Optional[torch.Tensor]        # (k, *L, 4, Nc) complex, on the device (None: made at admission)
     arrival: float = 0.0               # seconds since the start of the run
+    make_rhs: Optional[Callable[[], torch.Tensor]] = None  # builds rhs at admission when rhs is None
+    k: int = 0                         # right-hand sides of a request whose rhs is made at admission
     # filled by the server
     x: Optional[torch.Tensor] = None
     admitted: Optional[float] = None
@@ -45,7 +47,7 @@ class Request:
 
     @property
     def nrhs(self) -> int:
-        return int(self.rhs.shape[0])
+        return int(self.rhs.shape[0]) if self.rhs is not None else int(self.k)
 
     @property
     def latency(self) -> float:
@@ -79,6 +81,8 @@ class Lane:
 
     def admit(self, req: Request) -> None:
         lib = self.ctx._lib
+        if req.rhs is None:
+            req.rhs = req.make_rhs()
         dt = nat.WL_C128 if req.rhs.dtype == torch.complex128 else nat.WL_C64
         slots = [i for i, o in enumerate(self.owner) if o is None][: req.nrhs]
         assert len(slots) == req.nrhs
@@ -181,9 +185,12 @@ class Server:
         self.admitted_bytes += cost
         return lane
 
-    def run(self, requests: Sequence[Request], realtime: bool = True) -> Dict[str, Any]:
+    def run(self, requests: Sequence[Request], realtime: bool = True, release: bool = False) -> Dict[str, Any]:
         """Serve `requests` (sorted by arrival).  With realtime=True arrivals
-        follow the wall clock; otherwise all are available at t=0."""
+        follow the wall clock; otherwise all are available at t=0.  With
+        release=True a finished request drops its right-hand sides and
+        solution (its stats are kept), so that long request streams do not
+        hold every field on the device."""
         for r in requests:
             if r.nrhs > self.capacity:
                 raise ValueError(f"request {r.rid} has {r.nrhs} right-hand sides > lane capacity {self.capacity}")
@@ -223,7 +230,12 @@ class Server:
             for lane in active:
                 lane.step()
             for lane in active:
-                completed.extend(lane.harvest(now))
+                fin = lane.harvest(now)
+                if release:
+                    for r in fin:
+                        r.rhs = None
+                        r.x = None
+                completed.extend(fin)
         makespan = now()
         for l in list(self.lanes.values()):
             l.close()
@@ -319,15 +331,22 @@ def fp64_queue(requests: Sequence[Request], gauges: Dict, ctxs: Dict, tol: float
     return summarize(done, now())
 
 
-def poisson_requests(rate_hz: float, n: int, lattices: Sequence[Tuple[int, ...]], fields_per_lattice: int,
-                     mass_range=(0.02, 0.2), rhs_range=(1, 12), seed: int = 0, nc: int = 3,
-                     dtype=torch.complex128) -> List[Request]:
-    """n requests with exponential inter-arrival times (mean 1/rate_hz), the
-    lattice, field, mass and RHS count drawn uniformly; complex Gaussian
-    right-hand sides generated on the device."""
-    rng = np.random.default_rng(seed)
+def _gaussian_rhs(shape, seed: int, dtype) -> torch.Tensor:
     gen = torch.Generator(device="cuda")
     gen.manual_seed(seed)
+    re = torch.randn(shape, generator=gen, device="cuda", dtype=torch.float64)
+    im = torch.randn(shape, generator=gen, device="cuda", dtype=torch.float64)
+    return torch.complex(re, im).to(dtype) / np.sqrt(2.0)
+
+
+def poisson_requests(rate_hz: float, n: int, lattices: Sequence[Tuple[int, ...]], fields_per_lattice: int,
+                     mass_range=(0.02, 0.2), rhs_range=(1, 12), seed: int = 0, nc: int = 3,
+                     dtype=torch.complex128, lazy: bool = False) -> List[Request]:
+    """n requests with exponential inter-arrival times (mean 1/rate_hz), the
+    lattice, field, mass and RHS count drawn uniformly; complex Gaussian
+    right-hand sides generated on the device (seeded per request), now or,
+    with lazy=True, when the request is admitted."""
+    rng = np.random.default_rng(seed)
     t = 0.0
     out = []
     for i in range(n):
@@ -337,8 +356,10 @@ def poisson_requests(rate_hz: float, n: int, lattices: Sequence[Tuple[int, ...]]
         mass = float(rng.uniform(*mass_range))
         k = int(rng.integers(rhs_range[0], rhs_range[1] + 1))
         shape = (k, *L, 4, nc)
-        re = torch.randn(shape, generator=gen, device="cuda", dtype=torch.float64)
-        im = torch.randn(shape, generator=gen, device="cuda", dtype=torch.float64)
-        rhs = torch.complex(re, im).to(dtype) / np.sqrt(2.0)
-        out.append(Request(rid=i, key=(L, fid), mass=mass, rhs=rhs, arrival=t))
+        rseed = seed * 1000003 + i
+        if lazy:
+            out.append(Request(rid=i, key=(L, fid), mass=mass, rhs=None, arrival=t, k=k,
+                               make_rhs=lambda shape=shape, rseed=rseed: _gaussian_rhs(shape, rseed, dtype)))
+        else:
+            out.append(Request(rid=i, key=(L, fid), mass=mass, rhs=_gaussian_rhs(shape, rseed, dtype), arrival=t))
     return out

@@ -26,6 +26,7 @@ from . import _native as nat
 from ._arrays import from_device, to_device
 from .fermions.gamma import build_euclidean_gamma, gamma5
 from .fermions.wilson import WilsonDiracOperator
+from .chrono import SolverGuess
 from . import fields as _fields
 
 
@@ -88,6 +89,30 @@ class WilsonNf2Theory:
     preconditioner_kind: str = "none"
     fermion_bc: Optional[Any] = None
     dirac_kernel: str = "optimized"
+    # further reference constructor fields (su3_wilson_nf2.py:594-623, su3_ym.py:181-188,
+    # u1_wilson_nf2.py:296-320), accepted so that a caller constructs the drop-in unchanged:
+    # the solver-guess fields drive the chronological guesses of pseudofermion_action /
+    # pseudofermion_force; the others select JAX jit, GMRES, half-spinor or HMC-monomial
+    # options that have no meaning here and are validated, stored and otherwise unused
+    exp_method: str = "expm"
+    gmres_restart: int = 32
+    gmres_solve_method: str = "batched"
+    eo_use_half_spinor: bool = True
+    include_gauge_monomial: bool = True
+    include_fermion_monomial: bool = True
+    fermion_monomial_kind: str = "unpreconditioned"
+    gauge_timescale: int = 0
+    fermion_timescale: int = 1
+    pseudofermion_refresh: str = "heatbath"
+    pseudofermion_gamma: Optional[float] = None
+    pseudofermion_force_mode: str = "autodiff"
+    smd_gamma: float = 0.3
+    auto_refresh_pseudofermions: bool = True
+    use_solver_guess: bool = False
+    solver_guess_mode: str = "last"
+    solver_guess_history: int = 1
+    jit_dirac_kernels: bool = True
+    jit_solvers: bool = True
     # native solver policy (no counterpart in the reference)
     solver_precision: str = "f64"   # inner precision: f64 (plain CG) | f32 | f16 (reliable updates)
     solver_recon: int = 18          # 18 | 12: link reconstruction of the inner operator
@@ -124,6 +149,22 @@ class WilsonNf2Theory:
             raise ValueError("preconditioner_kind must be one of: none, jacobi")
         if self.dirac_kernel not in ("optimized", "reference"):
             raise ValueError("dirac_kernel must be one of: optimized, reference")
+        # reference validation of the remaining constructor fields (su3_wilson_nf2.py:625-657, su3_ym.py:204-206)
+        self.exp_method = str(self.exp_method).lower()
+        if self.exp_method not in ("expm", "su3"):
+            raise ValueError("exp_method must be one of: expm, su3")
+        self.fermion_monomial_kind = str(self.fermion_monomial_kind).lower()
+        if self.fermion_monomial_kind not in ("unpreconditioned", "eo_preconditioned"):
+            raise ValueError("fermion_monomial_kind must be one of: unpreconditioned, eo_preconditioned")
+        self.gmres_restart = int(self.gmres_restart)
+        if self.gmres_restart <= 0:
+            raise ValueError("gmres_restart must be positive")
+        self.solver_guess_mode = str(self.solver_guess_mode).lower()
+        self.solver_guess_history = int(self.solver_guess_history)
+        self.solver_guess = SolverGuess(self.solver_guess_mode, self.solver_guess_history)
+        if self.pseudofermion_gamma is None:
+            refresh = str(self.pseudofermion_refresh).lower()
+            self.pseudofermion_gamma = float(self.smd_gamma) if refresh == "ou" else 0.3
         if self.solver_precision not in nat.PREC:
             raise ValueError("solver_precision must be one of: f64, f32, f16")
         if int(self.solver_recon) not in (18, 12):
@@ -355,11 +396,29 @@ class WilsonNf2Theory:
         f2 = self.pseudofermion_force_from_solution(U, X_e, -alpha * ky)
         return f1 + f2
 
-    def pseudofermion_force(self, U, phi):
+    def _solve_with_history(self, U, phi, eta=None):
+        """solve_normal, from a chronological guess when use_solver_guess is
+        set (the monomial's action and force_analytic, su3_wilson_nf2.py:322-343,
+        :378-392)."""
+        if not self.use_solver_guess:
+            return self.solve_normal(U, phi)
+        x0 = self.solver_guess.predict(phi, eta)
+        return self.solve_normal(U, phi) if x0 is None else self.solve_normal_with_guess(U, phi, x0)
+
+    def clear_solver_guess(self) -> None:
+        """Forget the solution history (a pseudofermion refresh, su3_wilson_nf2.py:318-320)."""
+        self.solver_guess.clear()
+
+    def pseudofermion_force(self, U, phi, eta=None):
         """The analytic monomial force: X = solve_normal(U, phi), Y = D X, then
-        the force from the solution (su3_wilson_nf2.py:378-392)."""
-        X = self.solve_normal(U, phi)
+        the force from the solution (su3_wilson_nf2.py:378-392).  With
+        use_solver_guess the solve starts from the guess of solver_guess_mode
+        over the last solver_guess_history (X, Y) pairs ('mre' fits eta,
+        phi = D^dag eta, when given)."""
+        X = self._solve_with_history(U, phi, eta)
         Y = self.apply_D(U, X)
+        if self.use_solver_guess:
+            self.solver_guess.update(X, Y)
         return self.pseudofermion_force_from_solution(U, X, Y)
 
     def _m0_value(self) -> float:
@@ -450,7 +509,9 @@ class WilsonNf2Theory:
 
     def pseudofermion_action(self, U, phi):
         """S = Re <phi, (D^dag D)^-1 phi> per batch element (su3_wilson_nf2.py:1183-1189)."""
-        x = self.solve_normal(U, phi)
+        x = self._solve_with_history(U, phi)
+        if self.use_solver_guess:
+            self.solver_guess.update(x)
         ph = np.asarray(phi.cpu() if isinstance(phi, torch.Tensor) else phi)
         xx = np.asarray(x.cpu() if isinstance(x, torch.Tensor) else x)
         return np.real(np.sum(np.conj(ph) * xx, axis=tuple(range(1, ph.ndim))))

@@ -90,6 +90,8 @@ def test_every_entry_point_rejects_null_handles():
         "wl_batch_step": lambda: lib.wl_batch_step(None),
         "wl_batch_query": lambda: lib.wl_batch_query(None, None, None),
         "wl_batch_retire": lambda: lib.wl_batch_retire(None, 0, None, 0),
+        "wl_ctx_set_profiling": lambda: lib.wl_ctx_set_profiling(None, 1),
+        "wl_ctx_get_profile": lambda: lib.wl_ctx_get_profile(None, None),
     }
     for name, call in calls.items():
         assert call() == B, name
@@ -100,7 +102,24 @@ def test_every_entry_point_rejects_null_handles():
     # every declared entry point that takes a handle is covered above or in the test before
     covered = set(calls) | {"wl_apply", "wl_field_import", "wl_cg_normal", "wl_ctx_destroy", "wl_gauge_release",
                             "wl_field_destroy", "wl_field_bytes", "wl_batch_bytes", "wl_batch_destroy"}
-    assert set(nat.declared_symbols()) - covered == {"wl_version", "wl_last_error", "wl_ctx_create"}
+    assert set(nat.declared_symbols()) - covered == {"wl_version", "wl_last_error", "wl_ctx_create",
+                                                     "wl_stream_probe"}
+
+
+def test_stream_probe_validation():
+    """The HBM streaming probe checks its arguments before any launch."""
+    lib = nat.load()
+    B = nat.WL_ERR_BAD_ARG
+    buf = ctypes.c_void_p(1 << 20)  # never dereferenced: every call below fails validation
+    assert lib.wl_stream_probe(3, 0, 1, buf, buf, 4096, 0, None) == B
+    assert lib.wl_stream_probe(nat.WL_STREAM_COPY, 0, 3, buf, buf, 4096, 0, None) == B
+    assert lib.wl_stream_probe(nat.WL_STREAM_COPY, 0, 1, buf, buf, 4095, 0, None) == B
+    assert lib.wl_stream_probe(nat.WL_STREAM_COPY, 0, 1, buf, buf, 0, 0, None) == B
+    assert lib.wl_stream_probe(nat.WL_STREAM_READ, 0, 1, None, buf, 4096, 0, None) == B
+    assert lib.wl_stream_probe(nat.WL_STREAM_WRITE, 0, 1, None, None, 4096, 0, None) == B
+    assert lib.wl_stream_probe(nat.WL_STREAM_COPY, 1, 4, ctypes.c_void_p((1 << 20) + 8), buf, 4096, 0, None) == B
+    assert lib.wl_stream_probe(nat.WL_STREAM_COPY, 1, 4, buf, buf, 4096, -1, None) == B
+    assert "stream probe" in lib.wl_last_error().decode()
 
 
 def test_theory_validation_and_shapes():
