@@ -219,6 +219,22 @@ int wl_gauge_prepare(wl_gauge* g, int prec, int recon, void* stream);
 /* Operator on reference-layout arrays of ncol = batch * nrhs spinors. */
 int wl_apply(wl_ctx* ctx, wl_gauge* g, int op, const void* in, void* out, int nrhs, int dtype, void* stream);
 
+/* wl_apply without device allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * operator, arguments and results as wl_apply (the same kernels), with the
+ * FP64 staging buffers in a statically sized per-device scratch array of
+ * WL_NOALLOC_SCRATCH_BYTES (part of the library's code object, held by each GPU
+ * that loads it until the process ends) instead of the context workspace:
+ * 2 (3 for WL_OP_DDAGD) buffers of batch * nrhs FP64 columns must fit (one
+ * FP64 column holds 192 bytes per site plus 288 padding sites, so e.g. one
+ * column at 16^4 or twelve at 8^4); larger calls return WL_ERR_BAD_ARG and
+ * need wl_apply.  Every call on a GPU shares that GPU's array and it keeps
+ * the last call's contents: calls that overlap on one GPU (from any context
+ * or stream) race, so the caller orders them. */
+#define WL_NOALLOC_SCRATCH_BYTES (64u << 20)
+int wl_apply_noalloc(wl_ctx* ctx, wl_gauge* g, int op, const void* in, void* out, int nrhs, int dtype, void* stream);
+
 /* Resident internal-layout fields: per column, structure-of-arrays planes of
  * 16-byte chunks over the padded site count (F64: double2 per complex; F32:
  * float4 per two complex; F16: eight halves per four complex, then one plane

@@ -78,6 +78,7 @@ class WlProfile(ctypes.Structure):
 
 
 WL_STREAM_READ, WL_STREAM_WRITE, WL_STREAM_COPY = 0, 1, 2
+WL_NOALLOC_SCRATCH_BYTES = 64 << 20
 
 
 class NativeError(RuntimeError):
@@ -120,6 +121,7 @@ def load() -> ctypes.CDLL:
         "wl_gauge_release": (None, [vp]),
         "wl_gauge_prepare": (c_int, [vp, c_int, c_int, vp]),
         "wl_apply": (c_int, [vp, vp, c_int, vp, vp, c_int, c_int, vp]),
+        "wl_apply_noalloc": (c_int, [vp, vp, c_int, vp, vp, c_int, c_int, vp]),
         "wl_field_create": (c_int, [vp, c_int, c_int, POINTER(vp)]),
         "wl_field_destroy": (None, [vp]),
         "wl_field_import": (c_int, [vp, vp, c_int, vp]),

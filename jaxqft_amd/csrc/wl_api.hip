@@ -40,6 +40,7 @@ hipError_t convert_gauge(int prec, int nc, int recon, const void* src, void* dst
 hipError_t launch_eo(int prec, int nc, int recon, int dag, int mode, int epi, const DslashArgs& a, hipStream_t stream);
 hipError_t launch_pf_force(int nc, int dtype, const ForceArgs& a, int ncfg, hipStream_t st);
 hipError_t launch_lime_decode(int word_bytes, int out_double, const LimeArgs& a, size_t lds_bytes, hipStream_t st);
+void noalloc_init_device(int dev);
 hipError_t pack_spinor_eo(int prec, int nc, const void* src, int dtype, void* dst, const Geom& g, long long Vh,
                           long long Vhp, int q, int ncol, size_t col_bytes, hipStream_t st);
 hipError_t unpack_spinor_eo(int prec, int nc, const void* src, void* dst, int dtype, const Geom& g, long long Vhp, int q,
@@ -340,6 +341,8 @@ int wl_ctx_create(const wl_geom* geom, const wl_params* params, wl_ctx** out) {
     delete c;
     return fail(WL_ERR_HIP, "stream-order event");
   }
+  // the device address of wl_apply_noalloc's scratch array, taken once per device
+  noalloc_init_device(c->device);
   *out = c;
   return WL_OK;
 }

@@ -100,7 +100,9 @@ def test_every_entry_point_rejects_null_handles():
     assert lib.wl_batch_bytes(None, 4, 0) == 0
     lib.wl_batch_destroy(None)
     # every declared entry point that takes a handle is covered above or in the test before
-    covered = set(calls) | {"wl_apply", "wl_field_import", "wl_cg_normal", "wl_ctx_destroy", "wl_gauge_release",
+    assert lib.wl_apply_noalloc(None, None, 0, None, None, 1, 0, None) == B
+    assert "null context" in lib.wl_last_error().decode()
+    covered = set(calls) | {"wl_apply", "wl_apply_noalloc", "wl_field_import", "wl_cg_normal", "wl_ctx_destroy", "wl_gauge_release",
                             "wl_field_destroy", "wl_field_bytes", "wl_batch_bytes", "wl_batch_destroy"}
     assert set(nat.declared_symbols()) - covered == {"wl_version", "wl_last_error", "wl_ctx_create",
                                                      "wl_stream_probe"}
