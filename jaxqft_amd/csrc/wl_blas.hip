@@ -8,11 +8,14 @@ namespace wl {
 
 
 
+// Every vector a BLAS kernel touches is read or written once: streaming
+// (non-temporal) accesses (measured at 32^4 on 12 FP16 columns: p' = r + beta
+// p 378 -> 352 us, and 319 us together with the column-slowest order below).
 template <int P, int NC, class T>
 __device__ __forceinline__ void load_as(const void* base, size_t V, size_t site, cx<T> (&v)[4 * NC]) {
   using A = typename PT<P>::acc;
   cx<A> t[4 * NC];
-  SpinorIO<P, NC>::load(base, V, site, t);
+  SpinorIO<P, NC>::template load<true>(base, V, site, t);
 #pragma unroll
   for (int c = 0; c < 4 * NC; ++c) v[c] = {(T)t[c].re, (T)t[c].im};
 }
@@ -23,21 +26,22 @@ __device__ __forceinline__ void store_as(void* base, size_t V, size_t site, cons
   cx<A> t[4 * NC];
 #pragma unroll
   for (int c = 0; c < 4 * NC; ++c) t[c] = {(A)v[c].re, (A)v[c].im};
-  SpinorIO<P, NC>::store(base, V, site, t);
+  SpinorIO<P, NC>::template store<true>(base, V, site, t);
 }
 
 // A workgroup loops over its (column, tile) units: one unit per workgroup
 // normally, several when a launch gated on pending reliable updates is capped
 // (blas_grid), so that a mostly idle launch dispatches few workgroups; a
 // capped launch first checks, with one round of loads, whether the gate
-// admits any column at all.
+// admits any column at all.  Units are ordered column-slowest: each column is
+// streamed as one contiguous run.
 #define WL_BLAS_PROLOGUE                                            \
   const unsigned total_ = (unsigned)a.ntiles * (unsigned)a.ncol;    \
   if (gridDim.x < total_ && !gate_any_col(a.gate, a.red, a.ncol))   \
     return;                                                         \
   for (unsigned u_ = blockIdx.x; u_ < total_; u_ += gridDim.x) {    \
-  const int col = (int)(u_ % (unsigned)a.ncol);                     \
-  const int tile = (int)(u_ / (unsigned)a.ncol);                    \
+  const int col = (int)(u_ / (unsigned)a.ntiles);                   \
+  const int tile = (int)(u_ % (unsigned)a.ntiles);                  \
   if (gate_skip_col(a.gate, a.red, col)) continue;                  \
   const size_t V = (size_t)a.Vp;                                    \
   const size_t site = (size_t)tile * NT + threadIdx.x;              \

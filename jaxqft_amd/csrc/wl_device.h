@@ -86,7 +86,54 @@ __host__ __device__ inline size_t link_dir_bytes(int prec, int nc, int recon, si
 }
 
 // ---------------------------------------------------------------------------
+// Global accesses, plain (S = false) or streaming (S = true: non-temporal
+// hint, for vectors read or written once per kernel)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double2 nt_ld(const double2* p) {
+  return make_double2(__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y));
+}
+__device__ __forceinline__ float4 nt_ld(const float4* p) {
+  return make_float4(__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y),
+                     __builtin_nontemporal_load(&p->z), __builtin_nontemporal_load(&p->w));
+}
+__device__ __forceinline__ uint4 nt_ld(const uint4* p) {
+  return make_uint4(__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y),
+                    __builtin_nontemporal_load(&p->z), __builtin_nontemporal_load(&p->w));
+}
+__device__ __forceinline__ float nt_ld(const float* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void nt_st(double2* p, double2 v) {
+  __builtin_nontemporal_store(v.x, &p->x);
+  __builtin_nontemporal_store(v.y, &p->y);
+}
+__device__ __forceinline__ void nt_st(float4* p, float4 v) {
+  __builtin_nontemporal_store(v.x, &p->x);
+  __builtin_nontemporal_store(v.y, &p->y);
+  __builtin_nontemporal_store(v.z, &p->z);
+  __builtin_nontemporal_store(v.w, &p->w);
+}
+__device__ __forceinline__ void nt_st(uint4* p, uint4 v) {
+  __builtin_nontemporal_store(v.x, &p->x);
+  __builtin_nontemporal_store(v.y, &p->y);
+  __builtin_nontemporal_store(v.z, &p->z);
+  __builtin_nontemporal_store(v.w, &p->w);
+}
+__device__ __forceinline__ void nt_st(float* p, float v) { __builtin_nontemporal_store(v, p); }
+
+template <bool S, class T>
+__device__ __forceinline__ T gld(const T* p) {
+  if constexpr (S) return nt_ld(p);
+  else return *p;
+}
+
+template <bool S, class T>
+__device__ __forceinline__ void gst(T* p, T v) {
+  if constexpr (S) nt_st(p, v);
+  else *p = v;
+}
+
+// ---------------------------------------------------------------------------
 // spinor load/store, NS=4 spins x NC colours, index c = s*NC + col
+// (S: streaming accesses, see gld / gst)
 // ---------------------------------------------------------------------------
 template <int P, int NC>
 struct SpinorIO;
@@ -94,42 +141,46 @@ struct SpinorIO;
 template <int NC>
 struct SpinorIO<F64, NC> {
   static constexpr int N = 4 * NC;
+  template <bool S = false>
   __device__ static __forceinline__ void load(const void* __restrict__ base, size_t V, size_t site,
                                               cx<double> (&v)[N]) {
     const double2* p = reinterpret_cast<const double2*>(base);
 #pragma unroll
     for (int c = 0; c < N; ++c) {
-      double2 t = p[c * V + site];
+      double2 t = gld<S>(p + c * V + site);
       v[c] = {t.x, t.y};
     }
   }
+  template <bool S = false>
   __device__ static __forceinline__ void store(void* __restrict__ base, size_t V, size_t site,
                                                const cx<double> (&v)[N]) {
     double2* p = reinterpret_cast<double2*>(base);
 #pragma unroll
-    for (int c = 0; c < N; ++c) p[c * V + site] = make_double2(v[c].re, v[c].im);
+    for (int c = 0; c < N; ++c) gst<S>(p + c * V + site, make_double2(v[c].re, v[c].im));
   }
 };
 
 template <int NC>
 struct SpinorIO<F32, NC> {
   static constexpr int N = 4 * NC;
+  template <bool S = false>
   __device__ static __forceinline__ void load(const void* __restrict__ base, size_t V, size_t site,
                                               cx<float> (&v)[N]) {
     const float4* p = reinterpret_cast<const float4*>(base);
 #pragma unroll
     for (int j = 0; j < N / 2; ++j) {
-      float4 t = p[j * V + site];
+      float4 t = gld<S>(p + j * V + site);
       v[2 * j] = {t.x, t.y};
       v[2 * j + 1] = {t.z, t.w};
     }
   }
+  template <bool S = false>
   __device__ static __forceinline__ void store(void* __restrict__ base, size_t V, size_t site,
                                                const cx<float> (&v)[N]) {
     float4* p = reinterpret_cast<float4*>(base);
 #pragma unroll
     for (int j = 0; j < N / 2; ++j)
-      p[j * V + site] = make_float4(v[2 * j].re, v[2 * j].im, v[2 * j + 1].re, v[2 * j + 1].im);
+      gst<S>(p + j * V + site, make_float4(v[2 * j].re, v[2 * j].im, v[2 * j + 1].re, v[2 * j + 1].im));
   }
 };
 
@@ -146,14 +197,15 @@ template <int NC>
 struct SpinorIO<F16, NC> {
   static constexpr int N = 4 * NC;
   static constexpr int PLANES = N / 4;
+  template <bool S = false>
   __device__ static __forceinline__ void load(const void* __restrict__ base, size_t V, size_t site,
                                               cx<float> (&v)[N]) {
     const uint4* p = reinterpret_cast<const uint4*>(base);
     const float* sc = reinterpret_cast<const float*>(p + PLANES * V);
-    float s = sc[site];
+    float s = gld<S>(sc + site);
 #pragma unroll
     for (int j = 0; j < PLANES; ++j) {
-      uint4 t = p[j * V + site];
+      uint4 t = gld<S>(p + j * V + site);
       float2 a = h2f(t.x), b = h2f(t.y), c = h2f(t.z), d = h2f(t.w);
       v[4 * j + 0] = {a.x * s, a.y * s};
       v[4 * j + 1] = {b.x * s, b.y * s};
@@ -161,6 +213,7 @@ struct SpinorIO<F16, NC> {
       v[4 * j + 3] = {d.x * s, d.y * s};
     }
   }
+  template <bool S = false>
   __device__ static __forceinline__ void store(void* __restrict__ base, size_t V, size_t site,
                                                const cx<float> (&v)[N]) {
     uint4* p = reinterpret_cast<uint4*>(base);
@@ -176,9 +229,9 @@ struct SpinorIO<F16, NC> {
       t.y = f2h(v[4 * j + 1].re * inv, v[4 * j + 1].im * inv);
       t.z = f2h(v[4 * j + 2].re * inv, v[4 * j + 2].im * inv);
       t.w = f2h(v[4 * j + 3].re * inv, v[4 * j + 3].im * inv);
-      p[j * V + site] = t;
+      gst<S>(p + j * V + site, t);
     }
-    sc[site] = m;
+    gst<S>(sc + site, m);
   }
 };
 

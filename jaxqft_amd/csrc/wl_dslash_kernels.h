@@ -108,7 +108,9 @@ struct EpiAccum {
 };
 
 // Epilogue at site s of column col: res holds the operator result (modified
-// in place for EPI_RESID).  Stores what the epilogue stores (only when
+// in place for EPI_RESID).  The epilogue's own vectors (b, r, p, y) are read
+// and written once per launch at the site itself: streaming accesses, which
+// keep them from displacing the neighbour spinors in the caches.  Stores what the epilogue stores (only when
 // `valid`) and returns the site's contribution to the column's reduction
 // partial (0 for an invalid thread).
 template <int P, int NC, int EPI>
@@ -126,7 +128,7 @@ __device__ __forceinline__ double site_epilogue(const DslashArgs& a, int col, si
   } else if (EPI == EPI_RESID || EPI == EPI_RESID32 || EPI == EPI_RESID16) {
     const char* wb = reinterpret_cast<const char*>(a.w) + (size_t)col * a.col_bytes;
     cx<T> w[4 * NC];
-    SpinorIO<P, NC>::load(wb, V, s, w);
+    SpinorIO<P, NC>::template load<true>(wb, V, s, w);
 #pragma unroll
     for (int q = 0; q < 4 * NC; ++q) {
       res[q] = csub(w[q], res[q]);
@@ -140,7 +142,7 @@ __device__ __forceinline__ double site_epilogue(const DslashArgs& a, int col, si
       cx<float> rr[4 * NC];
 #pragma unroll
       for (int q = 0; q < 4 * NC; ++q) rr[q] = {(float)res[q].re, (float)res[q].im};
-      if (valid) SpinorIO<PR, NC>::store(reinterpret_cast<char*>(a.res) + (size_t)col * a.col_bytes_y, V, s, rr);
+      if (valid) SpinorIO<PR, NC>::template store<true>(reinterpret_cast<char*>(a.res) + (size_t)col * a.col_bytes_y, V, s, rr);
     }
   } else {
     constexpr int PY = EpiAccum<EPI>::prec;
@@ -149,7 +151,7 @@ __device__ __forceinline__ double site_epilogue(const DslashArgs& a, int col, si
     const double alpha = cs.alpha;
     char* rb = reinterpret_cast<char*>(a.res) + (size_t)col * a.col_bytes;
     cx<T> r[4 * NC];
-    SpinorIO<P, NC>::load(rb, V, s, r);
+    SpinorIO<P, NC>::template load<true>(rb, V, s, r);
     const T ar = (T)alpha;
 #pragma unroll
     for (int q = 0; q < 4 * NC; ++q) {
@@ -157,7 +159,7 @@ __device__ __forceinline__ double site_epilogue(const DslashArgs& a, int col, si
       r[q].im -= ar * res[q].im;
       part += (double)r[q].re * (double)r[q].re + (double)r[q].im * (double)r[q].im;
     }
-    if (valid) SpinorIO<P, NC>::store(rb, V, s, r);
+    if (valid) SpinorIO<P, NC>::template store<true>(rb, V, s, r);
     // y += alpha p every iteration, or (pairwise) nothing in the first
     // iteration of a pair and alpha_prev p_prev + alpha p in the second
     if (!(cs.pairwise && !cs.pend)) {
@@ -165,8 +167,8 @@ __device__ __forceinline__ double site_epilogue(const DslashArgs& a, int col, si
       char* yb = reinterpret_cast<char*>(a.y) + (size_t)col * a.col_bytes_y;
       cx<T> p[4 * NC];
       cx<TY> y[4 * NC];
-      SpinorIO<P, NC>::load(pb, V, s, p);
-      SpinorIO<PY, NC>::load(yb, V, s, y);
+      SpinorIO<P, NC>::template load<true>(pb, V, s, p);
+      SpinorIO<PY, NC>::template load<true>(yb, V, s, y);
       const TY ay = (TY)alpha;
 #pragma unroll
       for (int q = 0; q < 4 * NC; ++q) {
@@ -175,7 +177,7 @@ __device__ __forceinline__ double site_epilogue(const DslashArgs& a, int col, si
       }
       if (cs.pairwise) {
         const char* qb = reinterpret_cast<const char*>(a.w2) + (size_t)col * a.col_bytes;
-        SpinorIO<P, NC>::load(qb, V, s, p);
+        SpinorIO<P, NC>::template load<true>(qb, V, s, p);
         const TY aq = (TY)cs.alpha_prev;
 #pragma unroll
         for (int q = 0; q < 4 * NC; ++q) {
@@ -183,7 +185,7 @@ __device__ __forceinline__ double site_epilogue(const DslashArgs& a, int col, si
           y[q].im += aq * (TY)p[q].im;
         }
       }
-      if (valid) SpinorIO<PY, NC>::store(yb, V, s, y);
+      if (valid) SpinorIO<PY, NC>::template store<true>(yb, V, s, y);
     }
   }
   return valid ? part : 0.0;
