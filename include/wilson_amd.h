@@ -313,6 +313,8 @@ typedef struct {
   double us_op[2];     /* t = M p with |t|^2 partials */
   double us_op_dag[2]; /* M^dag t with r -= alpha M^dag t, |r|^2 partials and the y update */
   double us_xpay;      /* p' = r + beta p */
+  int iterations_without_update; /* of those, iterations queued without the fused (gated) update launches */
+  int pad;
 } wl_profile;
 int wl_ctx_set_profiling(wl_ctx* ctx, int on);
 int wl_ctx_get_profile(wl_ctx* ctx, wl_profile* out);

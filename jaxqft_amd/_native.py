@@ -74,7 +74,8 @@ class WlTuning(ctypes.Structure):
 
 
 class WlProfile(ctypes.Structure):
-    _fields_ = [("iterations", c_int * 2), ("us_op", c_double * 2), ("us_op_dag", c_double * 2), ("us_xpay", c_double)]
+    _fields_ = [("iterations", c_int * 2), ("us_op", c_double * 2), ("us_op_dag", c_double * 2), ("us_xpay", c_double),
+                ("iterations_without_update", c_int), ("pad", c_int)]
 
 
 WL_STREAM_READ, WL_STREAM_WRITE, WL_STREAM_COPY = 0, 1, 2
@@ -233,7 +234,8 @@ class Context:
         p = WlProfile()
         check(self._lib.wl_ctx_get_profile(self.h, byref(p)))
         return {"iterations": [p.iterations[0], p.iterations[1]], "us_op": [p.us_op[0], p.us_op[1]],
-                "us_op_dag": [p.us_op_dag[0], p.us_op_dag[1]], "us_xpay": p.us_xpay}
+                "us_op_dag": [p.us_op_dag[0], p.us_op_dag[1]], "us_xpay": p.us_xpay,
+                "iterations_without_update": p.iterations_without_update}
 
     def __del__(self):
         h = getattr(self, "h", None)

@@ -153,6 +153,7 @@ struct wl_ctx {
     int n = 0;
     double sum_op[2] = {0, 0}, sum_dag[2] = {0, 0}, sum_xpay = 0;
     int cnt[2] = {0, 0};
+    int no_update = 0;  // iterations queued without the fused update launches
   } prof;
 };
 
@@ -388,6 +389,7 @@ int wl_ctx_set_profiling(wl_ctx* c, int on) {
   p.n = 0;
   p.sum_op[0] = p.sum_op[1] = p.sum_dag[0] = p.sum_dag[1] = p.sum_xpay = 0.0;
   p.cnt[0] = p.cnt[1] = 0;
+  p.no_update = 0;
   return WL_OK;
 }
 
@@ -404,6 +406,7 @@ int wl_ctx_get_profile(wl_ctx* c, wl_profile* out) {
   }
   const int n = p.cnt[0] + p.cnt[1];
   out->us_xpay = n ? p.sum_xpay / n : 0.0;
+  out->iterations_without_update = p.no_update;
   return WL_OK;
 }
 
@@ -1100,7 +1103,7 @@ struct Solver {
   // p = pb[par]; the previous iteration's p, in pb[par ^ 1], lets a pairwise
   // column add alpha p of two iterations to y at once (every second
   // iteration), which halves the y traffic of the epilogue.
-  int iteration(int par) {
+  int iteration(int par, bool with_update = true) {
     hipEvent_t* ev = prof_slot(par);
     DslashArgs d;
     dargs(d, P, UP, R);
@@ -1119,7 +1122,7 @@ struct Solver {
     // Mixed precision: the reliable update runs on the device right after the
     // iteration that requested it (all its kernels are gated on the column's
     // need_update flag), so no host round trip is needed.
-    if (fused) return reliable_update(par);
+    if (fused && with_update) return reliable_update(par);
     return WL_OK;
   }
 };
@@ -1340,6 +1343,20 @@ static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int n
     bool all_done = true, any_update = false, any_active = false;
     double remaining = 0.0;
     bool have_rate = false;
+    // Mixed precision: the iterations of the next block at which some active
+    // column may ask for a reliable update -- its residual drop reaching
+    // delta, the forced update after MAX_INNER_ITERS, its stopping threshold
+    // or maxiter -- predicted from its observed decay with margins (a 25%
+    // faster decay than observed, two iterations early, three late): each
+    // column's first request h and, delta later, its next one h + T.  A column
+    // without a rate yet may ask at any iteration.
+    std::vector<char> may_update(sizes[0], 0);
+    auto mark = [&](double from, double to) {
+      const int a = std::max(0, (int)std::floor(from) - 2);
+      const int b = std::min((int)may_update.size() - 1, (int)std::ceil(to) + 3);
+      for (int j = a; j <= b; ++j) may_update[j] = 1;
+    };
+    double horizon = 1e30;
     for (int i = 0; i < ncol; ++i) {
       const ColState& cs = hc[i];
       if (!cs.done) all_done = false;
@@ -1352,8 +1369,33 @@ static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int n
         left = std::min(left, (double)(cs.maxiter - cs.iter));
         remaining = std::max(remaining, left);
         have_rate = true;
+        const double fast = 1.25 * rate;
+        double h = cs.rr > cs.stop2 ? std::log(cs.rr / cs.stop2) / fast : 0.0;
+        if (cs.delta > 0.0) {
+          const double tgt = cs.delta * cs.delta * cs.r0norm * cs.r0norm;
+          h = std::min(h, cs.rr > tgt ? std::log(cs.rr / tgt) / fast : 0.0);
+        }
+        h = std::min(h, (double)(MAX_INNER_ITERS - (cs.iter - cs.last_update)));
+        h = std::min(h, (double)(cs.maxiter - cs.iter));
+        horizon = std::min(horizon, h);
+        mark(h, h);
+        // after an update the residual restarts near its iterated value: the
+        // next request is a drop of delta (or the stopping threshold) later
+        double T = cs.delta > 0.0 ? std::log(1.0 / (cs.delta * cs.delta)) / fast : 1e30;
+        const double to_stop = cs.rr > cs.stop2 ? std::log(cs.rr / cs.stop2) / fast - h : 0.0;
+        T = std::min(T, std::max(0.0, to_stop));
+        T = std::min(T, (double)MAX_INNER_ITERS);
+        // the later requests of the block: windows every T (T >= 1), and
+        // every iteration from the first one where T cannot be trusted
+        if (T >= 4.0) {
+          for (double t = h + T; t < (double)may_update.size() + 3.0; t += T) mark(t, t);
+        } else {
+          mark(h + T, 1e9);
+        }
       } else {
         remaining = std::max(remaining, (double)poll);
+        horizon = 0.0;
+        mark(0.0, 1e9);
       }
       prev_rr[i] = cs.rr;
       prev_it[i] = cs.iter;
@@ -1376,7 +1418,19 @@ static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int n
     if (have_rate) {
       while (k + 1 < sizes.size() && sizes[k] > remaining + 1.0) ++k;
     }
-    if (execs[k]) {
+    // iterations at which no column can ask for an update run without the
+    // fused (gated, then idle) update launches; a column that asks outside the
+    // predicted windows keeps its state (it stops iterating) until the next
+    // fused update of the block or the host's update after the block, so the
+    // iterates, and the solution, are the same as with every update fused
+    int skipped = 0;
+    if (s.fused)
+      for (int i = 0; i < sizes[k]; ++i) skipped += may_update[i] ? 0 : 1;
+    if (skipped > 0) {
+      if (s.prof) s.prof->no_update += skipped;
+      for (int i = 0; i < sizes[k] && result == WL_OK; ++i) result = s.iteration(i & 1, may_update[i] != 0);
+      if (result != WL_OK) break;
+    } else if (execs[k]) {
       hipError_t e = hipGraphLaunch(execs[k], st);
       if (e != hipSuccess) {
         result = fail(WL_ERR_HIP, "graph launch: %s", hipGetErrorString(e));
