@@ -171,6 +171,23 @@ def measured_traffic(args, k, L, tuning):
     return None, None
 
 
+PROFILE_TRAFFIC_CG = Path(__file__).resolve().parent / "profiles" / "traffic_cg_step.json"
+
+
+def measured_traffic_cg(args, k, L):
+    """HBM bytes per launch of the D^dag + CG-step stencil (FETCH_SIZE +
+    WRITE_SIZE, calibrated against a copy of known size) from the committed
+    rocprofv3 summary, when that summary is of this workload; else None."""
+    try:
+        d = json.loads(PROFILE_TRAFFIC_CG.read_text())
+    except (OSError, ValueError):
+        return None, None
+    want = {"lattice": list(L), "nrhs": k, "inner": args.inner, "recon": args.recon, "accumulator": args.accum}
+    if any(d.get(key) != val for key, val in want.items()):
+        return None, None
+    return int(d["fetch_bytes_per_launch"] + d["write_bytes_per_launch"]), "profiles/" + PROFILE_TRAFFIC_CG.name
+
+
 def summarize(stats):
     return {
         "iters_max": max(s["iters"] for s in stats),
@@ -291,20 +308,29 @@ def main():
                 "GB/s": round(gbs, 1), "frac_hbm_peak": round(gbs / PEAK_HBM_GBS, 4), "bytes_per_site": bps,
                 "tune_us": tried,
             }
-            log(f"dslash {prec} r{recon} k={k}: {best[1]:.1f} us ({best[1]/k:.1f}/rhs) {gbs:.0f} GB/s {tried}")
+            log(f"dslash {prec} r{recon} k={k}: {best[1]:.1f} us ({best[1]/k:.1f}/rhs) {gbs:.0f} GB/s, "
+                f"{len(tried)} shapes tuned (table in the full record)")
             del fi, fo
 
         if args.solve_trace:
-            # one warm headline solve, a pause that marks the trace, the traced solve
+            # one warm headline solve; the calibration copy (1 GiB read and
+            # written by the streaming probe, for counter runs); a pause that
+            # marks the trace; the traced solve
             bk = nat.Field(ctx, nat.WL_F64, k)
             bk.import_(bk_t)
             xk = nat.Field(ctx, nat.WL_F64, k)
             B.solve(ctx, gauge, bk, xk, k, args.inner, args.recon)
             torch.cuda.synchronize()
+            calib = torch.empty(2 << 27, dtype=torch.float64, device="cuda")
+            nat.check(B.lib.wl_stream_probe(nat.WL_STREAM_COPY, 1, 1, c_void_p(calib.data_ptr()),
+                                            c_void_p(calib.data_ptr() + (1 << 30)), 1 << 30, 0, B.h))
+            torch.cuda.synchronize()
+            del calib
             time.sleep(1.0)
             t, st = B.timed_solves(1, ctx, gauge, bk, xk, k, args.inner, args.recon)
             log(f"traced mixed {k}-RHS solve {t*1e3:.1f} ms {summarize(st)}")
-            print(json.dumps({"solve_trace": True, "ms": round(t * 1e3, 2), **summarize(st)}))
+            print(json.dumps({"solve_trace": True, "ms": round(t * 1e3, 2), "calibration_copy_bytes_each_way": 1 << 30,
+                              **summarize(st)}))
             return
 
         # ------------------------------------------------------------ solves
@@ -440,6 +466,7 @@ def main():
             entry["frac_of_device_copy"] = round(entry["GB/s"] / copy["GB/s"], 4)
 
     cg_step = cg_step_roofline(args, V, k, prof, stp)
+    cg_step["traffic"], cg_step["traffic_source"] = measured_traffic_cg(args, k, L)
     detail = {
         "metric": METRIC,
         "value": round(k / mixk_s, 4),
@@ -592,7 +619,7 @@ def compact_record(args, d, L, k):
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
             "frac": dag.get("frac_hbm_peak"),
-            "traffic": None,
+            "traffic": cg.get("traffic"),
             "kernel_us": dag.get("us_mean"),
             "bytes_per_site": [dag.get("bytes_per_site_without_y"), dag.get("bytes_per_site_with_y")],
         },
