@@ -193,6 +193,13 @@ typedef struct wl_tuning {
   int eo_slab;        /* even-odd hop (wl_eo_apply, wl_cg_eo_normal): 0 = flat order of the
                          compact sites; s > 0 (must divide L1) = slabs of s consecutive x1
                          planes walked x0-fastest */
+  int lds_box;        /* FP16 SU(3) with 12-real links (18-real launches keep the other
+                         settings): 0 = off; k = 1..3 = the LDS box kernel, one workgroup
+                         per 256-site box of one configuration walking all of its columns
+                         two per pass, with the box's links loaded once and in-box
+                         neighbours read from LDS; box extents (x0,x1,x2,x3) 1 (2,2,4,16),
+                         2 (2,4,4,8), 3 (4,2,4,8), which must divide the lattice; takes
+                         precedence over every other shape setting */
 } wl_tuning;
 /* wl_ctx_get_tuning reports the values in effect for nrhs right-hand sides. */
 int wl_ctx_set_tuning(wl_ctx* ctx, int prec, int multi_rhs, const wl_tuning* t);

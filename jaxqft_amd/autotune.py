@@ -41,7 +41,8 @@ def tune(ctx: nat.Context, gauge: nat.Gauge, prec: int, recon: int, fin: nat.Fie
          nrhs: int = 1, marches: Iterable[int] = (1, 2, 4, 8), reps: int = 20,
          stream: Optional[torch.cuda.Stream] = None, slabs: Iterable[int] = (0, 2, 4, 8),
          cols_outer: Iterable[int] = (0, 2, 3), boxes: Iterable[int] = (1, 2, 7),
-         mrhs_marches: Iterable[int] = (2, 4, 8, 16), planes: Iterable[int] = (2, 4, 8)) -> Dict[str, object]:
+         mrhs_marches: Iterable[int] = (2, 4, 8, 16), planes: Iterable[int] = (2, 4, 8),
+         lds_boxes: Iterable[int] = (1, 2, 3)) -> Dict[str, object]:
     """Pick the fastest launch shape for (prec, recon) at nrhs right-hand sides
     per configuration and install it.  Candidates: every march length dividing
     L0 with one right-hand side per thread, plus 2 and 4 right-hand sides per
@@ -54,8 +55,8 @@ def tune(ctx: nat.Context, gauge: nat.Gauge, prec: int, recon: int, fin: nat.Fie
     groups, sweeping the lattice in turn, which keeps a block's spinor
     neighbours in L2 while its links stay shared; sweeping one column at a
     time, n = 1, re-reads the links and measured slower at 32^4).
-    Returns {"march", "rhs_per_thread", "xcd_remap", "slab", "block_shape", "plane", "cols_outer",
-    "us": {label: us}}."""
+    Returns {"march", "rhs_per_thread", "xcd_remap", "slab", "block_shape", "plane", "lds_box",
+    "cols_outer", "us": {label: us}}."""
     stream = stream or torch.cuda.current_stream()
     L0, L1 = ctx.lattice_shape[0], ctx.lattice_shape[1]
     multi = nrhs > 1
@@ -78,23 +79,29 @@ def tune(ctx: nat.Context, gauge: nat.Gauge, prec: int, recon: int, fin: nat.Fie
         if prec == nat.WL_F16 and nrhs % 2 == 0:
             # the FP16 two-RHS kernel marching along x0
             shapes += [(m, 2, ("s", 0)) for m in mrhs_marches if m > 1 and L0 % m == 0]
+    if prec == nat.WL_F16 and ctx.nc == 3 and recon == 12:
+        # the LDS box kernel (two columns per pass; 12-real links)
+        shapes += [(1, 2 if multi else 1, ("l", b)) for b in lds_boxes if _lds_box_fits(ctx.lattice_shape, b)]
     corders = [int(c) for c in cols_outer] if multi else [0]
 
     eo_slab = ctx.get_tuning(prec, nrhs)["eo_slab"]  # kept: the even-odd hop is tuned on its own
 
     def install(m, k, r, o, co):
         ctx.set_tuning(prec, multi, m, k if multi else 0, bool(r), o[1] if o[0] == "s" else 0, co, 0,
-                       o[1] if o[0] == "b" else 0, o[1] if o[0] == "p" else 0, eo_slab)
+                       o[1] if o[0] == "b" else 0, o[1] if o[0] == "p" else 0, eo_slab, o[1] if o[0] == "l" else 0)
 
     times = {}
     for co in corders:
         for r in (1, 0):
             for m, k, o in shapes:
+                if o[0] == "l" and (co != corders[0] or r != 1):
+                    continue  # the box kernel ignores the workgroup and column orders
                 install(m, k, r, o, co)
                 times[(m, k, r, o, co)] = time_dslash(ctx, gauge, prec, recon, fin, fout, nrhs, reps, stream)
     def as_dict(m, k, r, o, co):
         return {"march": m, "rhs_per_thread": k, "xcd_remap": r, "slab": o[1] if o[0] == "s" else 0,
-                "block_shape": o[1] if o[0] == "b" else 0, "plane": o[1] if o[0] == "p" else 0, "cols_outer": co}
+                "block_shape": o[1] if o[0] == "b" else 0, "plane": o[1] if o[0] == "p" else 0,
+                "lds_box": o[1] if o[0] == "l" else 0, "cols_outer": co}
 
     ranked = sorted(times, key=times.get)
     best = ranked[0]
@@ -109,7 +116,8 @@ def install_tuning(ctx: nat.Context, prec: int, multi: bool, t: Dict[str, object
     """Install a tuning dict as returned in `tune(...)["ranked"]` (the even-odd slab is kept)."""
     eo = ctx.get_tuning(prec, 2 if multi else 1)["eo_slab"]
     ctx.set_tuning(prec, multi, int(t["march"]), int(t["rhs_per_thread"]) if multi else 0, bool(t["xcd_remap"]),
-                   int(t["slab"]), int(t["cols_outer"]), 0, int(t["block_shape"]), int(t["plane"]), eo)
+                   int(t["slab"]), int(t["cols_outer"]), 0, int(t["block_shape"]), int(t["plane"]), eo,
+                   int(t.get("lds_box", 0)))
 
 
 def set_eo_slab(ctx: nat.Context, prec: int, multi: bool, eo_slab: int) -> None:
@@ -120,7 +128,7 @@ def set_eo_slab(ctx: nat.Context, prec: int, multi: bool, eo_slab: int) -> None:
 def tune_eo(ctx: nat.Context, gauge: nat.Gauge, b: torch.Tensor, x: torch.Tensor, nrhs: int, prec: int, recon: int,
             slabs: Iterable[int] = (0, 2, 4, 8), iters: int = 24, delta: float = 0.01,
             stream: Optional[torch.cuda.Stream] = None,
-            update_slabs: Iterable[int] = (0, 2, 4, 8)) -> Dict[str, object]:
+            update_slabs: Iterable[int] = (0, 2, 4, 8), update_iters: int = 8) -> Dict[str, object]:
     """Pick the x1 slab order of the even-odd hop for inner precision prec at
     nrhs right-hand sides per configuration and install it.  Each candidate
     (flat, or slab extents dividing L1) is timed on a solve of the even-odd
@@ -130,10 +138,14 @@ def tune_eo(ctx: nat.Context, gauge: nat.Gauge, b: torch.Tensor, x: torch.Tensor
     the ones a full solve repeats; the solves capture HIP graphs, so they run
     on `stream` or, by default, on a side stream ordered after the current
     one.  For a mixed-precision prec the slab of the FP64 hops (the reliable
-    updates, and FP64 even-odd operators at this nrhs) is then picked the
-    same way among `update_slabs`, with the inner slab installed (an empty
-    `update_slabs` leaves it alone).  Returns {"eo_slab", "ms": {slab: ms}}
-    and, when the FP64 slab was tuned, {"update_eo_slab", "update_ms"}."""
+    updates, plain FP64 even-odd solves and FP64 even-odd operators at this
+    nrhs share it) is then picked among `update_slabs`, with the inner slab
+    installed (an empty `update_slabs` leaves it alone).  Those candidates
+    are timed on an FP64-inner solve capped at `update_iters` iterations, in
+    which every hop is an FP64 hop: in the mixed solve the update hops are
+    idle (gated off) in all but a few iterations, so its time would rank
+    them by dispatch noise.  Returns {"eo_slab", "ms": {slab: ms}} and, when
+    the FP64 slab was tuned, {"update_eo_slab", "update_ms"}."""
     if stream is None:
         stream = torch.cuda.Stream()
         stream.wait_stream(torch.cuda.current_stream())
@@ -145,14 +157,15 @@ def tune_eo(ctx: nat.Context, gauge: nat.Gauge, b: torch.Tensor, x: torch.Tensor
                 or t.numel() != ncol * ctx.volume * 4 * ctx.nc):
             raise ValueError(f"{name} must be a contiguous complex128 CUDA tensor of {ncol} spinor fields")
     h = c_void_p(stream.cuda_stream)
-    sp, keep = nat.solve_params(1e-30, iters, nat.WL_STOP_TRUE_RESIDUAL, prec, recon, delta, 16, True)
     st = (nat.WlSolveStats * ncol)()
 
-    def run():
-        nat.check(ctx._lib.wl_cg_eo_normal(ctx.h, gauge.h, c_void_p(b.data_ptr()), c_void_p(x.data_ptr()), c_void_p(0),
-                                           nrhs, nat.WL_C128, byref(sp), st, h))
+    def sweep(p, cands, solve_prec, n_iter):
+        sp, keep = nat.solve_params(1e-30, n_iter, nat.WL_STOP_TRUE_RESIDUAL, solve_prec, recon, delta, 16, True)
 
-    def sweep(p, cands):
+        def run():
+            nat.check(ctx._lib.wl_cg_eo_normal(ctx.h, gauge.h, c_void_p(b.data_ptr()), c_void_p(x.data_ptr()),
+                                               c_void_p(0), nrhs, nat.WL_C128, byref(sp), st, h))
+
         times = {}
         for s in cands:
             if s and L1 % s:
@@ -170,16 +183,16 @@ def tune_eo(ctx: nat.Context, gauge: nat.Gauge, b: torch.Tensor, x: torch.Tensor
                 t = e0.elapsed_time(e1)
                 best = t if best is None else min(best, t)
             times[s] = best
+        del keep
         pick = min(times, key=times.get)
         set_eo_slab(ctx, p, multi, pick)
         return pick, {str(s): round(t, 3) for s, t in times.items()}
 
-    pick, ms = sweep(prec, slabs)
+    pick, ms = sweep(prec, slabs, prec, iters)
     out = {"eo_slab": pick, "ms": ms}
     update_slabs = tuple(update_slabs)
     if prec != nat.WL_F64 and update_slabs:
-        out["update_eo_slab"], out["update_ms"] = sweep(nat.WL_F64, update_slabs)
-    del keep
+        out["update_eo_slab"], out["update_ms"] = sweep(nat.WL_F64, update_slabs, nat.WL_F64, update_iters)
     torch.cuda.current_stream().wait_stream(stream)
     return out
 
@@ -187,6 +200,15 @@ def tune_eo(ctx: nat.Context, gauge: nat.Gauge, b: torch.Tensor, x: torch.Tensor
 # 4D workgroup box presets (log2 extents per direction), as in the library
 BOX_SHAPES = {1: (3, 2, 0, 3), 2: (2, 2, 1, 3), 3: (1, 2, 2, 3), 4: (2, 1, 2, 3), 5: (3, 1, 1, 3),
               6: (4, 1, 0, 3), 7: (2, 2, 0, 4), 8: (3, 1, 0, 4)}
+
+
+# LDS box kernel presets (log2 extents per direction), as in the library
+LDS_BOX_SHAPES = {1: (1, 1, 2, 4), 2: (1, 2, 2, 3), 3: (2, 1, 2, 3)}
+
+
+def _lds_box_fits(L, b: int) -> bool:
+    sh = LDS_BOX_SHAPES.get(int(b))
+    return sh is not None and all(int(L[mu]) % (1 << sh[mu]) == 0 for mu in range(4))
 
 
 def _box_fits(L, b: int) -> bool:

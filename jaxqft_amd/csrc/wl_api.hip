@@ -136,6 +136,7 @@ struct wl_ctx {
   int blocks_per_cu[2][3] = {{0, 0, 0}, {0, 0, 0}};  // resident workgroup cap (0 = none)
   int block_shape[2][3] = {{0, 0, 0}, {0, 0, 0}};    // 4D box preset of one-site kernels (0 = none)
   int plane[2][3] = {{0, 0, 0}, {0, 0, 0}};          // x1 march of the LDS plane kernel (0 = off)
+  int lds_box[2][3] = {{0, 0, 0}, {0, 0, 0}};        // LDS box kernel preset (0 = off; F16 SU(3))
   int eo_slab[2][3] = {{0, 0, 0}, {0, 0, 0}};        // x1 slab of the even-odd hop (0 = flat)
   int lds_per_cu = 0;     // bytes of LDS per CU
   int lds_per_block = 0;  // largest LDS allocation of one workgroup
@@ -429,6 +430,37 @@ static bool box_fits(const wl_ctx* c, int shape) {
   return true;
 }
 
+// LDS box kernel presets: log2 extents (x0, x1, x2, x3) of the 256-site box
+static const int kLdsBoxShapes[4][4] = {{0, 0, 0, 0}, {1, 1, 2, 4}, {1, 2, 2, 3}, {2, 1, 2, 3}};
+
+static bool lds_box_fits(const wl_ctx* c, int shape) {
+  if (shape <= 0 || shape > 3 || c->geom.nc != 3) return false;
+  for (int mu = 0; mu < 4; ++mu)
+    if (c->geom.L[mu] % (1 << kLdsBoxShapes[shape][mu]) != 0) return false;
+  return true;
+}
+
+// Splits the box grid (bn boxes per direction) into 8 regions, one per XCD:
+// halving x0, x1 and x2 when the box counts allow (measured best), else any
+// split of 8 over the directions; none (lregions = 0) when the number of
+// boxes is not a multiple of 8.
+static void lds_box_regions(DslashArgs& a) {
+  static const int kSplits[20][4] = {
+      {2, 2, 2, 1}, {8, 1, 1, 1}, {2, 2, 1, 2}, {2, 1, 2, 2}, {1, 2, 2, 2}, {4, 2, 1, 1}, {2, 4, 1, 1},
+      {4, 1, 2, 1}, {1, 4, 2, 1}, {2, 1, 4, 1}, {1, 2, 4, 1}, {1, 8, 1, 1}, {1, 1, 8, 1}, {4, 1, 1, 2},
+      {1, 4, 1, 2}, {1, 1, 4, 2}, {2, 1, 1, 4}, {1, 2, 1, 4}, {1, 1, 2, 4}, {1, 1, 1, 8}};
+  a.lregions = 0;
+  for (int mu = 0; mu < 4; ++mu) a.lsplit[mu] = 1;
+  for (const auto& e : kSplits) {
+    bool ok = true;
+    for (int mu = 0; mu < 4; ++mu) ok = ok && a.bn[mu] % e[mu] == 0;
+    if (!ok) continue;
+    for (int mu = 0; mu < 4; ++mu) a.lsplit[mu] = e[mu];
+    a.lregions = 1;
+    return;
+  }
+}
+
 int wl_ctx_set_tuning(wl_ctx* c, int prec, int multi_rhs, const wl_tuning* t) {
   CHK(check_ctx(c));
   if (!t) return fail(WL_ERR_BAD_ARG, "null tuning");
@@ -449,6 +481,9 @@ int wl_ctx_set_tuning(wl_ctx* c, int prec, int multi_rhs, const wl_tuning* t) {
     return fail(WL_ERR_BAD_ARG, "eo_slab %d must be 0 (flat) or divide L1=%d", t->eo_slab, c->geom.L[1]);
   if (t->slab < 0 || (t->slab > 0 && c->geom.L[1] % t->slab != 0))
     return fail(WL_ERR_BAD_ARG, "slab %d must be 0 (flat) or divide L1=%d", t->slab, c->geom.L[1]);
+  if (t->lds_box < 0 || t->lds_box > 3 || (t->lds_box > 0 && !lds_box_fits(c, t->lds_box)))
+    return fail(WL_ERR_BAD_ARG, "lds_box %d: must be 0 or a preset 1..3 whose extents divide the lattice (SU(3) only)",
+                t->lds_box);
   std::lock_guard<std::mutex> lk(c->mu);
   const int m = multi_rhs ? 1 : 0;
   c->march[m][prec] = t->march;
@@ -459,6 +494,7 @@ int wl_ctx_set_tuning(wl_ctx* c, int prec, int multi_rhs, const wl_tuning* t) {
   c->block_shape[m][prec] = t->block_shape;
   c->plane[m][prec] = t->plane;
   c->eo_slab[m][prec] = t->eo_slab;
+  c->lds_box[m][prec] = t->lds_box;
   if (multi_rhs) c->mrhs[prec] = t->rhs_per_thread;
   return WL_OK;
 }
@@ -488,6 +524,7 @@ int wl_ctx_get_tuning(wl_ctx* c, int prec, int nrhs, wl_tuning* t) {
   t->block_shape = t->march == 1 ? c->block_shape[m][prec] : 0;
   t->plane = c->plane[m][prec];
   t->eo_slab = c->eo_slab[m][prec];
+  t->lds_box = prec == F16 ? c->lds_box[m][prec] : 0;
   return WL_OK;
 }
 
@@ -657,7 +694,7 @@ static int gauge_encoding_eo(wl_gauge* g, int prec, int recon, hipStream_t st, c
 }
 
 // launch geometry once ncol and cols_per_cfg are set
-static void set_launch_shape(const wl_ctx* c, DslashArgs& a, int prec) {
+static void set_launch_shape(const wl_ctx* c, DslashArgs& a, int prec, int recon) {
   a.mrhs = mrhs_for(c, prec, a.cols_per_cfg);
   a.march = a.mrhs > 1 ? march_mrhs(c, prec, a.mrhs) : march_for(c, prec, a.cols_per_cfg);
   a.xcd_remap = c->xcd_remap[a.cols_per_cfg > 1 ? 1 : 0][prec];
@@ -709,6 +746,26 @@ static void set_launch_shape(const wl_ctx* c, DslashArgs& a, int prec) {
     a.ntiles = c->geom.L[0] * (c->geom.L[2] / prow) * a.nseg;
     const int need = (prow + 2) * c->geom.L[3] * (32 * c->geom.nc + 8);
     if (a.lds_bytes < need) a.lds_bytes = need;
+  }
+  // LDS box kernel (F16 SU(3), 12-real links): replaces every shape above
+  a.lbox = 0;
+  const int lb = c->lds_box[a.cols_per_cfg > 1 ? 1 : 0][prec];
+  if (prec == F16 && recon == 12 && lds_box_fits(c, lb)) {
+    a.lbox = 1;
+    a.plane = 0;
+    a.prow = 0;
+    a.box = 0;
+    a.slab = 0;
+    a.slab_tiles = 0;
+    a.march = 1;
+    a.nseg = c->geom.L[0];
+    a.lds_bytes = 0;
+    for (int mu = 0; mu < 4; ++mu) {
+      a.bsh[mu] = kLdsBoxShapes[lb][mu];
+      a.bn[mu] = c->geom.L[mu] >> a.bsh[mu];
+    }
+    a.ntiles = (int)(c->V / NT);
+    lds_box_regions(a);
   }
 }
 
@@ -773,7 +830,7 @@ static int dslash_internal(wl_ctx* c, wl_gauge* g, int op, int prec, int recon, 
   a.col_bytes = col_bytes;
   a.ncol = ncol;
   a.cols_per_cfg = nrhs;
-  set_launch_shape(c, a, prec);
+  set_launch_shape(c, a, prec, rc);
   a.in = in;
   a.out = out;
   hipError_t e = launch_dslash(prec, c->geom.nc, rc, op, 0, a, st);
@@ -977,7 +1034,7 @@ struct Solver {
       a.dir_bytes = a.cfg_bytes / 4;
       a.ncol = ncol;
       a.cols_per_cfg = nrhs;
-      set_launch_shape(c, a, prec);
+      set_launch_shape(c, a, prec, nc == 1 ? 18 : recon);
     }
     a.m0_from_state = 1;
     a.red = red;

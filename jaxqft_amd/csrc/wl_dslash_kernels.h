@@ -853,4 +853,201 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
   if (EPI != EPI_STORE) write_partial(a.red, col, tile, a.ntiles, part);
 }
 
+// LDS box kernel (FP16 storage, SU(3)).  A workgroup owns one box of 256
+// sites (extents 2^bsh, one site per thread) of one configuration and walks
+// all of that configuration's columns, two per pass.  The forward links
+// U_mu(x) of the box's sites (12-real storage) are staged once in LDS: a
+// backward link U_mu(x - mu)
+// inside the box is the forward link of that site, and only backward links
+// across the box's lower faces are read from memory.  Each pass stages the
+// two columns' spinors at the box's own sites in LDS; a neighbour inside the
+// box is then an LDS read, and only neighbours outside it are read from
+// memory.  Each link is rebuilt once per pass and applied to both columns.
+// One barrier after the staging and one after the last read of the pass (the
+// reduction's own barriers serve for the reducing epilogues).  With one
+// workgroup per box and all columns inside it, the links are read once per
+// launch and a box's out-of-box neighbours are read by its neighbour boxes in
+// the same pass of theirs; numbering boxes x3-fastest inside 8 regions of the
+// box grid, one region per XCD (workgroup b runs on XCD b mod 8), keeps those
+// neighbour boxes on the same L2.  51 KiB of LDS per workgroup: three
+// workgroups (wavefronts per SIMD) per CU.
+template <int RECON, bool DAG, bool DIAG, bool G5, int EPI>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_dslash_box(DslashArgs a) {
+  static_assert(RECON == 12, "the box kernel stages 12-real links");
+  constexpr int NC = 3;
+  using T = float;
+  __shared__ uint4 bsp[2][NC][NT];  // [column of the pass][plane][site]
+  __shared__ float bss[2][NT];      // per-site scales
+  __shared__ uint4 blk4[4][NT];     // forward links, words 0..3 of the 12-real storage
+  __shared__ uint2 blk2[4][NT];     // words 4..5
+  const int nbox = a.ntiles;
+  const int cfg = (int)blockIdx.x / nbox;
+  const int jb = (int)blockIdx.x - cfg * nbox;
+  int tt[4];
+  {
+    unsigned r = a.lregions ? ((unsigned)jb & 7u) : 0u;
+    unsigned i = a.lregions ? ((unsigned)jb >> 3) : (unsigned)jb;
+    int rc[4], rn[4];
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) {
+      rn[mu] = a.bn[mu] / a.lsplit[mu];
+      rc[mu] = (int)(r % (unsigned)a.lsplit[mu]);
+      r /= (unsigned)a.lsplit[mu];
+    }
+#pragma unroll
+    for (int mu = 3; mu >= 0; --mu) {
+      tt[mu] = rc[mu] * rn[mu] + (int)(i % (unsigned)rn[mu]);
+      i /= (unsigned)rn[mu];
+    }
+  }
+  const int tid = (int)threadIdx.x;
+  const int sh3 = a.bsh[3], sh2 = a.bsh[2], sh1 = a.bsh[1];
+  const int lstr[4] = {1 << (sh1 + sh2 + sh3), 1 << (sh2 + sh3), 1 << sh3, 1};
+  int x[4], lb[4];
+  lb[3] = tid & ((1 << sh3) - 1);
+  lb[2] = (tid >> sh3) & ((1 << sh2) - 1);
+  lb[1] = (tid >> (sh3 + sh2)) & ((1 << sh1) - 1);
+  lb[0] = tid >> (sh3 + sh2 + sh1);
+  unsigned s = 0;  // the volume is below 2^31
+#pragma unroll
+  for (int mu = 0; mu < 4; ++mu) {
+    x[mu] = (tt[mu] << a.bsh[mu]) + lb[mu];
+    s += (unsigned)x[mu] * (unsigned)a.g.stride[mu];
+  }
+  // neighbour n = 2 mu (forward) or 2 mu + 1 (backward): its site, and its
+  // slot in the box (-1 outside; a box spanning a whole axis wraps in itself)
+  auto nb_site = [&](int n) -> unsigned {
+    const int mu = n >> 1;
+    const unsigned L = (unsigned)a.g.L[mu], str = (unsigned)a.g.stride[mu];
+    if (n & 1) return x[mu] == 0 ? s + (L - 1) * str : s - str;
+    return x[mu] == (int)L - 1 ? s - (L - 1) * str : s + str;
+  };
+  auto nb_slot = [&](int n) -> int {
+    const int mu = n >> 1;
+    const int ext = 1 << a.bsh[mu];
+    const bool full = ext == a.g.L[mu];
+    if (n & 1) return lb[mu] > 0 ? tid - lstr[mu] : (full ? tid + (ext - 1) * lstr[mu] : -1);
+    return lb[mu] + 1 < ext ? tid + lstr[mu] : (full ? tid - (ext - 1) * lstr[mu] : -1);
+  };
+  const size_t V = (size_t)a.g.Vp;
+  const char* Ucfg = reinterpret_cast<const char*>(a.U) + (size_t)cfg * a.cfg_bytes;
+#pragma unroll
+  for (int mu = 0; mu < 4; ++mu) {
+    const uint4* pa = reinterpret_cast<const uint4*>(Ucfg + mu * a.dir_bytes);
+    blk4[mu][tid] = pa[s];
+    blk2[mu][tid] = reinterpret_cast<const uint2*>(pa + V)[s];
+  }
+  // the link of neighbour n: U_mu(x) forward, U_mu(x - mu) backward
+  auto link = [&](int n, hf2 (&u)[NC][NC]) {
+    const int mu = n >> 1;
+    const int slot = (n & 1) ? nb_slot(n) : tid;
+    uint32_t w[6];
+    if (slot >= 0) {
+      const uint4 t = blk4[mu][slot];
+      const uint2 q = blk2[mu][slot];
+      w[0] = t.x;
+      w[1] = t.y;
+      w[2] = t.z;
+      w[3] = t.w;
+      w[4] = q.x;
+      w[5] = q.y;
+    } else {
+      load_link_words<12>(Ucfg + mu * a.dir_bytes, V, nb_site(n), w);
+    }
+    half_link_from_words<12>(w, u);
+  };
+
+  const ColState* st = a.red.cols;
+  const int cpc = a.cols_per_cfg;
+  const int colb = cfg * cpc;
+  const int npass = (cpc + 1) >> 1;
+  constexpr int SF = DAG ? +1 : -1;
+  for (int p = 0; p < npass; ++p) {
+    bool on[2];
+    const char* inb[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int cc = 2 * p + k;
+      on[k] = cc < cpc && !(a.gate && gate_skip_col(a.gate, a.red, colb + cc));
+      inb[k] = reinterpret_cast<const char*>(a.in) + (size_t)(colb + (cc < cpc ? cc : 0)) * a.col_bytes;
+    }
+    if (!on[0] && !on[1]) continue;  // the same for every thread of the workgroup
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (!on[k]) continue;
+      HalfSpinor<NC> h;
+      load_half_spinor<NC>(inb[k], V, s, h);
+#pragma unroll
+      for (int j = 0; j < NC; ++j) bsp[k][j][tid] = make_uint4(h.w[4 * j], h.w[4 * j + 1], h.w[4 * j + 2], h.w[4 * j + 3]);
+      bss[k][tid] = h.s;
+    }
+    __syncthreads();
+    auto get = [&](int k, int slot, HalfSpinor<NC>& h) {
+#pragma unroll
+      for (int j = 0; j < NC; ++j) {
+        const uint4 t = bsp[k][j][slot];
+        h.w[4 * j] = t.x;
+        h.w[4 * j + 1] = t.y;
+        h.w[4 * j + 2] = t.z;
+        h.w[4 * j + 3] = t.w;
+      }
+      h.s = bss[k][slot];
+    };
+    cx<T> acc[2][4 * NC];
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+      for (int q = 0; q < 4 * NC; ++q) acc[k][q] = {T(0), T(0)};
+#define WL_BHOP(MU, N, S, ADJ, BND)                                    \
+  {                                                                   \
+    hf2 u[NC][NC];                                                    \
+    link(N, u);                                                       \
+    const cx<T> ph = {T(a.bc_re[MU]), T(a.bc_im[MU])};                \
+    const int tn = nb_slot(N);                                        \
+    _Pragma("unroll") for (int k = 0; k < 2; ++k) {                   \
+      if (!on[k]) continue;                                           \
+      HalfSpinor<NC> r;                                               \
+      if (tn >= 0) get(k, tn, r);                                     \
+      else load_half_spinor<NC>(inb[k], V, nb_site(N), r);            \
+      hop_half<NC, MU, S, ADJ, true>(u, r, BND, ph, acc[k]);          \
+    }                                                                 \
+  }
+    WL_BHOP(0, 0, SF, false, x[0] == a.g.L[0] - 1)
+    WL_BHOP(0, 1, -SF, true, x[0] == 0)
+    WL_BHOP(1, 2, SF, false, x[1] == a.g.L[1] - 1)
+    WL_BHOP(1, 3, -SF, true, x[1] == 0)
+    WL_BHOP(2, 4, SF, false, x[2] == a.g.L[2] - 1)
+    WL_BHOP(2, 5, -SF, true, x[2] == 0)
+    WL_BHOP(3, 6, SF, false, x[3] == a.g.L[3] - 1)
+    WL_BHOP(3, 7, -SF, true, x[3] == 0)
+#undef WL_BHOP
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (!on[k]) continue;
+      const int col = colb + 2 * p + k;
+      cx<T> res[4 * NC];
+      if (DIAG) {
+        // res = m0 psi(x) + acc, the hops already times -1/2
+        HalfSpinor<NC> c;
+        get(k, tid, c);
+        const float ms = (a.m0_from_state ? T(st[col].m0) : T(a.m0)) * c.s;
+#pragma unroll
+        for (int q = 0; q < 4 * NC; ++q) {
+          const float2 pc = h2f(c.w[q]);
+          res[q] = {fmaf(pc.x, ms, acc[k][q].re), fmaf(pc.y, ms, acc[k][q].im)};
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4 * NC; ++q) res[q] = acc[k][q];
+      }
+      if (G5) apply_g5<T, NC>(res);
+      const double part = site_epilogue<F16, NC, EPI>(a, col, V, (size_t)s, true, res);
+      if (EPI != EPI_STORE) write_partial(a.red, col, jb, nbox, part);
+    }
+    // the staged spinors are read until here (a reducing epilogue has just
+    // passed the reduction's barriers, after this pass's last read)
+    if (EPI == EPI_STORE) __syncthreads();
+  }
+}
+
 }  // namespace wl

@@ -19,6 +19,14 @@ static void launch_k(const DslashArgs& a, dim3 grid, hipStream_t stream) {
       return;
     }
   }
+  if constexpr (P == F16 && NC == 3 && RECON == 12) {
+    if (a.lbox) {
+      // one workgroup per (configuration, box), all columns of the configuration inside it
+      hipLaunchKernelGGL((k_dslash_box<RECON, DAG, DIAG, G5, EPI>), dim3(a.ntiles * (a.ncol / a.cols_per_cfg)),
+                         dim3(NT), 0, stream, a);
+      return;
+    }
+  }
   if constexpr (P == F16) {
     if (a.plane) {
       hipLaunchKernelGGL((k_dslash_plane<NC, RECON, DAG, DIAG, G5, EPI>), grid, dim3(NT), a.lds_bytes, stream, a);

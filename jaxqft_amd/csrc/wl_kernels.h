@@ -108,6 +108,10 @@ struct DslashArgs {
   int box;                // 1: workgroups cover compact 4D boxes (bsh, bn below); overrides slab
   int bsh[4];             // log2 of the box extent per direction
   int bn[4];              // boxes per direction (L_mu >> bsh_mu)
+  int lbox;               // 1: LDS box kernel (F16 SU(3)): a workgroup owns one 256-site box (bsh, bn)
+                          //    of one configuration and walks all of its columns, two per pass
+  int lregions;           // LDS box kernel: 1 when the box grid is split into 8 XCD regions (lsplit)
+  int lsplit[4];          // regions per direction (product 8), each region's boxes run on one XCD
   int cols_outer;         // column order: 0 columns (groups) fastest; n >= 1 blocks of n sweeping the lattice in turn
   int plane;              // 1: LDS-staged plane kernel (F16, one column per thread, marching x1; march = x1 sites)
   int prow;               // plane kernel: x2 rows per workgroup tile (prow * L3 <= NT, prow divides L2)

@@ -147,23 +147,40 @@ __device__ __forceinline__ void load_half_spinor(const void* __restrict__ base, 
 }
 
 // Links as stored half2 (row a, column b); 12-real storage rebuilds row 2 =
-// conj(row0 x row1) with FP32 dot products, rounded once to half.
-template <int NC, int RECON>
-__device__ __forceinline__ void load_half_link(const void* __restrict__ base, size_t V, size_t site,
-                                               hf2 (&u)[NC][NC]) {
-  if constexpr (NC == 1) {
-    u[0][0] = as_hf2(reinterpret_cast<const uint32_t*>(base)[site]);
-  } else if constexpr (RECON == 12) {
+// conj(row0 x row1) with FP32 dot products, rounded once to half.  An SU(3)
+// link is stored as LinkWords<RECON>::n half2 words: rows 0 and 1 (12-real,
+// one uint4 and one uint2 plane) or all nine entries (one word plane each).
+template <int RECON>
+struct LinkWords {
+  static constexpr int n = RECON == 12 ? 6 : 9;
+};
+
+template <int RECON>
+__device__ __forceinline__ void load_link_words(const void* __restrict__ base, size_t V, size_t site,
+                                                uint32_t (&w)[LinkWords<RECON>::n]) {
+  if constexpr (RECON == 12) {
     const uint4* pa = reinterpret_cast<const uint4*>(base);
     const uint2* pb = reinterpret_cast<const uint2*>(pa + V);
     const uint4 t = pa[site];
     const uint2 q = pb[site];
-    u[0][0] = as_hf2(t.x);
-    u[0][1] = as_hf2(t.y);
-    u[0][2] = as_hf2(t.z);
-    u[1][0] = as_hf2(t.w);
-    u[1][1] = as_hf2(q.x);
-    u[1][2] = as_hf2(q.y);
+    w[0] = t.x;
+    w[1] = t.y;
+    w[2] = t.z;
+    w[3] = t.w;
+    w[4] = q.x;
+    w[5] = q.y;
+  } else {
+    const uint32_t* p = reinterpret_cast<const uint32_t*>(base);
+#pragma unroll
+    for (int c = 0; c < 9; ++c) w[c] = p[c * V + site];
+  }
+}
+
+template <int RECON>
+__device__ __forceinline__ void half_link_from_words(const uint32_t (&w)[LinkWords<RECON>::n], hf2 (&u)[3][3]) {
+  if constexpr (RECON == 12) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) u[c / 3][c % 3] = as_hf2(w[c]);
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       const int k = (j + 1) % 3, l = (j + 2) % 3;
@@ -176,9 +193,20 @@ __device__ __forceinline__ void load_half_link(const void* __restrict__ base, si
       u[2][j] = hf2{(_Float16)re, (_Float16)(-im)};
     }
   } else {
-    const uint32_t* p = reinterpret_cast<const uint32_t*>(base);
 #pragma unroll
-    for (int c = 0; c < 9; ++c) u[c / 3][c % 3] = as_hf2(p[c * V + site]);
+    for (int c = 0; c < 9; ++c) u[c / 3][c % 3] = as_hf2(w[c]);
+  }
+}
+
+template <int NC, int RECON>
+__device__ __forceinline__ void load_half_link(const void* __restrict__ base, size_t V, size_t site,
+                                               hf2 (&u)[NC][NC]) {
+  if constexpr (NC == 1) {
+    u[0][0] = as_hf2(reinterpret_cast<const uint32_t*>(base)[site]);
+  } else {
+    uint32_t w[LinkWords<RECON>::n];
+    load_link_words<RECON>(base, V, site, w);
+    half_link_from_words<RECON>(w, u);
   }
 }
 
