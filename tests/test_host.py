@@ -168,18 +168,63 @@ def test_point_sources():
         assert src[k, 1, 0, 1, 3, k // 3, k % 3] == 1
 
 
+def _explicit_wilson(U, psi, L, mass, bc, dag):
+    """Textbook Wilson operator by a loop over sites, independent of the
+    oracle's rolls and sparse gamma tables:
+      (D psi)(x) = (m+4) psi(x) - 1/2 sum_mu [ (1 - s g_mu) U~_mu(x) psi(x+mu)
+                                             + (1 + s g_mu) U~_mu(x-mu)^dag psi(x-mu) ]
+    with s = +1 for D and -1 for D^dag (= g5 D g5), dense gamma matrices
+    g0 = s1 x s1, g1 = s1 x s2, g2 = s2 x 1, g3 = s3 x 1, and U~_mu(x) =
+    p_mu U_mu(x) on the slice x_mu = L_mu - 1."""
+    s1 = np.array([[0, 1], [1, 0]], complex)
+    s2 = np.array([[0, -1j], [1j, 0]], complex)
+    s3 = np.array([[1, 0], [0, -1]], complex)
+    g = [np.kron(s1, s1), np.kron(s1, s2), np.kron(s2, np.eye(2)), np.kron(s3, np.eye(2))]
+    sg = -1.0 if dag else 1.0
+    nc = psi.shape[-1]
+    out = np.zeros_like(psi)
+    for x in np.ndindex(*L):
+        acc = (mass + 4.0) * psi[(0, *x)]
+        for mu in range(4):
+            xf, xb = list(x), list(x)
+            xf[mu] = (x[mu] + 1) % L[mu]
+            xb[mu] = (x[mu] - 1) % L[mu]
+            uf = np.asarray(U[(0, mu, *x)]).reshape(nc, nc)
+            ub = np.asarray(U[(0, mu, *xb)]).reshape(nc, nc)
+            if x[mu] == L[mu] - 1:
+                uf = bc[mu] * uf
+            if xb[mu] == L[mu] - 1:
+                ub = bc[mu] * ub
+            fwd = (np.eye(4) - sg * g[mu]) @ (uf @ psi[(0, *xf)].T).T
+            bwd = (np.eye(4) + sg * g[mu]) @ (np.conj(ub.T) @ psi[(0, *xb)].T).T
+            acc = acc - 0.5 * (fwd + bwd)
+        out[(0, *x)] = acc
+    return out
+
+
 @pytest.mark.parametrize("name", ["op_su3_2x2x2x2", "op_su3_4x4x4x4", "op_su3_2x3x4x5", "op_u1_2x2x2x2",
                                   "op_u1_4x4x4x4"])
-def test_golden_operator_fixtures_reproduce(name):
+def test_golden_operator_fixtures_explicit(name):
+    """The committed operator fixtures (which the GPU tests compare against)
+    equal the explicit site-loop formula above: D and D^dag directly, g5 D
+    with g5 = s1 x s3, D^dag D as the explicit D^dag of the explicit D psi,
+    and the hopping terms as D - (m+4) and D^dag - (m+4)."""
     z = np.load(GOLD / f"{name}.npz")
     L = tuple(int(v) for v in z["L"])
-    nc = z["psi"].shape[-1]
-    ora = wo.WilsonOracle(L, nc, float(z["mass"]), fermion_bc=tuple(z["bc"]))
+    mass = float(z["mass"])
+    bc = tuple(complex(b) for b in z["bc"])
     U, psi = z["U"], z["psi"]
-    for key, fn in (("D", ora.apply_D), ("Ddag", ora.apply_Ddag), ("g5D", ora.apply_g5D),
-                    ("DdagD", ora.apply_normal), ("K", ora.apply_offdiag), ("Kdag", ora.apply_offdiag_dagger)):
-        ref = z[key]
-        assert np.linalg.norm(fn(U, psi) - ref) / np.linalg.norm(ref) < 1e-14, key
+    D = _explicit_wilson(U, psi, L, mass, bc, dag=False)
+    Dd = _explicit_wilson(U, psi, L, mass, bc, dag=True)
+    g5 = np.kron(np.array([[0, 1], [1, 0]]), np.diag([1, -1]))
+    expect = {"D": D, "Ddag": Dd, "g5D": np.einsum("st,...tc->...sc", g5, D),
+              "DdagD": _explicit_wilson(U, D, L, mass, bc, dag=True),
+              "K": D - (mass + 4.0) * psi, "Kdag": Dd - (mass + 4.0) * psi}
+    for key, ref in expect.items():
+        assert np.linalg.norm(z[key] - ref) / np.linalg.norm(ref) < 1e-14, key
+    # the oracle reproduces the fixtures bit for bit up to rounding
+    ora = wo.WilsonOracle(L, psi.shape[-1], mass, fermion_bc=bc)
+    assert np.linalg.norm(ora.apply_D(U, psi) - z["D"]) / np.linalg.norm(z["D"]) < 1e-14
 
 
 def test_golden_cg_fixture():
@@ -196,3 +241,23 @@ def test_golden_index_tables(L):
     for k in t:
         np.testing.assert_array_equal(t[k], z[k])
         assert t[k].dtype == np.int32
+
+
+def test_gamma_algebra_checker_residuals():
+    """check_gamma_algebra: zero on the exact bases of every dimension; a
+    basis with g1 scaled by (1 + e) reports (1 + e)^2 - 1 (its own
+    anticommutator is 2 (1 + e)^2); a non-Hermitian entry of size d reports
+    hermiticity error sqrt(2) d."""
+    from jaxqft_amd.fermions.gamma import build_euclidean_gamma, check_gamma_algebra
+
+    for d in (2, 3, 4, 5, 6):
+        r = check_gamma_algebra(build_euclidean_gamma(d))
+        assert r["max_rel_clifford_error"] == 0.0 and r["max_hermiticity_error"] == 0.0
+    g = build_euclidean_gamma(4)
+    g[1] = g[1] * 1.001
+    r = check_gamma_algebra(g)
+    assert abs(r["max_rel_clifford_error"] - (1.001 ** 2 - 1)) < 1e-12
+    assert r["max_hermiticity_error"] == 0.0
+    g = build_euclidean_gamma(4)
+    g[2, 0, 1] += 1e-3j
+    assert abs(check_gamma_algebra(g)["max_hermiticity_error"] - np.sqrt(2) * 1e-3) < 1e-15
