@@ -163,7 +163,7 @@ def measured_traffic(args, k, L, tuning):
     want = {"lattice": list(L), "nrhs": k, "inner": args.inner, "recon": args.recon}
     if any(d.get(key) != val for key, val in want.items()):
         return None, None
-    keys = ("march", "rhs_per_thread", "xcd_remap", "slab", "block_shape", "cols_outer", "plane")
+    keys = ("march", "rhs_per_thread", "xcd_remap", "slab", "slab_x2", "block_shape", "cols_outer", "plane")
     for e in d.get("entries", []):
         t = e.get("tuning", {})
         if all(int(t.get(key, 0)) == int(tuning.get(key, 0)) for key in keys):
@@ -303,7 +303,7 @@ def main():
             gbs = bps * V / (best[1] * 1e-6) / 1e9
             dslash_k[f"{prec}_r{recon}"] = {
                 "nrhs": k, "rhs_per_thread": best[0], "march": tk["march"], "xcd_remap": tk["xcd_remap"],
-                "slab": tk["slab"], "block_shape": tk["block_shape"],
+                "slab": tk["slab"], "slab_x2": tk["slab_x2"], "block_shape": tk["block_shape"],
                 "us": round(best[1], 2), "us_per_rhs": round(best[1] / k, 2),
                 "GB/s": round(gbs, 1), "frac_hbm_peak": round(gbs / PEAK_HBM_GBS, 4), "bytes_per_site": bps,
                 "tune_us": tried,

@@ -200,6 +200,11 @@ typedef struct wl_tuning {
                          neighbours read from LDS; box extents (x0,x1,x2,x3) 1 (2,2,4,16),
                          2 (2,4,4,8), 3 (4,2,4,8), which must divide the lattice; takes
                          precedence over every other shape setting */
+  int slab_x2;        /* slab order (slab > 0): 0 = a slab spans whole x1 planes; t > 0
+                         (must divide L2) = a slab spans t consecutive x2 values, so the
+                         slabs are (slab x t x L3) tiles walked x0-fastest, then by x2
+                         block, then by x1 block: each XCD sweeps x0 over a compact
+                         (x1, x2) block, whose x0 neighbours stay in its L2 */
 } wl_tuning;
 /* wl_ctx_get_tuning reports the values in effect for nrhs right-hand sides. */
 int wl_ctx_set_tuning(wl_ctx* ctx, int prec, int multi_rhs, const wl_tuning* t);

@@ -70,7 +70,7 @@ class WlSolveStats(ctypes.Structure):
 class WlTuning(ctypes.Structure):
     _fields_ = [("march", c_int), ("rhs_per_thread", c_int), ("xcd_remap", c_int), ("slab", c_int),
                 ("cols_outer", c_int), ("block_shape", c_int), ("blocks_per_cu", c_int), ("plane", c_int),
-                ("eo_slab", c_int), ("lds_box", c_int)]
+                ("eo_slab", c_int), ("lds_box", c_int), ("slab_x2", c_int)]
 
 
 class WlProfile(ctypes.Structure):
@@ -197,9 +197,10 @@ class Context:
 
     def set_tuning(self, prec: int, multi_rhs: bool, march: int = 0, rhs_per_thread: int = 0,
                    xcd_remap: bool = True, slab: int = 0, cols_outer: int = 0, blocks_per_cu: int = 0,
-                   block_shape: int = 0, plane: int = 0, eo_slab: int = 0, lds_box: int = 0) -> None:
+                   block_shape: int = 0, plane: int = 0, eo_slab: int = 0, lds_box: int = 0,
+                   slab_x2: int = 0) -> None:
         t = WlTuning(int(march), int(rhs_per_thread), 1 if xcd_remap else 0, int(slab), int(cols_outer),
-                     int(block_shape), int(blocks_per_cu), int(plane), int(eo_slab), int(lds_box))
+                     int(block_shape), int(blocks_per_cu), int(plane), int(eo_slab), int(lds_box), int(slab_x2))
         check(self._lib.wl_ctx_set_tuning(self.h, int(prec), 1 if multi_rhs else 0, byref(t)))
 
     def set_eo_slab(self, prec: int, multi_rhs: bool, eo_slab: int) -> None:
@@ -210,7 +211,8 @@ class Context:
         check(self._lib.wl_ctx_get_tuning(self.h, int(prec), int(nrhs), byref(t)))
         return {"march": t.march, "rhs_per_thread": t.rhs_per_thread, "xcd_remap": bool(t.xcd_remap),
                 "slab": t.slab, "cols_outer": int(t.cols_outer), "blocks_per_cu": t.blocks_per_cu,
-                "block_shape": t.block_shape, "plane": t.plane, "eo_slab": t.eo_slab, "lds_box": t.lds_box}
+                "block_shape": t.block_shape, "plane": t.plane, "eo_slab": t.eo_slab, "lds_box": t.lds_box,
+                "slab_x2": t.slab_x2}
 
     # single-RHS shortcuts
     def set_march(self, prec: int, march: int) -> None:

@@ -5,9 +5,9 @@ field with HIP events on the caller's stream and installs the fastest in the
 context.  The tunable is the number of x0 sites each thread marches over
 (``march``: 1 selects the one-site-per-thread kernel), for multi-RHS
 launches the number of right-hand sides sharing each link load in a thread,
-the site order of the one-site kernels (flat, x1 slabs walked x0-fastest, or
-compact 4D workgroup boxes) and the workgroup order (XCD-contiguous or
-hardware order).
+the site order of the one-site kernels (flat, x1 slabs or (x1, x2) tiles
+walked x0-fastest, or compact 4D workgroup boxes) and the workgroup order
+(XCD-contiguous or hardware order).
 """
 
 from __future__ import annotations
@@ -42,33 +42,38 @@ def tune(ctx: nat.Context, gauge: nat.Gauge, prec: int, recon: int, fin: nat.Fie
          stream: Optional[torch.cuda.Stream] = None, slabs: Iterable[int] = (0, 2, 4, 8),
          cols_outer: Iterable[int] = (0, 2, 3), boxes: Iterable[int] = (1, 2, 7),
          mrhs_marches: Iterable[int] = (2, 4, 8, 16), planes: Iterable[int] = (2, 4, 8),
-         lds_boxes: Iterable[int] = (1, 2, 3)) -> Dict[str, object]:
+         lds_boxes: Iterable[int] = (1, 2, 3),
+         tiles: Iterable = ((16, 8),)) -> Dict[str, object]:
     """Pick the fastest launch shape for (prec, recon) at nrhs right-hand sides
     per configuration and install it.  Candidates: every march length dividing
     L0 with one right-hand side per thread, plus 2 and 4 right-hand sides per
     thread when nrhs > 1 (for FP16 also two per thread marching the lengths of
     `mrhs_marches` along x0, and the LDS-staged plane kernel marching the
     lengths of `planes` along x1); the one-site-per-thread shapes in flat and x1-slab
-    site order (slab extents dividing L1); each in XCD-contiguous and hardware
+    site order (slab extents dividing L1) and in the (x1, x2) tile orders of
+    `tiles` ((slab, slab_x2) pairs dividing L1 and L2, walked x0-fastest so
+    that each XCD sweeps x0 over a compact block); each in XCD-contiguous and hardware
     workgroup order, and for nrhs > 1 in each column order of `cols_outer`
     (0: the columns of a tile back to back; n: blocks of n columns, or column
     groups, sweeping the lattice in turn, which keeps a block's spinor
     neighbours in L2 while its links stay shared; sweeping one column at a
     time, n = 1, re-reads the links and measured slower at 32^4).
-    Returns {"march", "rhs_per_thread", "xcd_remap", "slab", "block_shape", "plane", "lds_box",
+    Returns {"march", "rhs_per_thread", "xcd_remap", "slab", "slab_x2", "block_shape", "plane", "lds_box",
     "cols_outer", "us": {label: us}}."""
     stream = stream or torch.cuda.current_stream()
-    L0, L1 = ctx.lattice_shape[0], ctx.lattice_shape[1]
+    L0, L1, L2 = ctx.lattice_shape[0], ctx.lattice_shape[1], ctx.lattice_shape[2]
     multi = nrhs > 1
     gauge.prepare(prec, recon)
-    # site orders of the one-site kernels: ("s", slab) or ("b", box preset)
-    orders1 = [("s", s) for s in slabs if s == 0 or (s < L1 and L1 % s == 0)]
+    # site orders of the one-site kernels: ("s", (slab, slab_x2)) or ("b", box preset)
+    orders1 = [("s", (s, 0)) for s in slabs if s == 0 or (s < L1 and L1 % s == 0)]
+    orders1 += [("s", (int(a), int(c))) for a, c in tiles
+                if a <= L1 and L1 % a == 0 and c < L2 and L2 % c == 0]
     orders1 += [("b", b) for b in boxes if _box_fits(ctx.lattice_shape, b)]
     shapes = []
     for m in marches:
         if L0 % m:
             continue
-        shapes += [(m, 1, o) for o in (orders1 if m == 1 else [("s", 0)])]
+        shapes += [(m, 1, o) for o in (orders1 if m == 1 else [("s", (0, 0))])]
     if prec == nat.WL_F16:
         # the LDS-staged plane kernel, marching the lengths of `planes` along x1
         L1v, L3v = ctx.lattice_shape[1], ctx.lattice_shape[3]
@@ -78,7 +83,7 @@ def tune(ctx: nat.Context, gauge: nat.Gauge, prec: int, recon: int, fin: nat.Fie
         shapes += [(1, k, o) for k in (2, 4) if nrhs % k == 0 for o in orders1]
         if prec == nat.WL_F16 and nrhs % 2 == 0:
             # the FP16 two-RHS kernel marching along x0
-            shapes += [(m, 2, ("s", 0)) for m in mrhs_marches if m > 1 and L0 % m == 0]
+            shapes += [(m, 2, ("s", (0, 0))) for m in mrhs_marches if m > 1 and L0 % m == 0]
     if prec == nat.WL_F16 and ctx.nc == 3 and recon == 12:
         # the LDS box kernel (two columns per pass; 12-real links)
         shapes += [(1, 2 if multi else 1, ("l", b)) for b in lds_boxes if _lds_box_fits(ctx.lattice_shape, b)]
@@ -87,8 +92,9 @@ def tune(ctx: nat.Context, gauge: nat.Gauge, prec: int, recon: int, fin: nat.Fie
     eo_slab = ctx.get_tuning(prec, nrhs)["eo_slab"]  # kept: the even-odd hop is tuned on its own
 
     def install(m, k, r, o, co):
-        ctx.set_tuning(prec, multi, m, k if multi else 0, bool(r), o[1] if o[0] == "s" else 0, co, 0,
-                       o[1] if o[0] == "b" else 0, o[1] if o[0] == "p" else 0, eo_slab, o[1] if o[0] == "l" else 0)
+        ctx.set_tuning(prec, multi, m, k if multi else 0, bool(r), o[1][0] if o[0] == "s" else 0, co, 0,
+                       o[1] if o[0] == "b" else 0, o[1] if o[0] == "p" else 0, eo_slab, o[1] if o[0] == "l" else 0,
+                       o[1][1] if o[0] == "s" else 0)
 
     times = {}
     for co in corders:
@@ -99,7 +105,8 @@ def tune(ctx: nat.Context, gauge: nat.Gauge, prec: int, recon: int, fin: nat.Fie
                 install(m, k, r, o, co)
                 times[(m, k, r, o, co)] = time_dslash(ctx, gauge, prec, recon, fin, fout, nrhs, reps, stream)
     def as_dict(m, k, r, o, co):
-        return {"march": m, "rhs_per_thread": k, "xcd_remap": r, "slab": o[1] if o[0] == "s" else 0,
+        return {"march": m, "rhs_per_thread": k, "xcd_remap": r, "slab": o[1][0] if o[0] == "s" else 0,
+                "slab_x2": o[1][1] if o[0] == "s" else 0,
                 "block_shape": o[1] if o[0] == "b" else 0, "plane": o[1] if o[0] == "p" else 0,
                 "lds_box": o[1] if o[0] == "l" else 0, "cols_outer": co}
 
@@ -109,7 +116,14 @@ def tune(ctx: nat.Context, gauge: nat.Gauge, prec: int, recon: int, fin: nat.Fie
     m, k, r, o, co = best
     return {**as_dict(*best),
             "ranked": [as_dict(*c) for c in ranked],
-            "us": {f"m{m}k{k}x{r}{o[0]}{o[1]}" + (f"c{co}" if co else ""): t for (m, k, r, o, co), t in times.items()}}
+            "us": {f"m{m}k{k}x{r}{_label(o)}" + (f"c{co}" if co else ""): t for (m, k, r, o, co), t in times.items()}}
+
+
+def _label(o) -> str:
+    """Short name of a site order: s<slab>[t<slab_x2>], b<box>, p<plane>, l<lds box>."""
+    if o[0] == "s":
+        return f"s{o[1][0]}" + (f"t{o[1][1]}" if o[1][1] else "")
+    return f"{o[0]}{o[1]}"
 
 
 def install_tuning(ctx: nat.Context, prec: int, multi: bool, t: Dict[str, object]) -> None:
@@ -117,7 +131,7 @@ def install_tuning(ctx: nat.Context, prec: int, multi: bool, t: Dict[str, object
     eo = ctx.get_tuning(prec, 2 if multi else 1)["eo_slab"]
     ctx.set_tuning(prec, multi, int(t["march"]), int(t["rhs_per_thread"]) if multi else 0, bool(t["xcd_remap"]),
                    int(t["slab"]), int(t["cols_outer"]), 0, int(t["block_shape"]), int(t["plane"]), eo,
-                   int(t.get("lds_box", 0)))
+                   int(t.get("lds_box", 0)), int(t.get("slab_x2", 0)))
 
 
 def set_eo_slab(ctx: nat.Context, prec: int, multi: bool, eo_slab: int) -> None:
@@ -220,5 +234,6 @@ def tune_march(ctx, gauge, prec, recon, fin, fout, nrhs: int = 1, candidates=(1,
                stream=None) -> Dict[str, object]:
     """Single-RHS form of `tune`: {"best": march, "xcd_remap": r, "us": ...}."""
     t = tune(ctx, gauge, prec, recon, fin, fout, nrhs, candidates, reps, stream)
-    return {"best": t["march"], "xcd_remap": t["xcd_remap"], "slab": t["slab"], "block_shape": t["block_shape"],
+    return {"best": t["march"], "xcd_remap": t["xcd_remap"], "slab": t["slab"], "slab_x2": t["slab_x2"],
+            "block_shape": t["block_shape"],
             "plane": t["plane"], "us": t["us"]}

@@ -132,6 +132,7 @@ struct wl_ctx {
   int mrhs[3] = {0, 0, 0};                       // right-hand sides per thread (0 = automatic)
   int xcd_remap[2][3] = {{1, 1, 1}, {1, 1, 1}};  // XCD-contiguous workgroup order
   int slab[2][3] = {{0, 0, 0}, {0, 0, 0}};       // x1 extent of a slab (0 = flat site order)
+  int slab2[2][3] = {{0, 0, 0}, {0, 0, 0}};      // x2 extent of a slab (0 = whole x1 planes)
   int cols_outer[2][3] = {{0, 0, 0}, {0, 0, 0}}; // column order (0 fastest, n >= 1 blocks of n columns or groups)
   int blocks_per_cu[2][3] = {{0, 0, 0}, {0, 0, 0}};  // resident workgroup cap (0 = none)
   int block_shape[2][3] = {{0, 0, 0}, {0, 0, 0}};    // 4D box preset of one-site kernels (0 = none)
@@ -266,19 +267,19 @@ static int mrhs_for(const wl_ctx* c, int prec, int nrhs) {
   return (k > 1 && nrhs % k == 0) ? k : 1;
 }
 
-// tiles of a one-site-per-thread launch: flat, or L0 * (L1 / slab) slabs of
-// ceil(slab * L2 * L3 / NT) tiles
-static inline int site_tiles(const wl_ctx* c, int slab) {
+// tiles of a one-site-per-thread launch: flat, or L0 * (L1 / slab) * (L2 /
+// slab2) slabs of ceil(slab * slab2 * L3 / NT) tiles
+static inline int site_tiles(const wl_ctx* c, int slab, int slab2) {
   if (slab <= 0) return (int)((c->V + NT - 1) / NT);
-  const long long plane = (long long)c->geom.L[2] * c->geom.L[3];
-  return c->geom.L[0] * (c->geom.L[1] / slab) * (int)((slab * plane + NT - 1) / NT);
+  const long long sites = (long long)slab * slab2 * c->geom.L[3];
+  return c->geom.L[0] * (c->geom.L[1] / slab) * (c->geom.L[2] / slab2) * (int)((sites + NT - 1) / NT);
 }
 
 // upper bound of reduction partials per column over every kernel (a slab
-// order adds at most one partial tile per slab, i.e. at most L0 * L1)
+// order adds at most one partial tile per slab, i.e. at most L0 * L1 * L2)
 static inline size_t max_tiles(const wl_ctx* c) {
   size_t a = (size_t)nchunk_of(c) * c->geom.L[0];
-  size_t b = (size_t)((c->V + NT - 1) / NT) + (size_t)c->geom.L[0] * c->geom.L[1];
+  size_t b = (size_t)((c->V + NT - 1) / NT) + (size_t)c->geom.L[0] * c->geom.L[1] * c->geom.L[2];
   // the plane kernel: at most one tile per (x0, x1, x2)
   const size_t d = (size_t)c->geom.L[0] * c->geom.L[1] * c->geom.L[2];
   if (b > a) a = b;
@@ -481,6 +482,8 @@ int wl_ctx_set_tuning(wl_ctx* c, int prec, int multi_rhs, const wl_tuning* t) {
     return fail(WL_ERR_BAD_ARG, "eo_slab %d must be 0 (flat) or divide L1=%d", t->eo_slab, c->geom.L[1]);
   if (t->slab < 0 || (t->slab > 0 && c->geom.L[1] % t->slab != 0))
     return fail(WL_ERR_BAD_ARG, "slab %d must be 0 (flat) or divide L1=%d", t->slab, c->geom.L[1]);
+  if (t->slab_x2 < 0 || (t->slab_x2 > 0 && c->geom.L[2] % t->slab_x2 != 0))
+    return fail(WL_ERR_BAD_ARG, "slab_x2 %d must be 0 (whole x1 planes) or divide L2=%d", t->slab_x2, c->geom.L[2]);
   if (t->lds_box < 0 || t->lds_box > 3 || (t->lds_box > 0 && !lds_box_fits(c, t->lds_box)))
     return fail(WL_ERR_BAD_ARG, "lds_box %d: must be 0 or a preset 1..3 whose extents divide the lattice (SU(3) only)",
                 t->lds_box);
@@ -489,6 +492,7 @@ int wl_ctx_set_tuning(wl_ctx* c, int prec, int multi_rhs, const wl_tuning* t) {
   c->march[m][prec] = t->march;
   c->xcd_remap[m][prec] = t->xcd_remap ? 1 : 0;
   c->slab[m][prec] = t->slab;
+  c->slab2[m][prec] = t->slab_x2;
   c->cols_outer[m][prec] = t->cols_outer;
   c->blocks_per_cu[m][prec] = t->blocks_per_cu;
   c->block_shape[m][prec] = t->block_shape;
@@ -519,6 +523,7 @@ int wl_ctx_get_tuning(wl_ctx* c, int prec, int nrhs, wl_tuning* t) {
   t->rhs_per_thread = k;
   t->xcd_remap = c->xcd_remap[m][prec];
   t->slab = t->march == 1 ? c->slab[m][prec] : 0;
+  t->slab_x2 = t->slab ? c->slab2[m][prec] : 0;
   t->cols_outer = c->cols_outer[m][prec];
   t->blocks_per_cu = c->blocks_per_cu[m][prec];
   t->block_shape = t->march == 1 ? c->block_shape[m][prec] : 0;
@@ -700,6 +705,10 @@ static void set_launch_shape(const wl_ctx* c, DslashArgs& a, int prec, int recon
   a.xcd_remap = c->xcd_remap[a.cols_per_cfg > 1 ? 1 : 0][prec];
   a.slab = a.march == 1 ? c->slab[a.cols_per_cfg > 1 ? 1 : 0][prec] : 0;
   {
+    const int s2 = c->slab2[a.cols_per_cfg > 1 ? 1 : 0][prec];
+    a.slab2 = s2 > 0 ? s2 : c->geom.L[2];
+  }
+  {
     const int bs = a.march == 1 ? c->block_shape[a.cols_per_cfg > 1 ? 1 : 0][prec] : 0;
     a.box = box_fits(c, bs) ? 1 : 0;
     for (int mu = 0; mu < 4; ++mu) {
@@ -728,9 +737,9 @@ static void set_launch_shape(const wl_ctx* c, DslashArgs& a, int prec, int recon
     }
     a.lds_bytes = (int)b;
   }
-  a.slab_tiles = a.slab ? (int)(((long long)a.slab * c->geom.L[2] * c->geom.L[3] + NT - 1) / NT) : 0;
+  a.slab_tiles = a.slab ? (int)(((long long)a.slab * a.slab2 * c->geom.L[3] + NT - 1) / NT) : 0;
   a.nseg = c->geom.L[0] / a.march;
-  a.ntiles = a.march == 1 ? (a.box ? (int)(c->V / NT) : site_tiles(c, a.slab)) : nchunk_of(c) * a.nseg;
+  a.ntiles = a.march == 1 ? (a.box ? (int)(c->V / NT) : site_tiles(c, a.slab, a.slab2)) : nchunk_of(c) * a.nseg;
   // LDS-staged plane kernel (FP16, one column per thread): replaces the shape above
   a.plane = 0;
   a.prow = 0;
@@ -955,6 +964,10 @@ int wl_field_import(wl_field* f, const void* src, int dtype, void* stream) {
   if (!f || !src) return fail(WL_ERR_BAD_ARG, "null argument");
   if (dtype != WL_C128 && dtype != WL_C64) return fail(WL_ERR_BAD_ARG, "bad dtype %d", dtype);
   wl_ctx* c = f->ctx;
+  // ordered with the context's other calls like every call on it (a field
+  // written by a stencil on one stream and read here on another)
+  std::lock_guard<std::mutex> lk(c->mu);
+  StreamOrder order(c, (hipStream_t)stream);
   hipError_t e = pack_spinor(f->prec, c->geom.nc, src, dtype, f->buf.p, c->V, c->Vp, f->ncol, f->col_bytes, (hipStream_t)stream);
   if (e != hipSuccess) return fail(WL_ERR_HIP, "import: %s", hipGetErrorString(e));
   return WL_OK;
@@ -964,6 +977,8 @@ int wl_field_export(const wl_field* f, void* dst, int dtype, void* stream) {
   if (!f || !dst) return fail(WL_ERR_BAD_ARG, "null argument");
   if (dtype != WL_C128 && dtype != WL_C64) return fail(WL_ERR_BAD_ARG, "bad dtype %d", dtype);
   wl_ctx* c = f->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  StreamOrder order(c, (hipStream_t)stream);
   hipError_t e = unpack_spinor(f->prec, c->geom.nc, f->buf.p, dst, dtype, c->V, c->Vp, f->ncol, f->col_bytes, (hipStream_t)stream);
   if (e != hipSuccess) return fail(WL_ERR_HIP, "export: %s", hipGetErrorString(e));
   return WL_OK;

@@ -37,11 +37,13 @@ __device__ __forceinline__ void store_nt(void* base, size_t V, size_t site, cons
 
 // Site of this thread in `tile` for the one-site-per-thread kernels.  Flat
 // order: tile t covers sites [t NT, (t+1) NT).  Slab order: a slab is the
-// sites with one x0 and `slab` consecutive x1 values (slab * L2 * L3 sites,
-// ceil(./NT) tiles); slabs are numbered x0-fastest, so a tile's x0
-// neighbours are read by the tiles of the adjacent slabs moments before or
-// after it, and each XCD (contiguous tile range) sweeps x0 over its own x1
-// range.  `valid` is false for the padding threads of a slab's last tile.
+// sites with one x0, `slab` consecutive x1 values and `slab2` consecutive x2
+// values (slab * slab2 * L3 sites in x1, x2, x3 order, ceil(./NT) tiles);
+// slabs are numbered x0-fastest, then by x2 block, then by x1 block, so a
+// tile's x0 neighbours are read by the tiles of the adjacent slabs moments
+// before or after it, and each XCD (contiguous tile range) sweeps x0 over its
+// own (x1, x2) block.  slab2 = L2 gives slabs of whole x1 planes.  `valid` is
+// false for the padding threads of a slab's last tile.
 //
 // Box order: tile t is the 4D box (t0, t1, t2, t3) of extents 2^bsh (t3
 // fastest); thread (l0, l1, l2, l3) of it, l3 fastest, so a wavefront reads
@@ -56,13 +58,20 @@ __device__ __forceinline__ size_t site_of_tile_linear(const DslashArgs& a, int t
   const int sl = tile / a.slab_tiles;
   const int sub = tile - sl * a.slab_tiles;
   const int x0 = sl % a.g.L[0];
-  const int x1b = sl / a.g.L[0];
-  const unsigned plane = (unsigned)a.g.stride[1];  // L2 * L3
+  const int r = sl / a.g.L[0];
+  const int nb2 = a.g.L[2] / a.slab2;
+  const int b1 = r / nb2;
+  const int b2 = r - b1 * nb2;
+  const unsigned L3 = (unsigned)a.g.L[3];
+  const unsigned row = (unsigned)a.slab2 * L3;  // sites of one x1 value of the slab
   const unsigned i = (unsigned)sub * NT + threadIdx.x;
-  valid = i < (unsigned)a.slab * plane;
+  valid = i < (unsigned)a.slab * row;
   const unsigned ii = valid ? i : 0u;
-  const unsigned q = ii / plane;
-  return (size_t)x0 * (size_t)a.g.stride[0] + (size_t)(x1b * a.slab + (int)q) * plane + (ii - q * plane);
+  const unsigned q = ii / row;
+  const unsigned rem = ii - q * row;
+  const unsigned l2 = rem / L3;
+  return (size_t)x0 * (size_t)a.g.stride[0] + (size_t)(b1 * a.slab + (int)q) * (size_t)a.g.stride[1] +
+         (size_t)((unsigned)(b2 * a.slab2) + l2) * L3 + (rem - l2 * L3);
 }
 
 __device__ __forceinline__ size_t site_of_tile(const DslashArgs& a, int tile, bool& valid, int (&x)[4]) {

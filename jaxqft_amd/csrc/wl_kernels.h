@@ -105,6 +105,7 @@ struct DslashArgs {
   int mrhs;               // right-hand sides per thread (1, 2 or 4; > 1 implies one site per thread)
   int slab;               // one site per thread: 0 flat order, else x1 extent of a slab (x0-fastest slabs)
   int slab_tiles;         // tiles per slab
+  int slab2;              // x2 extent of a slab (divides L2; L2 = whole x1 planes)
   int box;                // 1: workgroups cover compact 4D boxes (bsh, bn below); overrides slab
   int bsh[4];             // log2 of the box extent per direction
   int bn[4];              // boxes per direction (L_mu >> bsh_mu)

@@ -295,15 +295,18 @@ def test_low_precision_complex_bc(nc, recon, prec, tol, nrhs, per_thread):
 
 
 @pytest.mark.parametrize("prec,recon,tol", [("f64", 18, 1e-13), ("f32", 12, 2e-6), ("f16", 12, 3e-3)])
-@pytest.mark.parametrize("per_thread,slab,cols_outer", [(1, 0, 0), (2, 0, 0), (4, 0, 0), (2, 2, 0), (4, 4, 0),
-                                                      (1, 2, 1), (2, 0, 1), (1, 4, 2), (1, 0, 4), (2, 0, 3),
-                                                      (2, 4, 2)])
-def test_multi_rhs_stencil(prec, recon, tol, per_thread, slab, cols_outer):
-    """k right-hand sides sharing each link load (1, 2 or 4 per thread; flat or
-    x1-slab site order; the columns of a tile back to back, or blocks of 1, 2,
+@pytest.mark.parametrize("per_thread,slab,cols_outer,slab_x2", [(1, 0, 0, 0), (2, 0, 0, 0), (4, 0, 0, 0), (2, 2, 0, 0),
+                                                              (4, 4, 0, 0), (1, 2, 1, 0), (2, 0, 1, 0), (1, 4, 2, 0),
+                                                              (1, 0, 4, 0), (2, 0, 3, 0), (2, 4, 2, 0), (2, 4, 0, 2),
+                                                              (1, 2, 3, 1), (2, 8, 0, 2), (4, 8, 1, 1)])
+def test_multi_rhs_stencil(prec, recon, tol, per_thread, slab, cols_outer, slab_x2):
+    """k right-hand sides sharing each link load (1, 2 or 4 per thread; flat,
+    x1-slab or (x1, x2)-tile site order, the 8 x 2 x 6 tiles being narrower
+    than a workgroup; the columns of a tile back to back, or blocks of 1, 2,
     3 (lowered to 2) or 4 columns sweeping the lattice in turn), two gauge
     configurations, against the oracle column by column."""
-    print(f"[gpu] multi-rhs stencil {prec} k/thread={per_thread} slab={slab} cols_outer={cols_outer}", flush=True)
+    print(f"[gpu] multi-rhs stencil {prec} k/thread={per_thread} slab={slab} x2={slab_x2} cols_outer={cols_outer}",
+          flush=True)
     import torch
 
     from jaxqft_amd import _native as nat
@@ -318,9 +321,10 @@ def test_multi_rhs_stencil(prec, recon, tol, per_thread, slab, cols_outer):
     psi = fields.random_fermion(rng, (B * k, *L, 4, 3))
     ctx = nat.Context(L, 3, B, 0.04, 1.0, bc)
     pc = nat.PREC[prec]
-    ctx.set_tuning(pc, True, 1, per_thread, True, slab, cols_outer)
+    ctx.set_tuning(pc, True, 1, per_thread, True, slab, cols_outer, slab_x2=slab_x2)
     assert ctx.get_tuning(pc, k)["rhs_per_thread"] == per_thread
     assert ctx.get_tuning(pc, k)["slab"] == slab
+    assert ctx.get_tuning(pc, k)["slab_x2"] == (slab_x2 if slab else 0)
     assert ctx.get_tuning(pc, k)["cols_outer"] == cols_outer
     g = nat.Gauge(ctx, torch.from_numpy(U).cuda(), nat.WL_LAYOUT_BMXYIJ)
     fi, fo = nat.Field(ctx, pc, B * k), nat.Field(ctx, pc, B * k)
@@ -430,6 +434,13 @@ def test_plane_kernel_rejected_shapes_fall_back():
     ctx = nat.Context((4, 6, 4, 8), 3, 1, 0.1, 1.0, (1, 1, 1, -1))
     with pytest.raises(nat.NativeError, match="plane"):
         ctx.set_tuning(nat.WL_F16, False, 1, 0, True, 0, 0, 0, 0, 4)
+    # a tile x2 extent must divide L2; it is reported only with a slab order
+    with pytest.raises(nat.NativeError, match="slab_x2"):
+        ctx.set_tuning(nat.WL_F16, True, 1, 2, True, 2, 0, slab_x2=3)
+    ctx.set_tuning(nat.WL_F16, True, 1, 2, True, 2, 0, slab_x2=2)
+    assert ctx.get_tuning(nat.WL_F16, 4)["slab_x2"] == 2
+    ctx.set_tuning(nat.WL_F16, True, 1, 2, True, 0, 0, slab_x2=2)
+    assert ctx.get_tuning(nat.WL_F16, 4)["slab_x2"] == 0
 
 
 def test_mass_change_reaches_raw_and_resident_links_alike():
