@@ -43,7 +43,7 @@ def tune(ctx: nat.Context, gauge: nat.Gauge, prec: int, recon: int, fin: nat.Fie
          cols_outer: Iterable[int] = (0, 2, 3), boxes: Iterable[int] = (1, 2, 7),
          mrhs_marches: Iterable[int] = (2, 4, 8, 16), planes: Iterable[int] = (2, 4, 8),
          lds_boxes: Iterable[int] = (1, 2, 3),
-         tiles: Iterable = ((16, 8),)) -> Dict[str, object]:
+         tiles: Iterable = ()) -> Dict[str, object]:
     """Pick the fastest launch shape for (prec, recon) at nrhs right-hand sides
     per configuration and install it.  Candidates: every march length dividing
     L0 with one right-hand side per thread, plus 2 and 4 right-hand sides per
