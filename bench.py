@@ -86,6 +86,9 @@ def parse():
     ap.add_argument("--solve-trace", action="store_true",
                     help="tune as the full run does, then run the headline solve once more after a 1 s pause "
                          "(for a kernel trace of that solve alone) and stop")
+    ap.add_argument("--eo-trace", action="store_true",
+                    help="tune the even-odd hop as the full run does, run the mixed k-RHS even-odd solve, pause 1 s, "
+                         "run it again (for a kernel trace of that solve alone) and stop")
     return ap.parse_args()
 
 
@@ -311,6 +314,21 @@ def main():
             log(f"dslash {prec} r{recon} k={k}: {best[1]:.1f} us ({best[1]/k:.1f}/rhs) {gbs:.0f} GB/s, "
                 f"{len(tried)} shapes tuned (table in the full record)")
             del fi, fo
+
+        if args.eo_trace:
+            # the mixed k-RHS even-odd solve of the full run (tuned the same
+            # way), once warm, then after a pause that marks the trace
+            xek = torch.empty_like(bk_t)
+            eo_tune = tune_eo(ctx, gauge, bk_t, xek, k, nat.PREC[args.inner], args.recon, delta=args.eo_delta,
+                              stream=stream)
+            B.solve_eo(ctx, gauge, bk_t, xek, k, args.inner, args.recon)
+            torch.cuda.synchronize()
+            time.sleep(1.0)
+            t, st = B.timed(1, B.solve_eo, ctx, gauge, bk_t, xek, k, args.inner, args.recon)
+            log(f"traced mixed {k}-RHS even-odd solve {t*1e3:.1f} ms {summarize(st)}")
+            print(json.dumps({"eo_trace": True, "ms": round(t * 1e3, 2), "inner_hop_eo_slab": eo_tune["eo_slab"],
+                              "update_hop_eo_slab": eo_tune.get("update_eo_slab"), **summarize(st)}))
+            return
 
         if args.solve_trace:
             # one warm headline solve; the calibration copy (1 GiB read and
