@@ -30,6 +30,13 @@ from .chrono import SolverGuess
 from . import fields as _fields
 
 
+def _vdot_batch(a, b):
+    """Re <a_b, b_b> for each batch element b (leading axis), as a NumPy array."""
+    aa = np.asarray(a.cpu() if isinstance(a, torch.Tensor) else a)
+    bb = np.asarray(b.cpu() if isinstance(b, torch.Tensor) else b)
+    return np.real(np.sum(np.conj(aa) * bb, axis=tuple(range(1, aa.ndim))))
+
+
 def _as_like(a, ref):
     """a as the same kind of array as ref (torch tensor on ref's device, or NumPy)."""
     if isinstance(ref, torch.Tensor):
@@ -139,10 +146,15 @@ class WilsonNf2Theory:
         self.solver_form = str(self.solver_form).lower()
         self.preconditioner_kind = str(self.preconditioner_kind).lower()
         self.dirac_kernel = str(self.dirac_kernel).lower()
-        if self.solver_kind != "cg":
-            raise ValueError("the native solver path provides solver_kind='cg' (CG on D^dag D) only")
-        if self.solver_form != "normal":
-            raise ValueError("the native solver path provides solver_form='normal' only")
+        # the reference's solver fields and checks (su3_wilson_nf2.py:634-639); every solve here is CG on
+        # the normal equations (D^dag D or S^dag S), whose solutions are those of the reference's
+        # non-Hermitian split solves (see "split solver forms" below)
+        if self.solver_kind not in ("cg", "bicgstab", "gmres"):
+            raise ValueError("solver_kind must be one of: cg, bicgstab, gmres")
+        if self.solver_form not in ("normal", "split", "eo_split"):
+            raise ValueError("solver_form must be one of: normal, split, eo_split")
+        if self.solver_form in ("split", "eo_split") and self.solver_kind == "cg":
+            raise ValueError("solver_form=split/eo_split requires a non-Hermitian solver (use solver_kind=bicgstab)")
         # A scalar Jacobi preconditioner 1/m0^2 leaves the CG iterates and the
         # residual-based stopping rule unchanged, so both kinds map to plain CG.
         if self.preconditioner_kind not in ("none", "jacobi"):
@@ -264,6 +276,18 @@ class WilsonNf2Theory:
             raise ValueError("kernel must be one of: optimized, reference")
         return self.apply_normal(U, psi)
 
+    # The reference's dense-gamma forms (su3_wilson_nf2.py:1157-1167) apply the
+    # same operator with the gamma matrices multiplied out; the projector
+    # kernels here compute it once.
+    def apply_D_dense(self, U, psi):
+        return self.apply_D(U, psi)
+
+    def apply_Ddag_dense(self, U, psi):
+        return self.apply_Ddag(U, psi)
+
+    def apply_normal_dense(self, U, psi):
+        return self.apply_normal(U, psi)
+
     # ------------------------------------------------- even-odd (checkerboard)
     def _eo(self, U, psi, op: int, normalized: bool = False, nrhs: int = 1):
         ctx, g = self._gauge(U)
@@ -317,6 +341,22 @@ class WilsonNf2Theory:
         self._check_fermion(psi_even, "psi_even")
         return self._eo(U, psi_even, nat.WL_EO_SCHUR_E_DAG if daggered else nat.WL_EO_SCHUR_E, normalized)
 
+    def _parity_project(self, psi, parity: int):
+        """psi on the sites of one parity (sum of coordinates mod 2), zero on the
+        others (su3_wilson_nf2.py:884-892)."""
+        even = np.indices(self.lattice_shape).sum(axis=0) % 2 == 0
+        keep = even if parity == 0 else ~even
+        mask = keep.reshape((1, *self.lattice_shape, 1, 1))
+        if isinstance(psi, torch.Tensor):
+            return psi * torch.as_tensor(mask, device=psi.device).to(psi.dtype)
+        return np.asarray(psi) * mask.astype(np.asarray(psi).real.dtype)
+
+    def project_even(self, psi):
+        return self._parity_project(psi, 0)
+
+    def project_odd(self, psi):
+        return self._parity_project(psi, 1)
+
     def _solve_eo(self, U, rhs, x0, nrhs: int = 1, masses=None, **overrides):
         ctx, g = self._gauge(U)
         pt, kind, orig = to_device(rhs)
@@ -352,10 +392,78 @@ class WilsonNf2Theory:
         return self._solve_eo(U, rhs, x0)
 
     def solve_eop_even(self, U, phi):
-        """solver_form 'normal' dispatch of su3_wilson_nf2.py:1403-1408."""
-        if self.solver_form != "normal":
-            raise ValueError(f"solver_form {self.solver_form!r}: only 'normal' has a native even-odd solver")
-        return self.solve_eop_even_normal(U, phi)
+        """solver_form dispatch of su3_wilson_nf2.py:1402-1408."""
+        if self.solver_form == "normal":
+            return self.solve_eop_even_normal(U, phi)
+        return self.solve_eop_even_split(U, phi)
+
+    # ------------------------------------------------- split solver forms
+    # The reference solves D x = b, D^dag x = b and their even-odd Schur forms
+    # with a non-Hermitian Krylov method (bicgstab or gmres).  Here each is CG
+    # on the normal equations, with the same solution: D x = b is D^dag D x =
+    # D^dag b, and D^dag x = b is x = D z with D^dag D z = b (likewise with the
+    # normalized even Schur complement S).  The stopping rule applies to the
+    # normal system's residual (cg_tol on its FP64 true residual by default).
+    def solve_direct(self, U, rhs):
+        """x with D x = rhs (su3_wilson_nf2.py:1271-1283): x = (D^dag D)^-1 D^dag rhs."""
+        self._check_fermion(rhs, "rhs")
+        return self.solve_normal(U, self.apply_Ddag(U, rhs))
+
+    def solve_dagger(self, U, rhs):
+        """x with D^dag x = rhs (su3_wilson_nf2.py:1285-1297): x = D (D^dag D)^-1 rhs."""
+        self._check_fermion(rhs, "rhs")
+        return self.apply_D(U, self.solve_normal(U, rhs))
+
+    def _solve_eo_reduced(self, U, rhs, dagger: bool):
+        """D x = rhs (or D^dag x = rhs) by the even-odd reduction of
+        su3_wilson_nf2.py:1299-1332: the even system (m0 - K_eo K_oe / m0) x_e =
+        b_e - K_eo b_o / m0 (its dagger for D^dag) is m0 times the normalized
+        Schur system, solved through the even-odd normal equations; then
+        x_o = (b_o - K_oe x_e) / m0."""
+        self._check_fermion(rhs, "rhs")
+        m0 = self._m0_value()
+        b_e, b_o = self.project_even(rhs), self.project_odd(rhs)
+        k_eo = self.apply_Kdag_eo if dagger else self.apply_K_eo
+        k_oe = self.apply_Kdag_oe if dagger else self.apply_K_oe
+        c = (b_e - _as_like(k_eo(U, b_o), b_e) / m0) / m0
+        if dagger:
+            x_e = self.apply_eo_schur_even(U, self.solve_eop_even_normal(U, c), normalized=True)
+        else:
+            x_e = self.solve_eop_even_normal(U, self.apply_eo_schur_even_dagger(U, c, normalized=True))
+        x_e = _as_like(x_e, b_e)
+        x_o = (b_o - _as_like(k_oe(U, x_e), b_o)) / m0
+        return x_e + x_o
+
+    def solve_direct_eo(self, U, rhs):
+        """x with D x = rhs through the even-odd reduction (su3_wilson_nf2.py:1334-1335)."""
+        return self._solve_eo_reduced(U, rhs, dagger=False)
+
+    def solve_dagger_eo(self, U, rhs):
+        """x with D^dag x = rhs through the even-odd reduction (su3_wilson_nf2.py:1337-1338)."""
+        return self._solve_eo_reduced(U, rhs, dagger=True)
+
+    def solve_eop_even_direct(self, U, rhs):
+        """x_e with S x_e = P_e rhs, S the normalized even Schur complement
+        (su3_wilson_nf2.py:1340-1353): x_e = (S^dag S)^-1 S^dag P_e rhs."""
+        self._check_fermion(rhs, "rhs")
+        return self.solve_eop_even_normal(U, self.apply_eo_schur_even_dagger(U, self.project_even(rhs),
+                                                                            normalized=True))
+
+    def solve_eop_even_dagger(self, U, rhs):
+        """x_e with S^dag x_e = P_e rhs (su3_wilson_nf2.py:1355-1368): x_e = S (S^dag S)^-1 P_e rhs."""
+        self._check_fermion(rhs, "rhs")
+        return self.apply_eo_schur_even(U, self.solve_eop_even_normal(U, rhs), normalized=True)
+
+    def solve_eop_even_split_with_intermediate(self, U, phi):
+        """y = S^-dag P_e phi and x = S^-1 y (su3_wilson_nf2.py:1394-1397): x = (S^dag S)^-1 P_e phi
+        from one even-odd normal solve, y = S x."""
+        self._check_fermion(phi, "phi")
+        x = self.solve_eop_even_normal(U, phi)
+        return x, self.apply_eo_schur_even(U, x, normalized=True)
+
+    def solve_eop_even_split(self, U, phi):
+        """su3_wilson_nf2.py:1399-1401."""
+        return self.solve_eop_even_split_with_intermediate(U, phi)[0]
 
     def solve_eop_even_normal_multi(self, U, rhs, masses: Optional[Sequence[float]] = None, **overrides):
         """k right-hand sides per configuration through one even-odd solve.
@@ -414,7 +522,11 @@ class WilsonNf2Theory:
         the force from the solution (su3_wilson_nf2.py:378-392).  With
         use_solver_guess the solve starts from the guess of solver_guess_mode
         over the last solver_guess_history (X, Y) pairs ('mre' fits eta,
-        phi = D^dag eta, when given)."""
+        phi = D^dag eta, when given).  A split solver form takes (X, Y) from
+        solve_split_with_intermediate, as the reference does."""
+        if self.solver_form in ("split", "eo_split"):
+            X, Y = self.solve_split_with_intermediate(U, phi)
+            return self.pseudofermion_force_from_solution(U, X, Y)
         X = self._solve_with_history(U, phi, eta)
         Y = self.apply_D(U, X)
         if self.use_solver_guess:
@@ -481,6 +593,28 @@ class WilsonNf2Theory:
         self._check_fermion(x0, "x0")
         return self._solve(U, phi, x0)
 
+    def solve_split_with_intermediate(self, U, phi):
+        """y = D^-dag phi and x = D^-1 y (su3_wilson_nf2.py:1434-1441).  For
+        solver_form 'eo_split' both solves go through the even-odd reduction, as
+        in the reference; otherwise x = (D^dag D)^-1 phi from one normal solve
+        and y = D x."""
+        self._check_fermion(phi, "phi")
+        if self.solver_form == "eo_split":
+            y = self.solve_dagger_eo(U, phi)
+            return self.solve_direct_eo(U, y), y
+        x = self.solve_normal(U, phi)
+        return x, self.apply_D(U, x)
+
+    def solve_split(self, U, phi):
+        """su3_wilson_nf2.py:1443-1445."""
+        return self.solve_split_with_intermediate(U, phi)[0]
+
+    def solve_pseudofermion(self, U, phi):
+        """solver_form dispatch of su3_wilson_nf2.py:1447-1454."""
+        if self.solver_form == "normal":
+            return self.solve_normal(U, phi)
+        return self.solve_split(U, phi)
+
     def solve_normal_multi(self, U, rhs, masses: Optional[Sequence[float]] = None, **overrides):
         """Solve k right-hand sides per configuration at once.
 
@@ -508,13 +642,25 @@ class WilsonNf2Theory:
         return self.apply_Ddag(U, chi)
 
     def pseudofermion_action(self, U, phi):
-        """S = Re <phi, (D^dag D)^-1 phi> per batch element (su3_wilson_nf2.py:1183-1189)."""
+        """S = Re <phi, (D^dag D)^-1 phi> per batch element (su3_wilson_nf2.py:1183-1196);
+        with a split solver form S = |chi|^2 with D^dag chi = phi, the same value."""
+        if self.solver_form in ("split", "eo_split"):
+            chi = self.solve_dagger_eo(U, phi) if self.solver_form == "eo_split" else self.solve_dagger(U, phi)
+            return _vdot_batch(chi, chi)
         x = self._solve_with_history(U, phi)
         if self.use_solver_guess:
             self.solver_guess.update(x)
-        ph = np.asarray(phi.cpu() if isinstance(phi, torch.Tensor) else phi)
-        xx = np.asarray(x.cpu() if isinstance(x, torch.Tensor) else x)
-        return np.real(np.sum(np.conj(ph) * xx, axis=tuple(range(1, ph.ndim))))
+        return _vdot_batch(phi, x)
+
+    def eop_pseudofermion_action(self, U, phi):
+        """S = Re <phi_e, (S^dag S)^-1 phi_e> with phi_e = P_e phi, or |chi|^2 with
+        S^dag chi = phi_e for a split solver form (su3_wilson_nf2.py:1198-1208)."""
+        self._check_fermion(phi, "phi")
+        phi_e = self.project_even(phi)
+        if self.solver_form in ("split", "eo_split"):
+            chi = self.solve_eop_even_dagger(U, phi_e)
+            return _vdot_batch(chi, chi)
+        return _vdot_batch(phi_e, self.solve_eop_even_normal(U, phi_e))
 
     def apply_gamma5(self, psi):
         """(gamma5 psi) on the spin axis; host-side helper for tests."""

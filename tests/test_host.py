@@ -137,11 +137,20 @@ def test_theory_validation_and_shapes():
     assert u1.fermion_shape() == (1, 4, 4, 4, 8, 4, 1)
     assert u1.field_shape() == (1, 4, 4, 4, 4, 8)
     assert u1.fermion_bc == (1, 1, 1, -1)
-    for bad in (dict(layout="XYZ"), dict(solver_kind="gmres"), dict(solver_form="split"),
+    for bad in (dict(layout="XYZ"), dict(solver_kind="minres"), dict(solver_form="split"),
+                dict(solver_form="eo_split"), dict(solver_form="direct", solver_kind="gmres"),
                 dict(preconditioner_kind="ilu"), dict(dirac_kernel="fast"), dict(fermion_bc=(1, 1)),
                 dict(solver_precision="f8"), dict(solver_recon=8)):
         with pytest.raises(ValueError):
             SU3WilsonNf2(lattice_shape=(4,) * 4, **bad)
+    # the reference's solver kinds and forms (su3_wilson_nf2.py:634-639): a split form needs a
+    # non-Hermitian solver kind there; here every one runs CG on the normal equations
+    for kind, form in (("cg", "normal"), ("bicgstab", "normal"), ("gmres", "normal"), ("bicgstab", "split"),
+                       ("gmres", "eo_split"), ("BiCGStab", "EO_Split")):
+        t = SU3WilsonNf2(lattice_shape=(4,) * 4, solver_kind=kind, solver_form=form)
+        assert (t.solver_kind, t.solver_form) == (kind.lower(), form.lower())
+    with pytest.raises(ValueError, match="requires a non-Hermitian solver"):
+        U1WilsonNf2(lattice_shape=(4,) * 4, solver_form="split")
     assert th.dirac.flops_per_site_matvec(nc=3) == 3600  # the reference's own estimator at Nc=3
     assert th.dirac.flops_per_site_dslash(nc=3) == 3552
 
