@@ -102,6 +102,10 @@ class Lane:
             nat.check(self.ctx._lib.wl_batch_step(self.h))
             self.stepped = True
 
+    def ready(self) -> bool:
+        """Whether the work queued on this lane has finished (no waiting)."""
+        return self.stream.query()
+
     def harvest(self, now_fn) -> List[Request]:
         """Wait for this lane's queued work, retire finished columns and return
         the requests whose every column has finished."""
@@ -143,7 +147,13 @@ class Lane:
 
 class Server:
     """Scheduler: FIFO admission per lane with an HBM budget, continuous
-    batching, lanes on separate streams."""
+    batching, lanes on separate streams.  Each lane always has one block of
+    iterations queued while it has work; a lane is harvested (finished
+    requests retired, freed columns reused) as soon as its own block has
+    finished, so a lane on a large lattice does not hold back the lanes on
+    small ones."""
+
+    IDLE_POLL_S = 2e-4  # host sleep while every lane's block is still running
 
     def __init__(self, gauge_fields: Dict[Tuple[Tuple[int, ...], int], Any], nc: int = 3, bc=(1, 1, 1, -1),
                  hbm_budget_bytes: float = 64e9, capacity: int = 16, inner: str = "f16", recon: int = 12,
@@ -228,8 +238,13 @@ class Server:
                     raise RuntimeError("admission stalled")
                 continue
             for lane in active:
-                lane.step()
-            for lane in active:
+                if not lane.stepped:
+                    lane.step()
+            ready = [lane for lane in active if lane.stepped and lane.ready()]
+            if not ready:
+                time.sleep(self.IDLE_POLL_S)
+                continue
+            for lane in ready:
                 fin = lane.harvest(now)
                 if release:
                     for r in fin:

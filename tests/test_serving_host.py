@@ -11,9 +11,10 @@ from jaxqft_amd.serving import Request, Server
 
 class FakeLane:
     """Columns finish after a number of steps that depends on the mass; the
-    'solution' of column j is rhs[j] / (m + 4)^2."""
+    'solution' of column j is rhs[j] / (m + 4)^2.  A step's block stays in
+    flight for `polls` readiness checks (a lane on a large lattice)."""
 
-    def __init__(self, key, capacity, log):
+    def __init__(self, key, capacity, log, polls=0):
         self.key = key
         self.capacity = capacity
         self.owner = [None] * capacity
@@ -21,6 +22,8 @@ class FakeLane:
         self.stepped = False
         self.log = log
         self.closed = False
+        self.polls = polls
+        self.wait = 0
 
     @property
     def free(self):
@@ -47,6 +50,14 @@ class FakeLane:
                 if self.owner[s] is not None and self.left[s] > 0:
                     self.left[s] -= 1
             self.stepped = True
+            self.wait = self.polls
+            self.log.append(("step", None, self.key))
+
+    def ready(self):
+        if self.wait > 0:
+            self.wait -= 1
+            return False
+        return True
 
     def harvest(self, now_fn):
         if not self.stepped:
@@ -74,13 +85,14 @@ class FakeLane:
 
 
 class FakeBackend:
-    def __init__(self, capacity, lane_bytes=100):
+    def __init__(self, capacity, lane_bytes=100, polls=None):
         self.capacity = capacity
         self.lane_bytes = lane_bytes
         self.log = []
         self.live = 0
         self.max_live = 0
         self.lanes = []
+        self.polls = polls or {}
 
     def add_field(self, key, U):
         return 10
@@ -89,7 +101,7 @@ class FakeBackend:
         return self.lane_bytes
 
     def new_lane(self, key):
-        lane = FakeLane(key, self.capacity, self.log)
+        lane = FakeLane(key, self.capacity, self.log, self.polls.get(key, 0))
         self.lanes.append(lane)
         live = sum(1 for l in self.lanes if not l.closed)
         self.max_live = max(self.max_live, live)
@@ -173,3 +185,25 @@ def test_request_larger_than_budget_raises():
     with pytest.raises(ValueError, match="capacity"):
         Server(fields_for([key]), capacity=4, hbm_budget_bytes=1e9, backend=FakeBackend(4)).run(
             [Request(rid=0, key=key, mass=0.1, rhs=np.ones((5, 2, 2), complex))], realtime=False)
+
+
+def test_slow_lane_does_not_hold_back_fast_lanes():
+    """A lane whose block of iterations is still in flight (a large lattice)
+    is not waited for: the other lane is harvested and stepped again meanwhile,
+    so its requests finish before the slow lane's first block does."""
+    big, small = ((32, 32, 32, 32), 0), ((8, 8, 8, 8), 0)
+    be = FakeBackend(capacity=4, polls={big: 50})
+    srv = Server(fields_for([big, small]), capacity=4, hbm_budget_bytes=1e9, backend=be)
+    srv.IDLE_POLL_S = 0.0
+    reqs = [Request(rid=0, key=big, mass=0.02, rhs=np.ones((2, 2, 2), complex))]
+    reqs += [Request(rid=i, key=small, mass=0.1, rhs=np.ones((2, 2, 2), complex)) for i in range(1, 6)]
+    res = srv.run(reqs, realtime=False)
+    assert res["requests"] == 6
+    order = [(e[0], e[1], e[2]) for e in be.log]
+    first_big_done = order.index(("done", 0, big))
+    small_done = [i for i, e in enumerate(order) if e[0] == "done" and e[2] == small]
+    assert len(small_done) == 5 and max(small_done) < first_big_done
+    # the small lane was stepped many times while the big lane's only block was in flight
+    steps_small = sum(1 for e in order[:first_big_done] if e[0] == "step" and e[2] == small)
+    steps_big = sum(1 for e in order[:first_big_done] if e[0] == "step" and e[2] == big)
+    assert steps_big == 1 and steps_small >= 5
