@@ -60,6 +60,9 @@ def parse():
                     help="reliable-update trigger of the even-odd solves (the Schur system's residual falls faster per "
                          "iteration; at 32^4 0.005 saves one update against 0.01 at the same 27 iterations)")
     ap.add_argument("--accum", type=str, default="f16", help="inner solution accumulator of the headline solve: f16 | auto (f32) | f64")
+    ap.add_argument("--search", type=str, default="recurrence",
+                    help="search-direction update of the mixed solves: recurrence (t = D r + beta t, two launches "
+                         "per iteration) | xpay (p = r + beta p in its own pass, three)")
     ap.add_argument("--poll", type=int, default=16)
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--dslash-reps", type=int, default=50)
@@ -114,7 +117,7 @@ class Bench:
         a = self.args
         sp, keep = nat.solve_params(a.tol, 20000, nat.WL_STOP_TRUE_RESIDUAL, nat.PREC[prec], recon,
                                     a.delta if delta is None else delta, a.poll, bool(a.graph), masses,
-                                    accum=a.accum if delta is None else "auto")
+                                    accum=a.accum if delta is None else "auto", search=a.search)
         st = (nat.WlSolveStats * b.ncol)()
         nat.check(self.lib.wl_cg_normal_field(ctx.h, gauge.h, b.h, x.h, 0, nrhs, byref(sp), st, self.h))
         del keep
@@ -125,7 +128,7 @@ class Bench:
         nat = self.nat
         a = self.args
         sp, keep = nat.solve_params(a.tol, 20000, nat.WL_STOP_TRUE_RESIDUAL, nat.PREC[prec], recon, a.eo_delta, a.poll,
-                                    bool(a.graph), None, accum=a.accum)
+                                    bool(a.graph), None, accum=a.accum, search=a.search)
         st = (nat.WlSolveStats * nrhs)()
         nat.check(self.lib.wl_cg_eo_normal(ctx.h, gauge.h, c_void_p(bt.data_ptr()), c_void_p(xt.data_ptr()), c_void_p(0),
                                            nrhs, nat.WL_C128, byref(sp), st, self.h))
@@ -185,7 +188,8 @@ def measured_traffic_cg(args, k, L):
         d = json.loads(PROFILE_TRAFFIC_CG.read_text())
     except (OSError, ValueError):
         return None, None
-    want = {"lattice": list(L), "nrhs": k, "inner": args.inner, "recon": args.recon, "accumulator": args.accum}
+    want = {"lattice": list(L), "nrhs": k, "inner": args.inner, "recon": args.recon, "accumulator": args.accum,
+            "search": args.search}
     if any(d.get(key) != val for key, val in want.items()):
         return None, None
     return int(d["fetch_bytes_per_launch"] + d["write_bytes_per_launch"]), "profiles/" + PROFILE_TRAFFIC_CG.name
@@ -556,7 +560,13 @@ def cg_step_roofline(args, V, k, prof, stats):
 
     Per site, links counted once (12-real FP16: 96 B) and, per right-hand
     side, each spinor field moved once in the storage precision (FP16: 48 B
-    of halves + 4 B scale):
+    of halves + 4 B scale).  Search direction by recurrence (the default):
+      D with |t|^2 (t = D r + beta t_prev):  links + k (r read + t_prev read + t written)
+      D^dag with the CG step:                links + k (t read + r read + r written
+                                             + p_prev read + p written)
+                                             plus, every second iteration (pairwise y),
+                                             k (y read + y written)
+    With p = r + beta p in a pass of its own (--search xpay):
       D with |t|^2 (t = D p):   links + k (p read + t written)
       D^dag with the CG step:   links + k (t read + r read + r written)
                                 plus, every second iteration (pairwise y),
@@ -568,9 +578,15 @@ def cg_step_roofline(args, V, k, prof, stats):
     links = dslash_bytes_per_site(args.inner, args.recon, 0)
     acc = {"f16": "f16", "f32": "f32", "f64": "f64", "auto": "f32"}[args.accum]
     spin_y = (dslash_bytes_per_site(acc, 18, 1) - dslash_bytes_per_site(acc, 18, 0)) // 2
-    op_b = (links + k * 2 * spin1) * V
-    dag_b0 = (links + k * 3 * spin1) * V
-    dag_b1 = (links + k * (5 * spin1 + 2 * spin_y)) * V
+    rec = args.search == "recurrence"
+    if rec:
+        op_b = (links + k * 3 * spin1) * V
+        dag_b0 = (links + k * 5 * spin1) * V
+        dag_b1 = (links + k * (5 * spin1 + 2 * spin_y)) * V
+    else:
+        op_b = (links + k * 2 * spin1) * V
+        dag_b0 = (links + k * 3 * spin1) * V
+        dag_b1 = (links + k * (5 * spin1 + 2 * spin_y)) * V
     t_op = prof["us_op"]
     t_dag = prof["us_op_dag"]
     n = prof["iterations"]
@@ -587,17 +603,19 @@ def cg_step_roofline(args, V, k, prof, stats):
         "profiled_solve": summarize(stats),
         "timed_iterations": n,
         "dag_cg_step": {
-            "kernel": f"D^dag {args.inner} r{args.recon}, {k} RHS, fused CG step (y in {acc}, pairwise)",
+            "kernel": f"D^dag {args.inner} r{args.recon}, {k} RHS, fused CG step "
+                      f"({'p = r + beta p_prev, ' if rec else ''}y in {acc}, pairwise)",
             "us_mean": round(pair_us / 2, 2), "us_by_parity": [round(t, 2) for t in t_dag], "y_parity": y_par,
             "bytes_per_site_without_y": dag_b0 // V, "bytes_per_site_with_y": dag_b1 // V,
             "GB/s": round(dag_gbs, 1), "frac_hbm_peak": round(dag_gbs / PEAK_HBM_GBS, 4),
         },
         "d_norm": {
-            "kernel": f"D {args.inner} r{args.recon}, {k} RHS, |t|^2 partials",
+            "kernel": f"D {args.inner} r{args.recon}, {k} RHS, {'t = D r + beta t_prev, ' if rec else ''}|t|^2 partials",
             "us_mean": round(op_us, 2), "bytes_per_site": op_b // V,
             "GB/s": round(op_gbs, 1), "frac_hbm_peak": round(op_gbs / PEAK_HBM_GBS, 4),
         },
-        "xpay_us": round(prof["us_xpay"], 2),
+        "search": args.search,
+        "xpay_us": round(prof["us_xpay"], 2) if not rec else None,
     }
 
 
@@ -628,7 +646,7 @@ def compact_record(args, d, L, k):
                         "per-site FP32 scales, FP32 arithmetic), FP64 reliable updates delta=%g, FP64 true "
                         "residual %g" % ("x".join(map(str, L)), args.mass, k, args.inner, args.recon, args.delta,
                                          args.tol),
-            "lattice": list(L), "nrhs": k,
+            "lattice": list(L), "nrhs": k, "search": args.search,
         },
         "roofline": {
             "bound": "hbm",

@@ -131,12 +131,23 @@ typedef struct {
   int use_graph;    /* capture the poll_every-iteration body as a HIP graph */
   const double* mass; /* host array of per-column masses, or NULL for the context mass */
   int accum_prec;     /* WL_ACCUM_*: storage of the inner solution accumulator between reliable updates */
+  int search_update;  /* WL_SEARCH_*: how a mixed-precision solve forms the search direction p and
+                         t = D p (an FP64 solve always takes WL_SEARCH_XPAY) */
 } wl_solve_params;
 
 #define WL_ACCUM_AUTO 0 /* FP32 for a mixed-precision solve */
 #define WL_ACCUM_F64 1
 #define WL_ACCUM_F32 2
 #define WL_ACCUM_F16 3
+
+/* WL_SEARCH_RECURRENCE: the D launch of an iteration takes r and forms
+ * t = D r + beta t_prev (= D (r + beta p_prev)), and the D^dag launch forms
+ * p = r + beta p_prev beside its update of r; two launches per iteration.
+ * WL_SEARCH_XPAY: p = r + beta p_prev in a pass of its own and t = D p;
+ * three launches per iteration.  Both give the same iterates in exact
+ * arithmetic; they differ in FP16/FP32 rounding. */
+#define WL_SEARCH_RECURRENCE 0
+#define WL_SEARCH_XPAY 1
 
 typedef struct {
   int iters;            /* CG iterations performed */

@@ -29,6 +29,8 @@ WL_STOP_TRUE_RESIDUAL, WL_STOP_ITERATED = 0, 1
 WL_LIME_LEXI, WL_LIME_SITE, WL_LIME_COUNT = 0, 1, 2
 WL_ACCUM_AUTO, WL_ACCUM_F64, WL_ACCUM_F32, WL_ACCUM_F16 = 0, 1, 2, 3
 ACCUM = {None: WL_ACCUM_AUTO, "auto": WL_ACCUM_AUTO, "f64": WL_ACCUM_F64, "f32": WL_ACCUM_F32, "f16": WL_ACCUM_F16}
+WL_SEARCH_RECURRENCE, WL_SEARCH_XPAY = 0, 1
+SEARCH = {"recurrence": WL_SEARCH_RECURRENCE, "xpay": WL_SEARCH_XPAY}
 STATUS_NAMES = {-1: "running", 0: "converged", 1: "maxiter", 2: "nan", 3: "breakdown"}
 PREC = {"f64": WL_F64, "f32": WL_F32, "f16": WL_F16}
 
@@ -53,6 +55,7 @@ class WlSolveParams(ctypes.Structure):
         ("use_graph", c_int),
         ("mass", POINTER(c_double)),
         ("accum_prec", c_int),
+        ("search_update", c_int),
     ]
 
 
@@ -311,9 +314,10 @@ class Field:
 
 def solve_params(tol: float, maxiter: int, stop_rule: int = WL_STOP_TRUE_RESIDUAL, inner_prec: int = WL_F64,
                  inner_recon: int = 18, delta: float = 0.1, poll_every: int = 8, use_graph: bool = False,
-                 masses=None, accum: str = "auto"):
+                 masses=None, accum: str = "auto", search: str = "recurrence"):
     sp = WlSolveParams()
     sp.accum_prec = ACCUM[accum]
+    sp.search_update = SEARCH[search]
     sp.tol = float(tol)
     sp.maxiter = int(maxiter)
     sp.stop_rule = int(stop_rule)

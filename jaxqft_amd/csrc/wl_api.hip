@@ -1019,6 +1019,8 @@ struct Solver {
   int PY;         // precision of the inner solution accumulator
   bool mixed;
   bool fused;     // reliable updates run inside every iteration (device-driven)
+  bool rec = false;  // mixed precision: search direction by recurrence (WL_SEARCH_RECURRENCE): pb[0]
+                     // holds the column's p, t holds D p, and both are updated in place
   bool eo = false;  // operator M = the normalized even Schur complement S_e (else M = D)
   size_t cb64, cbP, cbY;
   const void* U64;
@@ -1101,13 +1103,15 @@ struct Solver {
   hipError_t accumulate(BlasArgs a) {
     a.a = x; a.col_bytes_a = cb64;
     a.b = y; a.col_bytes_b = cbY;
-    a.c = pb[0]; a.d = pb[1]; a.col_bytes_c = cbP; a.col_bytes_d = cbP;
+    a.c = pb[0]; a.d = rec ? pb[0] : pb[1]; a.col_bytes_c = cbP; a.col_bytes_d = cbP;
     return launch_accumulate(F64, PY, P, nc, a, st);
   }
 
   // gated reliable update before an iteration that reads pb[par ^ 1]:
   // x += y; r = b - A x (FP64 arithmetic, stored in the inner precision);
-  // pb[par ^ 1] = r + beta p (p: each column's latest search direction)
+  // pb[par ^ 1] = r + beta p (p: each column's latest search direction).
+  // With the search direction by recurrence the next iteration's launches
+  // form p = r + beta p and D p from the new r themselves.
   int reliable_update(int par) {
     hipError_t e;
     if (mixed) {
@@ -1128,7 +1132,7 @@ struct Solver {
     } else {
       CHK(apply(d, F64, R64, true, thi, rhi, mid64, EPI_RESID, RA_TRUERES));
     }
-    CHK(xpay(bargs(2), par, true));
+    if (!rec) CHK(xpay(bargs(2), par, true));
     e = launch_finalize_update(red.cols, ncol, st);
     if (e != hipSuccess) return fail(WL_ERR_HIP, "finalize: %s", hipGetErrorString(e));
     return WL_OK;
@@ -1150,7 +1154,9 @@ struct Solver {
   // The first update of a solve from x = 0: r_hi = b - A 0 = b exactly, so
   // the FP64 operator applications are skipped: r = b (in the inner
   // precision), |r_hi|^2 = |b|^2 decides convergence, p = r (into pb[0],
-  // where the first iteration reads it; pb[1] is zero and beta is 0).
+  // where the first iteration reads it; pb[1] is zero and beta is 0; with the
+  // search direction by recurrence beta = 0 makes the first iteration's
+  // launches set p = r and t = D r).
   int initial_update_zero_guess() {
     hipError_t e;
     {
@@ -1162,7 +1168,7 @@ struct Solver {
     }
     e = launch_trueres_from_b2(red.cols, ncol, st);
     if (e != hipSuccess) return fail(WL_ERR_HIP, "true residual: %s", hipGetErrorString(e));
-    CHK(xpay(bargs(2), 1, true));
+    if (!rec) CHK(xpay(bargs(2), 1, true));
     e = launch_finalize_update(red.cols, ncol, st);
     if (e != hipSuccess) return fail(WL_ERR_HIP, "finalize: %s", hipGetErrorString(e));
     return WL_OK;
@@ -1175,6 +1181,12 @@ struct Solver {
   // p = pb[par]; the previous iteration's p, in pb[par ^ 1], lets a pairwise
   // column add alpha p of two iterations to y at once (every second
   // iteration), which halves the y traffic of the epilogue.
+  //
+  // With the search direction by recurrence (rec) the iteration is two
+  // launches: t = D r + beta t_prev (= D p for p = r + beta p_prev, t_prev
+  // read at the site itself) with |t|^2, then D^dag t with p = r + beta p_prev
+  // formed and stored in place beside r -= alpha A p, |r|^2 and the y update.
+  // No pass of its own for p, and p is read only at the site itself.
   int iteration(int par, bool with_update = true) {
     hipEvent_t* ev = prof_slot(par);
     DslashArgs d;
@@ -1182,14 +1194,15 @@ struct Solver {
     d.gate = 1;
     d.col_bytes = cbP;
     d.red.par = par;
+    d.search_rec = rec ? 1 : 0;
     if (ev) HIPCHK(hipEventRecord(ev[0], st));
-    CHK(apply(d, P, R, false, pb[par], t, midP, EPI_NORM, RA_PAP));
+    CHK(apply(d, P, R, false, rec ? r : pb[par], t, midP, EPI_NORM, RA_PAP));
     if (ev) HIPCHK(hipEventRecord(ev[1], st));
     d.w = pb[par]; d.w2 = pb[par ^ 1]; d.res = r; d.y = y; d.col_bytes_y = cbY;
     const int epi = PY == F64 ? EPI_CG64 : (PY == F32 ? EPI_CG32 : EPI_CG16);
-    CHK(apply(d, P, R, true, t, nullptr, midP, epi, RA_ITER));
+    CHK(apply(d, P, R, true, t, rec ? pb[0] : nullptr, midP, epi, RA_ITER));
     if (ev) HIPCHK(hipEventRecord(ev[2], st));
-    CHK(xpay(bargs(1), par, false));
+    if (!rec) CHK(xpay(bargs(1), par, false));
     if (ev) HIPCHK(hipEventRecord(ev[3], st));
     // Mixed precision: the reliable update runs on the device right after the
     // iteration that requested it (all its kernels are gated on the column's
@@ -1260,10 +1273,13 @@ static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int n
   s.nrhs = nrhs;
   s.eo = eo;
   s.ntiles = (int)(((eo ? c->Vh : c->V) + NT - 1) / NT);
+  if (sp->search_update != WL_SEARCH_RECURRENCE && sp->search_update != WL_SEARCH_XPAY)
+    return fail(WL_ERR_BAD_ARG, "bad search_update %d", sp->search_update);
   s.P = sp->inner_prec;
   s.R = (nc == 1) ? 18 : (sp->inner_recon == 12 ? 12 : 18);
   s.mixed = s.P != F64;
   s.fused = s.mixed;
+  s.rec = s.mixed && sp->search_update == WL_SEARCH_RECURRENCE;
   s.PY = accum_prec_of(sp, s.P);
   s.cb64 = eo ? col_bytes_eo(c, F64) : col_bytes_of(c, F64);
   s.cbP = eo ? col_bytes_eo(c, s.P) : col_bytes_of(c, s.P);
@@ -1435,14 +1451,17 @@ static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int n
       if (cs.need_update && !cs.done) any_update = true;
       if (!cs.active) continue;
       any_active = true;
-      if (cs.iter > prev_it[i] && prev_rr[i] > cs.rr && cs.rr > 0.0 && cs.stop2 > 0.0) {
+      // the inner stopping threshold as the device applies it (with the
+      // margin of the last true / iterated residual ratio)
+      const double stop2 = cs.mode == 1 ? cs.stop2 : cs.stop2 / std::max(1.0, cs.gap2);
+      if (cs.iter > prev_it[i] && prev_rr[i] > cs.rr && cs.rr > 0.0 && stop2 > 0.0) {
         double rate = std::log(prev_rr[i] / cs.rr) / (cs.iter - prev_it[i]);
-        double left = cs.rr > cs.stop2 ? std::log(cs.rr / cs.stop2) / rate : 1.0;
+        double left = cs.rr > stop2 ? std::log(cs.rr / stop2) / rate : 1.0;
         left = std::min(left, (double)(cs.maxiter - cs.iter));
         remaining = std::max(remaining, left);
         have_rate = true;
         const double fast = 1.25 * rate;
-        double h = cs.rr > cs.stop2 ? std::log(cs.rr / cs.stop2) / fast : 0.0;
+        double h = cs.rr > stop2 ? std::log(cs.rr / stop2) / fast : 0.0;
         if (cs.delta > 0.0) {
           const double tgt = cs.delta * cs.delta * cs.r0norm * cs.r0norm;
           h = std::min(h, cs.rr > tgt ? std::log(cs.rr / tgt) / fast : 0.0);
@@ -1454,7 +1473,7 @@ static int cg_fields(wl_ctx* c, wl_gauge* g, void* b, void* x, int use_x0, int n
         // after an update the residual restarts near its iterated value: the
         // next request is a drop of delta (or the stopping threshold) later
         double T = cs.delta > 0.0 ? std::log(1.0 / (cs.delta * cs.delta)) / fast : 1e30;
-        const double to_stop = cs.rr > cs.stop2 ? std::log(cs.rr / cs.stop2) / fast - h : 0.0;
+        const double to_stop = cs.rr > stop2 ? std::log(cs.rr / stop2) / fast - h : 0.0;
         T = std::min(T, std::max(0.0, to_stop));
         T = std::min(T, (double)MAX_INNER_ITERS);
         // the later requests of the block: windows every T (T >= 1), and
@@ -1739,6 +1758,8 @@ int wl_batch_create(wl_ctx* c, wl_gauge* g, int capacity, const wl_solve_params*
   if (capacity < 1) return fail(WL_ERR_BAD_ARG, "capacity must be >= 1");
   if (sp->inner_prec < F64 || sp->inner_prec > F16) return fail(WL_ERR_BAD_ARG, "bad inner precision %d", sp->inner_prec);
   if (!(sp->tol >= 0.0) || sp->maxiter < 0) return fail(WL_ERR_BAD_ARG, "bad tol / maxiter");
+  if (sp->search_update != WL_SEARCH_RECURRENCE && sp->search_update != WL_SEARCH_XPAY)
+    return fail(WL_ERR_BAD_ARG, "bad search_update %d", sp->search_update);
   if (stream == nullptr && sp->use_graph) return fail(WL_ERR_BAD_ARG, "graph capture needs a non-default stream");
   std::lock_guard<std::mutex> lk(c->mu);
   wl_batch* bt = new wl_batch();
@@ -1764,6 +1785,7 @@ int wl_batch_create(wl_ctx* c, wl_gauge* g, int capacity, const wl_solve_params*
   s.R = (nc == 1) ? 18 : (sp->inner_recon == 12 ? 12 : 18);
   s.mixed = s.P != F64;
   s.fused = true;
+  s.rec = s.mixed && sp->search_update == WL_SEARCH_RECURRENCE;
   s.PY = accum_prec_of(sp, s.P);
   s.cb64 = col_bytes_of(c, F64);
   s.cbP = col_bytes_of(c, s.P);

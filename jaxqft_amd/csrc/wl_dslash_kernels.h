@@ -3,7 +3,10 @@
 // wl_kernels.h).  One CG iteration on A = D^dag D is two stencil launches:
 //   t = D p         with |t|^2 = <p, A p>          -> alpha = rr / |t|^2
 //   A p = D^dag t   with y += alpha p, r -= alpha A p, |r|^2 -> beta
-// so A p is never written to memory.
+// so A p is never written to memory.  With the search direction by
+// recurrence (search_rec) the first launch takes r and forms
+// t = D r + beta t_prev = D (r + beta p_prev), and the second forms
+// p = r + beta p_prev beside its update of r, so p needs no pass of its own.
 //
 // The 1D grid is ntiles x ncol workgroups, ordered spinor-column-fastest (the
 // columns of one tile run back to back and share its link lines in L2) or
@@ -111,6 +114,51 @@ __device__ __forceinline__ size_t site_of_tile(const DslashArgs& a, int tile, bo
 }
 
 
+// A spinor held as stored: FP16 keeps its half2 words and the site's scale
+// (13 registers instead of 24 expanded values), so that a spinor read once can
+// be used twice without being held expanded; other precisions keep the values.
+// get(q) returns component q in the arithmetic precision.  Streaming loads.
+template <int P, int NC>
+struct Stored {
+  using T = typename PT<P>::acc;
+  cx<T> v[4 * NC];
+  __device__ __forceinline__ void load(const void* base, size_t V, size_t site) {
+    SpinorIO<P, NC>::template load<true>(base, V, site, v);
+  }
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int q = 0; q < 4 * NC; ++q) v[q] = {T(0), T(0)};
+  }
+  __device__ __forceinline__ cx<T> get(int q) const { return v[q]; }
+};
+
+template <int NC>
+struct Stored<F16, NC> {
+  uint32_t w[4 * NC];
+  float s;
+  __device__ __forceinline__ void load(const void* base, size_t V, size_t site) {
+    const uint4* p = reinterpret_cast<const uint4*>(base);
+    s = gld<true>(reinterpret_cast<const float*>(p + NC * V) + site);
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+      const uint4 t = gld<true>(p + j * V + site);
+      w[4 * j] = t.x;
+      w[4 * j + 1] = t.y;
+      w[4 * j + 2] = t.z;
+      w[4 * j + 3] = t.w;
+    }
+  }
+  __device__ __forceinline__ void zero() {
+    s = 0.f;
+#pragma unroll
+    for (int q = 0; q < 4 * NC; ++q) w[q] = 0u;
+  }
+  __device__ __forceinline__ cx<float> get(int q) const {
+    const float2 h = h2f(w[q]);
+    return {h.x * s, h.y * s};
+  }
+};
+
 template <int EPI>
 struct EpiAccum {
   static constexpr int prec = EPI == EPI_CG64 ? F64 : (EPI == EPI_CG32 ? F32 : F16);
@@ -130,6 +178,21 @@ __device__ __forceinline__ double site_epilogue(const DslashArgs& a, int col, si
   double part = 0.0;
   if (EPI == EPI_STORE || EPI == EPI_NORM) {
     if (EPI == EPI_NORM) {
+      if (a.search_rec) {
+        // t = D r + beta t_prev = D (r + beta p_prev), t_prev at the site
+        // itself; beta = 0 starts a new direction, and t_prev is not read
+        // (it may hold anything then)
+        const T beta = (T)a.red.cols[col].beta;
+        if (beta != T(0)) {
+          cx<T> tp[4 * NC];
+          SpinorIO<P, NC>::template load<true>(outb(), V, s, tp);
+#pragma unroll
+          for (int q = 0; q < 4 * NC; ++q) {
+            res[q].re += beta * tp[q].re;
+            res[q].im += beta * tp[q].im;
+          }
+        }
+      }
 #pragma unroll
       for (int q = 0; q < 4 * NC; ++q) part += (double)res[q].re * (double)res[q].re + (double)res[q].im * (double)res[q].im;
     }
@@ -159,9 +222,65 @@ __device__ __forceinline__ double site_epilogue(const DslashArgs& a, int col, si
     const ColState& cs = a.red.cols[col];
     const double alpha = cs.alpha;
     char* rb = reinterpret_cast<char*>(a.res) + (size_t)col * a.col_bytes;
+    const T ar = (T)alpha;
+    if (a.search_rec) {
+      // The site's vectors r, p_prev and y are read together and held as
+      // stored (FP16: 13 registers each, so that the three reads are in
+      // flight at once), then: r -= alpha A p, stored; p = r + beta p_prev
+      // with the old r, stored in place of p_prev (out); y += alpha p every
+      // iteration, or (pairwise) nothing in the first iteration of a pair and
+      // alpha_prev p_prev + alpha p in the second.  p_prev is read when beta
+      // is not 0 or when it still owes its alpha_prev to y.
+      const double beta = cs.beta;
+      const bool second = cs.pairwise && cs.pend;
+      const bool prev = beta != 0.0 || second;
+      const bool yupd = !(cs.pairwise && !cs.pend);
+      char* yb = reinterpret_cast<char*>(a.y) + (size_t)col * a.col_bytes_y;
+      Stored<P, NC> r, pp;
+      r.load(rb, V, s);
+      if (prev) {
+        pp.load(outb(), V, s);
+      } else {
+        pp.zero();
+      }
+      Stored<PY, NC> ys;
+      if (yupd) ys.load(yb, V, s);
+      {
+        cx<T> rn[4 * NC];
+#pragma unroll
+        for (int q = 0; q < 4 * NC; ++q) {
+          const cx<T> h = r.get(q);
+          rn[q].re = h.re - ar * res[q].re;
+          rn[q].im = h.im - ar * res[q].im;
+          part += (double)rn[q].re * (double)rn[q].re + (double)rn[q].im * (double)rn[q].im;
+        }
+        if (valid) SpinorIO<P, NC>::template store<true>(rb, V, s, rn);
+      }
+      // p = r + beta p_prev
+      const T bt = (T)beta;
+      cx<T> p[4 * NC];
+#pragma unroll
+      for (int q = 0; q < 4 * NC; ++q) {
+        const cx<T> h = r.get(q), g = pp.get(q);
+        p[q] = {h.re + bt * g.re, h.im + bt * g.im};
+      }
+      if (valid) SpinorIO<P, NC>::template store<true>(outb(), V, s, p);
+      if (yupd) {
+        const TY ay = (TY)alpha;
+        const TY aq = (TY)(second ? cs.alpha_prev : 0.0);
+        cx<TY> w[4 * NC];  // takes p's registers when y has the inner precision's arithmetic
+#pragma unroll
+        for (int q = 0; q < 4 * NC; ++q) {
+          const cx<T> g = pp.get(q);
+          const cx<TY> yv = ys.get(q);
+          w[q] = {yv.re + (ay * (TY)p[q].re + aq * (TY)g.re), yv.im + (ay * (TY)p[q].im + aq * (TY)g.im)};
+        }
+        if (valid) SpinorIO<PY, NC>::template store<true>(yb, V, s, w);
+      }
+      return valid ? part : 0.0;
+    }
     cx<T> r[4 * NC];
     SpinorIO<P, NC>::template load<true>(rb, V, s, r);
-    const T ar = (T)alpha;
 #pragma unroll
     for (int q = 0; q < 4 * NC; ++q) {
       r[q].re -= ar * res[q].re;

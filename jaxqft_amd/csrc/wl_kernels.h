@@ -34,6 +34,8 @@ struct ColState {
   double delta;     // reliable-update trigger (0 disables)
   double rtrue2;    // last true residual |b - A x|^2
   double m0;        // diagonal m + 4r of this column's operator
+  double gap2;      // (true / iterated residual)^2 measured at the last reliable update (0: none yet);
+                    // the inner stopping test takes it as at least 1
   int iter;
   int maxiter;
   int active;       // iterate this column
@@ -75,7 +77,7 @@ enum Epilogue {
   EPI_STORE = 0,  // out = result
   EPI_NORM = 1,   // out = result; partial |result|^2
   EPI_RESID = 2,  // out = w - result; partial |out|^2
-  EPI_CG64 = 3,   // CG step with A p = result (not stored): y += alpha w, r -= alpha A p; partial |r|^2; y in F64
+  EPI_CG64 = 3,   // CG step with A p = result (not stored): y += alpha p, r -= alpha A p; partial |r|^2; y in F64
   EPI_CG32 = 4,   //   ... y in F32
   EPI_CG16 = 5,   //   ... y in F16
   EPI_RESID32 = 6,  // FP64 true residual stored in the inner precision: res = w - result in F32
@@ -122,6 +124,9 @@ struct DslashArgs {
   double r;               // Wilson r
   int m0_from_state;      // take m0 per column from red.cols[col].m0
   int gate;               // 0 all, 1 active columns, 2 columns awaiting a reliable update, 3 column red.only_col
+  int search_rec;         // CG epilogues with the search direction by recurrence: EPI_NORM forms
+                          // out = result + beta out_prev (in = r), the CG steps form p = r + beta p_prev
+                          // in out (in place) beside r -= alpha A p; beta = the column's beta
   Reduction red;
   // even-odd (checkerboard) kernels: compact fields of one parity hold the
   // sites of that parity in ascending full-lattice order (compact index =

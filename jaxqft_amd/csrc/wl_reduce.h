@@ -69,9 +69,14 @@ __device__ __forceinline__ void apply_action(int action, ColState& s, double sum
       s.rr = sum;
       s.beta = sum / s.rr_old;
       s.iter += 1;
+      // Stopping on the true residual, the iterated one must reach the target
+      // with the margin by which it last fell short of the true one (the
+      // inner precision's rounding separates them): the convergence check
+      // that follows then rarely fails and costs an extra update.
+      const double gap2 = s.mode == 1 ? 1.0 : fmax(1.0, s.gap2);
       if (!isfinite(sum)) {
         col_finish(s, WL_STATUS_NAN);
-      } else if (sum <= s.stop2) {
+      } else if (sum * gap2 <= s.stop2) {
         if (s.mode == 1) {
           col_finish(s, WL_STATUS_CONVERGED);
         } else {
@@ -93,6 +98,9 @@ __device__ __forceinline__ void apply_action(int action, ColState& s, double sum
       break;
     case RA_TRUERES: {
       s.rtrue2 = sum;
+      // rr is still the iterated residual that asked for this update (0 at
+      // the start of a solve)
+      if (s.rr > 0.0 && isfinite(sum)) s.gap2 = sum / s.rr;
       if (!isfinite(sum)) {
         col_finish(s, WL_STATUS_NAN);
       } else if (sum <= s.tol2b2) {

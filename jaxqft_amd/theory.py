@@ -128,6 +128,7 @@ class WilsonNf2Theory:
     poll_every: int = 8             # iterations per host poll (rounded up to an even number)
     use_graph: bool = False
     solver_accum: str = "auto"      # inner solution accumulator storage: auto (f32) | f64 | f32 | f16
+    solver_search: str = "recurrence"  # mixed precision: t = D r + beta t_prev (recurrence) | p = r + beta p (xpay)
     Nc: int = field(default=3, init=False)
     last_solve_stats: List[Dict[str, Any]] = field(default_factory=list, init=False, repr=False)
 
@@ -185,6 +186,8 @@ class WilsonNf2Theory:
             raise ValueError("solver_stop must be 'true' or 'iterated'")
         if self.solver_accum not in nat.ACCUM:
             raise ValueError("solver_accum must be one of: auto, f64, f32, f16")
+        if self.solver_search not in nat.SEARCH:
+            raise ValueError("solver_search must be 'recurrence' or 'xpay'")
         self.fermion_bc = parse_fermion_bc(self.fermion_bc, self.Nd, self._default_antiperiodic_t)
         self.dirac = WilsonDiracOperator(ndim=self.Nd, mass=self.mass, wilson_r=self.wilson_r, dtype=self.dtype)
         self.gamma = self.dirac.gamma
@@ -549,6 +552,7 @@ class WilsonNf2Theory:
             use_graph=self.use_graph,
             masses=masses,
             accum=self.solver_accum,
+            search=self.solver_search,
         )
         pol.update(overrides)
         return nat.solve_params(**pol)

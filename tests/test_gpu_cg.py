@@ -31,6 +31,8 @@ def true_resid(L, nc, mass, bc, U, b, x):
     dict(solver_precision="f16", solver_recon=12, use_graph=True),
     dict(solver_precision="f16", solver_recon=12, solver_accum="f16", reliable_delta=0.01),
     dict(solver_precision="f32", solver_accum="f64", use_graph=True),
+    dict(solver_precision="f16", solver_recon=12, solver_accum="f16", reliable_delta=0.01, solver_search="xpay"),
+    dict(solver_precision="f32", solver_search="xpay", use_graph=True),
 ])
 def test_config1_cg_4x4x4x4(policy):
     print(f"[gpu] cg config-1 {policy}", flush=True)
@@ -200,6 +202,57 @@ def test_mixed_solve_independent_of_block_length(prec):
                 assert true_resid(L, 3, 0.05, bc, U, b[:, j], x[:, j]) <= 1e-10
         else:
             assert got[1] == ref[1] and np.array_equal(got[0], ref[0]), (poll, graph)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f16"])
+def test_search_direction_forms_agree(prec):
+    """The mixed solve's two search-direction forms -- by recurrence (t = D r
+    + beta t_prev, p = r + beta p_prev formed in the D^dag launch; the
+    default) and in a pass of its own (p = r + beta p, t = D p) -- are the same
+    CG in exact arithmetic.  At 8^4 with three right-hand sides each form
+    meets the true residual 1e-10 by the oracle on every column, the two
+    solutions agree to solver accuracy, and their iteration counts differ by
+    rounding only.  The C ABI rejects any other form."""
+    print(f"[gpu] search-direction forms {prec}", flush=True)
+    import ctypes
+
+    import torch
+
+    from jaxqft_amd import _native as nat
+    from jaxqft_amd import fields
+    from jaxqft_amd.su3_wilson_nf2 import SU3WilsonNf2
+
+    L = (8, 8, 8, 8)
+    bc = (1, 1, 1, -1)
+    rng = fields.rng_for(690)
+    U = fields.su3_hot_start(rng, 1, L, 0.5)
+    b = fields.random_fermion(rng, (1, 3, *L, 4, 3))
+    got = {}
+    for form in ("recurrence", "xpay"):
+        th = SU3WilsonNf2(lattice_shape=L, mass=0.1, fermion_bc=bc, dtype=np.complex128, cg_tol=1e-10,
+                          cg_maxiter=3000, solver_precision=prec, solver_recon=12, reliable_delta=0.01,
+                          solver_accum="f16" if prec == "f16" else "auto", use_graph=True, solver_search=form)
+        x = np.asarray(th.solve_normal_multi(U, b))
+        st = th.last_solve_stats
+        assert all(c["status"] == "converged" for c in st), (form, st)
+        for j in range(3):
+            assert true_resid(L, 3, 0.1, bc, U, b[:, j], x[:, j]) <= 1e-10, (form, j)
+        got[form] = (x, [c["iters"] for c in st])
+    (xr, ir), (xp, ip) = got["recurrence"], got["xpay"]
+    assert np.linalg.norm(xr - xp) / np.linalg.norm(xp) < 1e-8
+    assert all(abs(a - c) <= 3 for a, c in zip(ir, ip)), (ir, ip)
+    # an undefined form is an argument error, reported before any device work
+    lib = nat.load()
+    ctx = nat.Context(L, 3, 1, 0.1, 1.0, bc)
+    g = nat.Gauge(ctx, torch.from_numpy(U).cuda(), nat.WL_LAYOUT_BMXYIJ)
+    fb, fx = nat.Field(ctx, nat.WL_F64, 1), nat.Field(ctx, nat.WL_F64, 1)
+    fb.import_(torch.from_numpy(b[:, 0]).cuda())
+    sp, keep = nat.solve_params(1e-10, 100, inner_prec=nat.PREC[prec])
+    sp.search_update = 7
+    st = (nat.WlSolveStats * 1)()
+    rc = lib.wl_cg_normal_field(ctx.h, g.h, fb.h, fx.h, 0, 1, ctypes.byref(sp), st, None)
+    assert rc == nat.WL_ERR_BAD_ARG and "search_update" in lib.wl_last_error().decode()
+    torch.cuda.synchronize()
 
 
 @pytest.mark.parametrize("perturb", [0.0, 1e-8])

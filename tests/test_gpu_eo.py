@@ -121,6 +121,7 @@ def _schur_normal_resid(ora, U, b, x):
     dict(solver_precision="f32", solver_recon=12),
     dict(solver_precision="f16", solver_recon=12, reliable_delta=0.01),
     dict(solver_precision="f16", solver_recon=18, solver_accum="f16"),
+    dict(solver_precision="f16", solver_recon=12, reliable_delta=0.01, solver_search="xpay"),
 ])
 @pytest.mark.parametrize("nc", [3, 1])
 def test_eo_cg_matches_oracle(policy, nc):

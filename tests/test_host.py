@@ -140,9 +140,13 @@ def test_theory_validation_and_shapes():
     for bad in (dict(layout="XYZ"), dict(solver_kind="minres"), dict(solver_form="split"),
                 dict(solver_form="eo_split"), dict(solver_form="direct", solver_kind="gmres"),
                 dict(preconditioner_kind="ilu"), dict(dirac_kernel="fast"), dict(fermion_bc=(1, 1)),
-                dict(solver_precision="f8"), dict(solver_recon=8)):
+                dict(solver_precision="f8"), dict(solver_recon=8), dict(solver_search="pipelined")):
         with pytest.raises(ValueError):
             SU3WilsonNf2(lattice_shape=(4,) * 4, **bad)
+    # the search-direction forms map onto the C ABI's values, recurrence by default
+    assert nat.solve_params(1e-10, 10)[0].search_update == nat.WL_SEARCH_RECURRENCE
+    assert nat.solve_params(1e-10, 10, search="xpay")[0].search_update == nat.WL_SEARCH_XPAY
+    assert nat.WlSolveParams.search_update.offset == nat.WlSolveParams.accum_prec.offset + 4
     # the reference's solver kinds and forms (su3_wilson_nf2.py:634-639): a split form needs a
     # non-Hermitian solver kind there; here every one runs CG on the normal equations
     for kind, form in (("cg", "normal"), ("bicgstab", "normal"), ("gmres", "normal"), ("bicgstab", "split"),
