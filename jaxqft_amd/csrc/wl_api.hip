@@ -24,7 +24,7 @@ hipError_t launch_xpay(int p, int nc, BlasArgs a, hipStream_t stream);
 hipError_t launch_accumulate(int px, int py, int pp, int nc, BlasArgs a, hipStream_t stream);
 hipError_t launch_convert(int po, int pi, int nc, BlasArgs a, hipStream_t stream);
 hipError_t launch_finalize_update(ColState* cols, int ncol, hipStream_t stream);
-hipError_t launch_trueres_from_b2(ColState* cols, int ncol, hipStream_t stream);
+hipError_t launch_trueres_from_b2(ColState* cols, int ncol, int then_finalize, hipStream_t stream);
 hipError_t launch_admit_finish(ColState* cols, int col, double tol2, hipStream_t stream);
 hipError_t pack_spinor(int prec, int nc, const void* src, int dtype, void* dst, long long V, long long Vp, int ncol,
                        size_t col_bytes, hipStream_t st);
@@ -1031,7 +1031,7 @@ struct Solver {
   void *r, *t, *y;
   void* pb[2];    // search directions, used in turn: iteration k reads pb[k % 2] and writes pb[(k + 1) % 2]
   void *mid64 = nullptr, *midP = nullptr;  // odd-parity intermediates of S_e (even-odd only)
-  Reduction red;
+  Reduction red{};  // copied into every launch's arguments: fields not set stay zero
   wl_ctx::Prof* prof = nullptr;  // kernel timing of host-driven iterations (null: off)
 
   // the four events of the next timed iteration, or null
@@ -1124,6 +1124,10 @@ struct Solver {
     d.col_bytes = cb64;
     CHK(apply(d, F64, R64, false, x, thi, mid64, EPI_STORE, RA_NONE));
     d.w = b;
+    // with the search direction by recurrence nothing runs between the true
+    // residual and the update's bookkeeping, so its reduction closes the
+    // update (one launch fewer)
+    d.red.then_finalize = rec ? 1 : 0;
     if (mixed) {
       // r = b - A x straight into the inner precision (no FP64 copy of it)
       d.res = r;
@@ -1132,7 +1136,8 @@ struct Solver {
     } else {
       CHK(apply(d, F64, R64, true, thi, rhi, mid64, EPI_RESID, RA_TRUERES));
     }
-    if (!rec) CHK(xpay(bargs(2), par, true));
+    if (rec) return WL_OK;
+    CHK(xpay(bargs(2), par, true));
     e = launch_finalize_update(red.cols, ncol, st);
     if (e != hipSuccess) return fail(WL_ERR_HIP, "finalize: %s", hipGetErrorString(e));
     return WL_OK;
@@ -1166,9 +1171,10 @@ struct Solver {
       e = launch_convert(mixed ? P : F64, F64, nc, a, st);
       if (e != hipSuccess) return fail(WL_ERR_HIP, "convert: %s", hipGetErrorString(e));
     }
-    e = launch_trueres_from_b2(red.cols, ncol, st);
+    e = launch_trueres_from_b2(red.cols, ncol, rec ? 1 : 0, st);
     if (e != hipSuccess) return fail(WL_ERR_HIP, "true residual: %s", hipGetErrorString(e));
-    if (!rec) CHK(xpay(bargs(2), 1, true));
+    if (rec) return WL_OK;
+    CHK(xpay(bargs(2), 1, true));
     e = launch_finalize_update(red.cols, ncol, st);
     if (e != hipSuccess) return fail(WL_ERR_HIP, "finalize: %s", hipGetErrorString(e));
     return WL_OK;

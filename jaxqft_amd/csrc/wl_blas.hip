@@ -136,7 +136,10 @@ __global__ __launch_bounds__(NT) void k_reduce_finish(Reduction red, int nblk, i
   double s = 0.0;
   for (int i = threadIdx.x; i < nblk; i += NT) s += parts[i];
   s = block_sum<NT>(s, lds);
-  if (threadIdx.x == 0) apply_action(red.action, red.cols[col], s, red.par);
+  if (threadIdx.x == 0) {
+    apply_action(red.action, red.cols[col], s, red.par);
+    if (red.then_finalize) finalize_col(red.cols[col]);
+  }
 }
 
 hipError_t launch_reduce_finish(const Reduction& red, int ncol, int nblk, int gate, hipStream_t stream) {
@@ -163,29 +166,20 @@ hipError_t launch_admit_finish(ColState* cols, int col, double tol2, hipStream_t
 // First update of a solve from a zero initial guess: the true residual is b
 // itself, so its norm is |b|^2 and the convergence decision of a true-residual
 // reduction is taken on that (a zero right-hand side converges here).
-__global__ void k_trueres_from_b2(ColState* cols, int ncol) {
+__global__ void k_trueres_from_b2(ColState* cols, int ncol, int then_finalize) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncol) return;
   ColState& s = cols[c];
   if (!(s.need_update && !s.done)) return;
   apply_action(RA_TRUERES, s, s.b2);
+  if (then_finalize) finalize_col(s);
 }
 
 // After a reliable update: restart the iterated residual from the true one.
 __global__ void k_finalize_update(ColState* cols, int ncol) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncol) return;
-  ColState& s = cols[c];
-  if (!s.need_update) return;
-  s.pend = 0;  // the update's x += y took any alpha p held back
-  if (!s.done) {
-    s.rr = s.rtrue2;
-    s.r0norm = sqrt(s.rtrue2);
-    s.nupdates += 1;
-    s.last_update = s.iter;
-    s.active = 1;
-  }
-  s.need_update = 0;
+  finalize_col(cols[c]);
 }
 
 // ---------------------------------------------------------------------------
@@ -249,8 +243,8 @@ hipError_t launch_convert(int po, int pi, int nc, BlasArgs a, hipStream_t stream
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_trueres_from_b2(ColState* cols, int ncol, hipStream_t stream) {
-  hipLaunchKernelGGL(k_trueres_from_b2, dim3((ncol + 63) / 64), dim3(64), 0, stream, cols, ncol);
+hipError_t launch_trueres_from_b2(ColState* cols, int ncol, int then_finalize, hipStream_t stream) {
+  hipLaunchKernelGGL(k_trueres_from_b2, dim3((ncol + 63) / 64), dim3(64), 0, stream, cols, ncol, then_finalize);
   return hipGetLastError();
 }
 

@@ -69,6 +69,7 @@ struct Reduction {
   int action;
   int only_col;          // column of gate 3
   int par;               // RA_ITER: search-direction buffer of the iteration (pairwise y updates)
+  int then_finalize;     // RA_TRUERES: close the reliable update in the same kernel (finalize_col)
 };
 
 // Stencil epilogues.  EPI_NORM and the CG steps feed the fused CG iteration

@@ -118,6 +118,22 @@ __device__ __forceinline__ void apply_action(int action, ColState& s, double sum
   }
 }
 
+// Closes a reliable update of a column that asked for one: the iterated
+// residual restarts from the true one (a held-back alpha p went into the
+// update's x += y), and the column iterates again unless it has finished.
+__device__ __forceinline__ void finalize_col(ColState& s) {
+  if (!s.need_update) return;
+  s.pend = 0;
+  if (!s.done) {
+    s.rr = s.rtrue2;
+    s.r0norm = sqrt(s.rtrue2);
+    s.nupdates += 1;
+    s.last_update = s.iter;
+    s.active = 1;
+  }
+  s.need_update = 0;
+}
+
 // Phase 1, called by every thread of the workgroup: write this workgroup's
 // FP64 partial (plain store; the kernel boundary publishes it).
 __device__ __forceinline__ void write_partial(const Reduction& red, int col, int blk, int nblk, double v) {
