@@ -33,6 +33,7 @@ def true_resid(L, nc, mass, bc, U, b, x):
     dict(solver_precision="f32", solver_accum="f64", use_graph=True),
     dict(solver_precision="f16", solver_recon=12, solver_accum="f16", reliable_delta=0.01, solver_search="xpay"),
     dict(solver_precision="f32", solver_search="xpay", use_graph=True),
+    dict(solver_precision="f16", solver_recon=12, solver_stop="iterated"),
 ])
 def test_config1_cg_4x4x4x4(policy):
     print(f"[gpu] cg config-1 {policy}", flush=True)
