@@ -345,6 +345,39 @@ def test_multi_rhs_same_alone_and_in_the_mix(prec):
         assert true_resid(L, 3, masses[j], (1, 1, 1, -1), U, rhs[:, j], X[:, j]) <= 1e-10
 
 
+@pytest.mark.parametrize("search", ["recurrence", "xpay"])
+@pytest.mark.parametrize("prec", ["f32", "f16"])
+def test_mixed_batch_with_zero_and_nan_columns(prec, search):
+    """Columns that stop at once share a mixed solve with ordinary ones: a zero
+    source converges at its first true residual (x = 0 exactly, no search
+    direction is ever formed), a source holding a NaN stops with status nan,
+    and the two Gaussian columns beside them are bit-identical to the same
+    sources solved alone and meet the oracle's true residual 1e-10."""
+    print(f"[gpu] zero and nan columns {prec} {search}", flush=True)
+    from jaxqft_amd import fields
+    from jaxqft_amd.su3_wilson_nf2 import SU3WilsonNf2
+
+    L = (4, 4, 4, 8)
+    bc = (1, 1, 1, -1)
+    rng = fields.rng_for(710)
+    U = fields.su3_hot_start(rng, 1, L, 0.5)
+    rhs = fields.random_fermion(rng, (1, 4, *L, 4, 3))
+    rhs[0, 1] = 0.0
+    rhs[0, 2, 1, 2, 3, 4, 0, 2] = np.nan
+    th = SU3WilsonNf2(lattice_shape=L, mass=0.05, fermion_bc=bc, dtype=np.complex128, cg_tol=1e-10,
+                      cg_maxiter=3000, solver_precision=prec, solver_recon=12, reliable_delta=0.01,
+                      solver_search=search, use_graph=True)
+    X = np.asarray(th.solve_normal_multi(U, rhs))
+    st = list(th.last_solve_stats)
+    assert [s["status"] for s in st] == ["converged", "converged", "nan", "converged"], st
+    assert np.all(X[:, 1] == 0) and st[1]["iters"] == 0
+    for j in (0, 3):
+        xj = np.asarray(th.solve_normal_multi(U, rhs[:, j:j + 1]))
+        assert np.array_equal(xj[:, 0], X[:, j]), j
+        assert th.last_solve_stats[0]["iters"] == st[j]["iters"]
+        assert true_resid(L, 3, 0.05, bc, U, rhs[:, j], X[:, j]) <= 1e-10
+
+
 @pytest.mark.parametrize("prec", ["f64", "f32", "f16"])
 @pytest.mark.parametrize("march,slab,box", [(2, 0, 0), (8, 0, 0), (1, 2, 0), (1, 0, 1)])
 def test_cg_with_marching_stencil(prec, march, slab, box):
