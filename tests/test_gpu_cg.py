@@ -346,6 +346,33 @@ def test_multi_rhs_same_alone_and_in_the_mix(prec):
 
 
 @pytest.mark.parametrize("search", ["recurrence", "xpay"])
+def test_mixed_maxiter_reports_the_true_residual(search):
+    """A mixed solve stopped by cg_maxiter ends with one FP64 true-residual
+    evaluation: status maxiter after exactly maxiter iterations, the x it
+    returns holds every iteration's correction, and the reported true
+    residual is the oracle's residual of that x."""
+    print(f"[gpu] mixed maxiter {search}", flush=True)
+    from jaxqft_amd import fields
+    from jaxqft_amd.su3_wilson_nf2 import SU3WilsonNf2
+
+    L = (4, 4, 4, 8)
+    bc = (1, 1, 1, -1)
+    rng = fields.rng_for(720)
+    U = fields.su3_hot_start(rng, 1, L, 0.5)
+    b = fields.random_fermion(rng, (1, *L, 4, 3))
+    for maxiter in (7, 12):
+        th = SU3WilsonNf2(lattice_shape=L, mass=0.05, fermion_bc=bc, dtype=np.complex128, cg_tol=1e-10,
+                          cg_maxiter=maxiter, solver_precision="f16", solver_recon=12, reliable_delta=0.01,
+                          solver_search=search)
+        x = th.solve_normal(U, b)
+        st = th.last_solve_stats[0]
+        assert st["status"] == "maxiter" and st["iters"] == maxiter, st
+        res = true_resid(L, 3, 0.05, bc, U, b, x)
+        assert 1e-10 < res < 1.0
+        assert abs(res - st["true_resid"]) <= 1e-6 * res, (res, st)
+
+
+@pytest.mark.parametrize("search", ["recurrence", "xpay"])
 @pytest.mark.parametrize("prec", ["f32", "f16"])
 def test_mixed_batch_with_zero_and_nan_columns(prec, search):
     """Columns that stop at once share a mixed solve with ordinary ones: a zero
