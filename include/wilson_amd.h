@@ -297,6 +297,13 @@ int wl_eo_apply(wl_ctx* ctx, wl_gauge* g, int op, int normalized, const void* in
  * F: (batch, 4, *L, Nc, Nc) for WL_LAYOUT_BMXYIJ or (batch, *L, 4, Nc, Nc) for
  * WL_LAYOUT_BXYMIJ (U(1): no (Nc, Nc) axes); all three of element type dtype. */
 int wl_pf_force(wl_ctx* ctx, wl_gauge* g, const void* X, const void* Y, void* F, int layout, int dtype, void* stream);
+/* The same force from resident FP64 fields X and Y (one column per
+ * configuration, e.g. X from wl_cg_normal_field and Y = D X from wl_dslash),
+ * so that a monomial force step (su3_wilson_nf2.py:378-392) keeps its
+ * solution on the device in the internal layout; F as for wl_pf_force.
+ * Results are identical to wl_pf_force on the exported fields. */
+int wl_pf_force_field(wl_ctx* ctx, wl_gauge* g, const wl_field* X, const wl_field* Y, void* F, int layout, int dtype,
+                      void* stream);
 /* Solve S_e^dag S_e x_e = P_e b with the normalized even Schur complement
  * (replaces solve_eop_even_normal / solve_eop_even_normal_with_guess,
  * su3_wilson_nf2.py:1373-1392); same solver parameters and statistics as

@@ -136,6 +136,7 @@ def load() -> ctypes.CDLL:
         "wl_cg_normal_field": (c_int, [vp, vp, vp, vp, c_int, c_int, POINTER(WlSolveParams), POINTER(WlSolveStats), vp]),
         "wl_eo_apply": (c_int, [vp, vp, c_int, c_int, vp, vp, c_int, c_int, vp]),
         "wl_pf_force": (c_int, [vp, vp, vp, vp, vp, c_int, c_int, vp]),
+        "wl_pf_force_field": (c_int, [vp, vp, vp, vp, vp, c_int, c_int, vp]),
         "wl_cg_eo_normal": (c_int, [vp, vp, vp, vp, vp, c_int, c_int, POINTER(WlSolveParams), POINTER(WlSolveStats), vp]),
         "wl_lime_decode": (c_int, [vp, c_size_t, c_int, POINTER(c_int), c_int, c_int, c_int, c_int, vp, c_int, vp]),
         "wl_ctx_set_profiling": (c_int, [vp, c_int]),

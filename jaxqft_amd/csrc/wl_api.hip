@@ -1678,6 +1678,40 @@ int wl_pf_force(wl_ctx* c, wl_gauge* g, const void* X, const void* Y, void* F, i
   return WL_OK;
 }
 
+int wl_pf_force_field(wl_ctx* c, wl_gauge* g, const wl_field* X, const wl_field* Y, void* F, int layout, int dtype,
+                      void* stream) {
+  CHK(check_ctx(c));
+  if (!g || g->ctx != c || !X || !Y || X->ctx != c || Y->ctx != c)
+    return fail(WL_ERR_BAD_ARG, "null or foreign handle");
+  if (!F) return fail(WL_ERR_BAD_ARG, "null force pointer");
+  if (X->prec != F64 || Y->prec != F64) return fail(WL_ERR_BAD_ARG, "X and Y must be FP64 fields");
+  if (X->ncol != c->geom.batch || Y->ncol != c->geom.batch)
+    return fail(WL_ERR_BAD_ARG, "X and Y must have one column per configuration (%d)", c->geom.batch);
+  if (dtype != WL_C128 && dtype != WL_C64) return fail(WL_ERR_BAD_ARG, "bad dtype %d", dtype);
+  if (layout != WL_LAYOUT_BMXYIJ && layout != WL_LAYOUT_BXYMIJ) return fail(WL_ERR_BAD_ARG, "bad layout %d", layout);
+  std::lock_guard<std::mutex> lk(c->mu);
+  StreamOrder order(c, (hipStream_t)stream);  // X, Y may come from a call on another stream
+  ForceArgs a;
+  memset(&a, 0, sizeof(a));
+  a.g = c->g;
+  a.U = g->master.p;
+  a.cfg_bytes = cfg_bytes(c, F64, 18);
+  a.dir_bytes = a.cfg_bytes / 4;
+  a.X = X->buf.p;
+  a.Y = Y->buf.p;
+  a.F = F;
+  a.layout = layout;
+  a.col_bytes = X->col_bytes;
+  for (int i = 0; i < 4; ++i) {
+    a.bc_re[i] = c->params.bc_re[i];
+    a.bc_im[i] = c->params.bc_im[i];
+  }
+  a.r = c->params.wilson_r;
+  hipError_t e = launch_pf_force(c->geom.nc, dtype, a, c->geom.batch, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(WL_ERR_HIP, "force: %s", hipGetErrorString(e));
+  return WL_OK;
+}
+
 int wl_cg_eo_normal(wl_ctx* c, wl_gauge* g, const void* b, void* x, const void* x0, int nrhs, int dtype,
                     const wl_solve_params* sp, wl_solve_stats* stats, void* stream) {
   CHK(check_ctx(c));

@@ -5,10 +5,10 @@
 //   c2[d][c] = sum_{s,t,a} conj(Y(x+mu)[s][a]) (1 + g_mu)[s][t] conj(U~_mu(x)[d][a]) X(x)[t][c]
 //   F_mu(x) = TA(k),  k[i][j] = c1[j][i] - c2[j][i],
 // TA the traceless anti-Hermitian part (U(1): (k - conj k) / 2), U~ the link
-// with the fermion boundary phase (Wilson r = 1).  One thread per link; X, Y
-// are read from and F written to the reference layouts directly (a site's
-// spinor and a link's matrix are contiguous, so a wavefront's accesses are
-// contiguous runs).
+// with the fermion boundary phase (Wilson r = 1).  One thread per link; F is
+// written in the reference layout, X and Y are read from the reference layout
+// (a site's spinor and a link's matrix are contiguous, so a wavefront's
+// accesses are contiguous runs) or from resident FP64 fields.
 #include "wl_device.h"
 #include "wl_kernels.h"
 #include "wl_stencil.h"
@@ -24,6 +24,21 @@ __device__ __forceinline__ void load_aos(const S* __restrict__ base, size_t idx,
     for (int c = 0; c < NC; ++c) v[s][c] = {(double)p[2 * (s * NC + c)], (double)p[2 * (s * NC + c) + 1]};
 }
 
+// The spinor of configuration cfg at site idx: from the reference layout, or
+// (SOA) from a resident FP64 field, whose planes give each wavefront's load
+// one contiguous run.  Same values either way.
+template <bool SOA, class S, int NC>
+__device__ __forceinline__ void load_spinor(const ForceArgs& a, const void* base, int cfg, size_t idx,
+                                            cx<double> (&v)[4][NC]) {
+  if constexpr (SOA) {
+    const char* col = reinterpret_cast<const char*>(base) + (size_t)cfg * a.col_bytes;
+    SpinorIO<F64, NC>::load(col, (size_t)a.g.Vp, idx, reinterpret_cast<cx<double>(&)[4 * NC]>(v));
+  } else {
+    const S* b = reinterpret_cast<const S*>(base) + (size_t)cfg * (size_t)a.g.V * (4 * NC) * 2;
+    load_aos<S, NC>(b, idx, v);
+  }
+}
+
 // One link's force through the projected rows.  With r = 1, (1 + S g_mu) =
 // 2 P (P a Hermitian projector), and every spin row rebuilt from the two
 // independent rows h0, h1 of spin_project carries a unimodular coefficient,
@@ -35,11 +50,9 @@ __device__ __forceinline__ void load_aos(const S* __restrict__ base, size_t idx,
 //   c2[d][c] = w sum_k conj((U hY'_k)[d]) hX'_k[c], hY' = rows of (1 + g) Y(x+mu), hX' = rows of (1 + g) X(x)
 // Each spinor is projected as soon as it is loaded, so only the link, the
 // accumulator and one pair of projected rows are live at a time.
-template <int MU, int NC, class S>
+template <int MU, int NC, class S, bool SOA>
 __device__ __forceinline__ void force_link(const ForceArgs& a, int cfg, size_t site, const int (&x)[4]) {
   const size_t V = (size_t)a.g.V;
-  const S* Xb = reinterpret_cast<const S*>(a.X) + (size_t)cfg * V * (4 * NC) * 2;
-  const S* Yb = reinterpret_cast<const S*>(a.Y) + (size_t)cfg * V * (4 * NC) * 2;
   const int L = a.g.L[MU];
   const size_t str = (size_t)a.g.stride[MU];
   const bool top = x[MU] == L - 1;
@@ -60,12 +73,12 @@ __device__ __forceinline__ void force_link(const ForceArgs& a, int cfg, size_t s
     cx<double> hY[2][NC], uh[2][NC];
     {
       cx<double> t[4][NC];
-      load_aos<S, NC>(Yb, site, t);
+      load_spinor<SOA, S, NC>(a, a.Y, cfg, site, t);
       spin_project<MU, -1>(reinterpret_cast<const cx<double>(&)[4 * NC]>(t), hY);
     }
     {
       cx<double> t[4][NC], hX[2][NC];
-      load_aos<S, NC>(Xb, sp, t);
+      load_spinor<SOA, S, NC>(a, a.X, cfg, sp, t);
       spin_project<MU, -1>(reinterpret_cast<const cx<double>(&)[4 * NC]>(t), hX);
       color_mul<false>(u, hX, uh);
     }
@@ -81,13 +94,13 @@ __device__ __forceinline__ void force_link(const ForceArgs& a, int cfg, size_t s
     cx<double> uy[2][NC], hX0[2][NC];
     {
       cx<double> t[4][NC], hYp[2][NC];
-      load_aos<S, NC>(Yb, sp, t);
+      load_spinor<SOA, S, NC>(a, a.Y, cfg, sp, t);
       spin_project<MU, +1>(reinterpret_cast<const cx<double>(&)[4 * NC]>(t), hYp);
       color_mul<false>(u, hYp, uy);
     }
     {
       cx<double> t[4][NC];
-      load_aos<S, NC>(Xb, site, t);
+      load_spinor<SOA, S, NC>(a, a.X, cfg, site, t);
       spin_project<MU, +1>(reinterpret_cast<const cx<double>(&)[4 * NC]>(t), hX0);
     }
 #pragma unroll
@@ -128,7 +141,7 @@ __device__ __forceinline__ void force_link(const ForceArgs& a, int cfg, size_t s
 // path); the four wavefronts read the same 64 sites' X and Y (L1/L2 reuse).
 constexpr int kForceSites = NT / 4;
 
-template <int NC, class S>
+template <int NC, class S, bool SOA>
 __global__ __launch_bounds__(NT) void k_pf_force(ForceArgs a) {
   const int mu = (int)(threadIdx.x / kForceSites);
   const size_t site = (size_t)blockIdx.x * kForceSites + (threadIdx.x % kForceSites);
@@ -145,22 +158,29 @@ __global__ __launch_bounds__(NT) void k_pf_force(ForceArgs a) {
     x[0] = (int)(rem / a.g.L[1]);
   }
   switch (mu) {
-    case 0: force_link<0, NC, S>(a, cfg, site, x); break;
-    case 1: force_link<1, NC, S>(a, cfg, site, x); break;
-    case 2: force_link<2, NC, S>(a, cfg, site, x); break;
-    default: force_link<3, NC, S>(a, cfg, site, x); break;
+    case 0: force_link<0, NC, S, SOA>(a, cfg, site, x); break;
+    case 1: force_link<1, NC, S, SOA>(a, cfg, site, x); break;
+    case 2: force_link<2, NC, S, SOA>(a, cfg, site, x); break;
+    default: force_link<3, NC, S, SOA>(a, cfg, site, x); break;
   }
 }
 
+template <bool SOA>
+static void launch_pf_force_t(int nc, int dtype, const ForceArgs& a, dim3 grid, dim3 block, hipStream_t st) {
+  if (nc == 3) {
+    if (dtype == 0) hipLaunchKernelGGL((k_pf_force<3, double, SOA>), grid, block, 0, st, a);
+    else hipLaunchKernelGGL((k_pf_force<3, float, SOA>), grid, block, 0, st, a);
+  } else {
+    if (dtype == 0) hipLaunchKernelGGL((k_pf_force<1, double, SOA>), grid, block, 0, st, a);
+    else hipLaunchKernelGGL((k_pf_force<1, float, SOA>), grid, block, 0, st, a);
+  }
+}
+
+// X, Y in the reference layout (a.col_bytes == 0) or as resident FP64 fields
 hipError_t launch_pf_force(int nc, int dtype, const ForceArgs& a, int ncfg, hipStream_t st) {
   dim3 grid((unsigned)((a.g.V + kForceSites - 1) / kForceSites), (unsigned)ncfg), block(NT);
-  if (nc == 3) {
-    if (dtype == 0) hipLaunchKernelGGL((k_pf_force<3, double>), grid, block, 0, st, a);
-    else hipLaunchKernelGGL((k_pf_force<3, float>), grid, block, 0, st, a);
-  } else {
-    if (dtype == 0) hipLaunchKernelGGL((k_pf_force<1, double>), grid, block, 0, st, a);
-    else hipLaunchKernelGGL((k_pf_force<1, float>), grid, block, 0, st, a);
-  }
+  if (a.col_bytes) launch_pf_force_t<true>(nc, dtype, a, grid, block, st);
+  else launch_pf_force_t<false>(nc, dtype, a, grid, block, st);
   return hipGetLastError();
 }
 

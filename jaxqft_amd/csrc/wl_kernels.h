@@ -162,6 +162,8 @@ struct ForceArgs {
   const void* Y;
   void* F;           // (B, 4, *L, NC, NC) (layout 0) or (B, *L, 4, NC, NC) (layout 1); U(1): no (NC, NC)
   int layout;
+  size_t col_bytes;  // X, Y as resident FP64 fields (internal layout, one column per configuration):
+                     // bytes of one column; 0: X, Y in the reference layout
   double bc_re[4], bc_im[4];
   double r;
 };

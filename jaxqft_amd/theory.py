@@ -195,6 +195,7 @@ class WilsonNf2Theory:
         self._m0 = float(self.mass + self.wilson_r * self.Nd)
         self._rng = _fields.rng_for(self.seed)
         self._ctx: Optional[nat.Context] = None
+        self._force_fields = None  # (ctx, b, x, y): resident FP64 fields of the force step
 
     # ------------------------------------------------------------------ shapes
     def fermion_shape(self) -> Tuple[int, ...]:
@@ -526,15 +527,80 @@ class WilsonNf2Theory:
         use_solver_guess the solve starts from the guess of solver_guess_mode
         over the last solver_guess_history (X, Y) pairs ('mre' fits eta,
         phi = D^dag eta, when given).  A split solver form takes (X, Y) from
-        solve_split_with_intermediate, as the reference does."""
+        solve_split_with_intermediate, as the reference does.
+
+        The normal form runs on resident FP64 fields (_force_resident): X and
+        Y stay in the library's layout from the solve through D to the force,
+        with the same results as the three separate calls."""
         if self.solver_form in ("split", "eo_split"):
             X, Y = self.solve_split_with_intermediate(U, phi)
             return self.pseudofermion_force_from_solution(U, X, Y)
-        X = self._solve_with_history(U, phi, eta)
-        Y = self.apply_D(U, X)
+        x0 = self.solver_guess.predict(phi, eta) if self.use_solver_guess else None
+        F, X, Y = self._force_resident(U, phi, x0, keep_xy=self.use_solver_guess)
         if self.use_solver_guess:
             self.solver_guess.update(X, Y)
-        return self.pseudofermion_force_from_solution(U, X, Y)
+        return F
+
+    def _resident_fields(self, ctx):
+        """The (b, x, y) FP64 fields of the force step, one column per
+        configuration, kept across calls on the same context."""
+        if self._force_fields is None or self._force_fields[0] is not ctx:
+            self._force_fields = (ctx,) + tuple(nat.Field(ctx, nat.WL_F64, self.Bs) for _ in range(3))
+        return self._force_fields[1:]
+
+    def _force_resident(self, U, phi, x0, keep_xy: bool = False):
+        """X = (D^dag D)^-1 phi (from x0 when given), Y = D X and the force from
+        them on resident fields: wl_cg_normal_field, wl_dslash and
+        wl_pf_force_field, so X and Y are never converted to the reference
+        layout and back.  Returns (F, X, Y) with X and Y exported only when
+        keep_xy (else None).  Same arithmetic as solve_normal, apply_D and
+        pseudofermion_force_from_solution in turn."""
+        self._check_fermion(phi, "phi")
+        ctx, g = self._gauge(U)
+        pt, kind, orig = to_device(phi)
+        x0t = None
+        if x0 is not None:
+            self._check_fermion(x0, "x0")
+            x0t, _, _ = to_device(x0, dtype=pt.dtype)
+        fb, fx, fy = self._resident_fields(ctx)
+        sp, keep = self._solve_params()
+        stats = (nat.WlSolveStats * self.Bs)()
+        dt = nat.WL_C128 if pt.dtype == torch.complex128 else nat.WL_C64
+        F = torch.empty(tuple(self.field_shape()), dtype=pt.dtype, device=pt.device)
+        X = Y = None
+        lib = ctx._lib
+
+        def run():
+            h = nat.c_void_p(nat.current_stream())
+            fb.import_(pt)
+            if x0t is not None:
+                fx.import_(x0t)
+            nat.check(lib.wl_cg_normal_field(ctx.h, g.h, fb.h, fx.h, 1 if x0t is not None else 0, 1, nat.byref(sp),
+                                             stats, h))
+            nat.check(lib.wl_dslash(ctx.h, g.h, nat.WL_OP_D, 18, fx.h, fy.h, 1, h))
+            nat.check(lib.wl_pf_force_field(ctx.h, g.h, fx.h, fy.h, nat.c_void_p(F.data_ptr()), self._layout_code(),
+                                            dt, h))
+            if keep_xy:
+                xs, ys = torch.empty_like(pt), torch.empty_like(pt)
+                fx.export(xs)
+                fy.export(ys)
+                return xs, ys
+            return None, None
+
+        if sp.use_graph and nat.current_stream() == 0:
+            # graph capture needs a stream other than the default one
+            side = torch.cuda.Stream()
+            side.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(side):
+                X, Y = run()
+            torch.cuda.current_stream().wait_stream(side)
+        else:
+            X, Y = run()
+        del keep
+        self.last_solve_stats = nat.stats_list(stats, self.Bs)
+        if keep_xy:
+            X, Y = from_device(X, kind, orig), from_device(Y, kind, orig)
+        return from_device(F, kind, orig), X, Y
 
     def _m0_value(self) -> float:
         return float(self.mass) + float(self.wilson_r) * self.Nd

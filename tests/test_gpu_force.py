@@ -101,6 +101,68 @@ def test_monomial_force_composition():
     assert rel(F, ref) < 1e-12
 
 
+@pytest.mark.parametrize("layout", ["BMXYIJ", "BXYMIJ"])
+@pytest.mark.parametrize("nc", [3, 1])
+def test_force_on_resident_fields(nc, layout):
+    """The monomial force runs solve, D and force on resident FP64 fields
+    (wl_cg_normal_field, wl_dslash, wl_pf_force_field): in complex128 it equals
+    solve_normal, apply_D and the force from the solution in turn bit for bit,
+    from a zero start and from a chronological guess; wl_pf_force_field on the
+    fields equals wl_pf_force on their exported arrays bit for bit, and checks
+    its handles."""
+    print(f"[gpu] resident force nc={nc} {layout}", flush=True)
+    import ctypes
+
+    import torch
+
+    from jaxqft_amd import _native as nat
+    from jaxqft_amd import fields
+
+    L = (4, 6, 4, 8)
+    bc = (np.exp(0.3j), 1, -1, -1)
+    rng = fields.rng_for(1840 + nc)
+    U = fields.su3_hot_start(rng, 2, L, 0.6, layout=layout) if nc == 3 else fields.u1_hot_start(rng, 2, L, 0.6,
+                                                                                                 layout=layout)
+    phi = fields.random_fermion(rng, (2, *L, 4, nc))
+    th = theory(L, nc, 0.05, bc, batch=2, layout=layout, cg_tol=1e-12, cg_maxiter=3000)
+    F = th.pseudofermion_force(U, phi)
+    X = th.solve_normal(U, phi)
+    Y = th.apply_D(U, X)
+    assert np.array_equal(F, th.pseudofermion_force_from_solution(U, X, Y))
+    # from a guess: X0 = X + a small perturbation, as a chronological predictor gives
+    X0 = X + 1e-3 * fields.random_fermion(rng, X.shape)
+    Fg = th._force_resident(U, phi, X0, keep_xy=True)
+    Xg = th.solve_normal_with_guess(U, phi, X0)
+    assert np.array_equal(Fg[1], Xg) and np.array_equal(Fg[2], th.apply_D(U, Xg))
+    assert np.array_equal(Fg[0], th.pseudofermion_force_from_solution(U, Xg, Fg[2]))
+    # the entry point itself: complex128 against wl_pf_force on the arrays;
+    # complex64 output is the FP64 force rounded once at the store
+    ctx, g = th._gauge(U)
+    fx, fy = nat.Field(ctx, nat.WL_F64, 2), nat.Field(ctx, nat.WL_F64, 2)
+    Xt, Yt = torch.from_numpy(X).cuda(), torch.from_numpy(Y).cuda()
+    fx.import_(Xt)
+    fy.import_(Yt)
+    lay = th._layout_code()
+    Ff = torch.empty(th.field_shape(), dtype=torch.complex128, device="cuda")
+    Fa = torch.empty_like(Ff)
+    Ff32 = torch.empty(th.field_shape(), dtype=torch.complex64, device="cuda")
+    vp = ctypes.c_void_p
+    nat.check(ctx._lib.wl_pf_force_field(ctx.h, g.h, fx.h, fy.h, vp(Ff.data_ptr()), lay, nat.WL_C128, None))
+    nat.check(ctx._lib.wl_pf_force(ctx.h, g.h, vp(Xt.data_ptr()), vp(Yt.data_ptr()), vp(Fa.data_ptr()), lay,
+                                   nat.WL_C128, None))
+    nat.check(ctx._lib.wl_pf_force_field(ctx.h, g.h, fx.h, fy.h, vp(Ff32.data_ptr()), lay, nat.WL_C64, None))
+    torch.cuda.synchronize()
+    assert torch.equal(Ff, Fa)
+    assert torch.equal(Ff32, Fa.to(torch.complex64))
+    # handle checks: an FP32 field, a field with the wrong column count
+    f32 = nat.Field(ctx, nat.WL_F32, 2)
+    f3 = nat.Field(ctx, nat.WL_F64, 3)
+    for a, b in ((f32, fy), (fx, f3)):
+        rc = ctx._lib.wl_pf_force_field(ctx.h, g.h, a.h, b.h, vp(Ff.data_ptr()), lay, nat.WL_C128, None)
+        assert rc == nat.WL_ERR_BAD_ARG
+    assert "column" in ctx._lib.wl_last_error().decode()
+
+
 def test_force_is_action_derivative_16x4():
     """At 16^4, with the action S = phi^dag (D^dag D)^-1 phi evaluated by the GPU
     solver (FP64, tol 1e-13), (S(eps) - S(-eps)) / 2 eps for U -> exp(eps H) U

@@ -82,6 +82,7 @@ def test_every_entry_point_rejects_null_handles():
         "wl_cg_normal_field": lambda: lib.wl_cg_normal_field(None, None, None, None, 0, 1, ctypes.byref(sp), None, None),
         "wl_eo_apply": lambda: lib.wl_eo_apply(None, None, 0, 1, None, None, 1, 0, None),
         "wl_pf_force": lambda: lib.wl_pf_force(None, None, None, None, None, 0, 0, None),
+        "wl_pf_force_field": lambda: lib.wl_pf_force_field(None, None, None, None, None, 0, 0, None),
         "wl_cg_eo_normal": lambda: lib.wl_cg_eo_normal(None, None, None, None, None, 1, 0, ctypes.byref(sp), None,
                                                        None),
         "wl_lime_decode": lambda: lib.wl_lime_decode(None, 0, 8, dims, 4, 1, 12, 0, None, 0, None),
