@@ -175,6 +175,8 @@ class WilsonNf2Theory:
         self.solver_guess_mode = str(self.solver_guess_mode).lower()
         self.solver_guess_history = int(self.solver_guess_history)
         self.solver_guess = SolverGuess(self.solver_guess_mode, self.solver_guess_history)
+        # the even-odd monomial keeps a history of its own (su3_wilson_nf2.py:416-421)
+        self.eop_solver_guess = SolverGuess(self.solver_guess_mode, self.solver_guess_history)
         if self.pseudofermion_gamma is None:
             refresh = str(self.pseudofermion_refresh).lower()
             self.pseudofermion_gamma = float(self.smd_gamma) if refresh == "ou" else 0.3
@@ -518,8 +520,29 @@ class WilsonNf2Theory:
         return self.solve_normal(U, phi) if x0 is None else self.solve_normal_with_guess(U, phi, x0)
 
     def clear_solver_guess(self) -> None:
-        """Forget the solution history (a pseudofermion refresh, su3_wilson_nf2.py:318-320)."""
+        """Forget the solution histories (a pseudofermion refresh, su3_wilson_nf2.py:318-320, :432-435)."""
         self.solver_guess.clear()
+        self.eop_solver_guess.clear()
+
+    def eop_pseudofermion_force(self, U, phi, eta=None):
+        """The even-odd monomial's analytic force (su3_wilson_nf2.py:566-579,
+        u1_wilson_nf2.py:275-281): x_e = (S^dag S)^-1 phi_e with the normalized
+        even Schur complement, y_e = S x_e, then
+        eop_pseudofermion_force_from_solution.  With use_solver_guess the solve
+        starts from the guess of eop_solver_guess, a history separate from the
+        standard monomial's ('mre' fits eta, phi = S^dag eta, when given).  A
+        split solver form takes (x_e, y_e) from
+        solve_eop_even_split_with_intermediate, as the reference does."""
+        self._check_fermion(phi, "phi")
+        if self.solver_form in ("split", "eo_split"):
+            Xe, Ye = self.solve_eop_even_split_with_intermediate(U, phi)
+            return self.eop_pseudofermion_force_from_solution(U, Xe, Ye)
+        x0 = self.eop_solver_guess.predict(phi, eta) if self.use_solver_guess else None
+        Xe = self.solve_eop_even_normal(U, phi) if x0 is None else self.solve_eop_even_normal_with_guess(U, phi, x0)
+        Ye = self.apply_eo_schur_even(U, Xe, normalized=True)
+        if self.use_solver_guess:
+            self.eop_solver_guess.update(Xe, Ye)
+        return self.eop_pseudofermion_force_from_solution(U, Xe, Ye)
 
     def pseudofermion_force(self, U, phi, eta=None):
         """The analytic monomial force: X = solve_normal(U, phi), Y = D X, then

@@ -84,6 +84,42 @@ def test_eo_force_matches_oracle(nc):
     assert rel(got, ora.eop_pseudofermion_force_from_solution(U, Xe, Ye)) < 1e-12
 
 
+@pytest.mark.parametrize("nc", [3, 1])
+def test_eo_monomial_force(nc):
+    """The even-odd monomial's force from phi (su3_wilson_nf2.py:566-579): x_e
+    solves S^dag S x_e = phi_e to the solver tolerance (oracle operator), the
+    force equals the oracle's force from (x_e, S x_e); the eo_split form gives
+    the same force, and a second call from the chronological guess (a history
+    separate from the standard monomial's) starts at the solution."""
+    print(f"[gpu] eo monomial force nc={nc}", flush=True)
+    from jaxqft_amd import fields
+    from oracle.wilson_oracle import WilsonOracle
+
+    L = (4, 4, 4, 8)
+    bc = (1, np.exp(0.4j), 1, -1)
+    rng = fields.rng_for(1850 + nc)
+    U = fields.su3_hot_start(rng, 1, L, 0.5) if nc == 3 else fields.u1_hot_start(rng, 1, L, 0.5)
+    ora = WilsonOracle(L, nc, 0.07, fermion_bc=bc)
+    phi = ora.project_even(fields.random_fermion(rng, (1, *L, 4, nc)))
+    th = theory(L, nc, 0.07, bc, cg_tol=1e-12, cg_maxiter=3000)
+    F = th.eop_pseudofermion_force(U, phi)
+    Xe = th.solve_eop_even_normal(U, phi)
+    Ye = ora.apply_eo_schur_even(U, Xe, normalized=True)
+    res = ora.apply_eo_schur_even_dagger(U, Ye, normalized=True) - phi
+    assert np.linalg.norm(res) <= 2e-12 * np.linalg.norm(phi)
+    assert rel(F, ora.eop_pseudofermion_force_from_solution(U, Xe, Ye)) < 1e-12
+    ths = theory(L, nc, 0.07, bc, cg_tol=1e-12, cg_maxiter=3000, solver_form="eo_split", solver_kind="gmres")
+    assert rel(ths.eop_pseudofermion_force(U, phi), F) < 1e-12
+    thg = theory(L, nc, 0.07, bc, cg_tol=1e-12, cg_maxiter=3000, use_solver_guess=True)
+    assert rel(thg.eop_pseudofermion_force(U, phi), F) < 1e-12
+    it_cold = thg.last_solve_stats[0]["iters"]
+    assert it_cold > 10 and len(thg.eop_solver_guess.xs) == 1 and not thg.solver_guess.xs
+    assert rel(thg.eop_pseudofermion_force(U, phi), F) < 1e-10
+    assert thg.last_solve_stats[0]["iters"] <= 2
+    thg.clear_solver_guess()
+    assert not thg.eop_solver_guess.xs
+
+
 def test_monomial_force_composition():
     print("[gpu] force from phi", flush=True)
     from jaxqft_amd import fields
@@ -161,6 +197,34 @@ def test_force_on_resident_fields(nc, layout):
         rc = ctx._lib.wl_pf_force_field(ctx.h, g.h, a.h, b.h, vp(Ff.data_ptr()), lay, nat.WL_C128, None)
         assert rc == nat.WL_ERR_BAD_ARG
     assert "column" in ctx._lib.wl_last_error().decode()
+
+
+def test_monomial_force_with_guess():
+    """pseudofermion_force with use_solver_guess (su3_wilson_nf2.py:378-392)
+    keeps (X, D X) in its history: with phi = D^dag eta the 'mre' guess fits
+    eta, so a second call starts at the solution; the forces equal the cold
+    one."""
+    print("[gpu] force with guess", flush=True)
+    from jaxqft_amd import fields
+
+    L = (4, 4, 4, 8)
+    bc = (1, 1, 1, -1)
+    rng = fields.rng_for(1860)
+    U = fields.su3_hot_start(rng, 2, L, 0.5)
+    eta = fields.random_fermion(rng, (2, *L, 4, 3))
+    th = theory(L, 3, 0.1, bc, batch=2, cg_tol=1e-12, cg_maxiter=3000)
+    phi = th.apply_Ddag(U, eta)
+    F = th.pseudofermion_force(U, phi)
+    thg = theory(L, 3, 0.1, bc, batch=2, cg_tol=1e-12, cg_maxiter=3000, use_solver_guess=True,
+                 solver_guess_mode="mre", solver_guess_history=3)
+    assert np.array_equal(thg.pseudofermion_force(U, phi, eta), F)
+    cold = [st["iters"] for st in thg.last_solve_stats]
+    X, Y = thg.solver_guess.xs[0], thg.solver_guess.ys[0]
+    assert np.array_equal(Y, thg.apply_D(U, X)) and rel(Y, eta) < 1e-10
+    assert rel(thg.pseudofermion_force(U, phi, eta), F) < 1e-10
+    warm = [st["iters"] for st in thg.last_solve_stats]
+    assert min(cold) > 10 and max(warm) <= 5, (cold, warm)
+    assert len(thg.solver_guess.xs) == 2 and not thg.eop_solver_guess.xs
 
 
 def test_force_is_action_derivative_16x4():
