@@ -858,9 +858,12 @@ def config4(B, args):
     """24^3x48, eps=0.6, m=0.03: k in {1, 4, 12, 24} right-hand sides (k=12:
     D^dag applied to the 12 unit point sources at a random site, as the
     propagator loop of measurements.py:333-345 does; otherwise complex
-    Gaussian), FP32 inner stencil, FP64 reliable updates."""
+    Gaussian), FP32 inner stencil, FP64 reliable updates.  Before each k the
+    autotuner picks the FP32 inner and FP64 update launch shapes for k
+    right-hand sides on this lattice."""
     torch, nat = B.torch, B.nat
     from jaxqft_amd import fields
+    from jaxqft_amd.autotune import tune
 
     L = (24, 24, 24, 48)
     rng = fields.rng_for(fields.SEED_BASE + 3)
@@ -883,10 +886,19 @@ def config4(B, args):
             b = nat.Field(ctx, nat.WL_F64, k)
             b.import_(bt)
             x = nat.Field(ctx, nat.WL_F64, k)
+            tuned = {}
+            for prec in ("f32", "f64"):
+                pc = nat.PREC[prec]
+                fi, fo = nat.Field(ctx, pc, k), nat.Field(ctx, pc, k)
+                fi.import_(bt)
+                tk = tune(ctx, gauge, pc, 12, fi, fo, nrhs=k, reps=3, stream=B.stream)
+                tuned[prec] = {"rhs_per_thread": tk["rhs_per_thread"], "march": tk["march"], "slab": tk["slab"],
+                               "cols_outer": tk["cols_outer"], "block_shape": tk["block_shape"]}
+                del fi, fo
             B.solve(ctx, gauge, b, x, k, "f32", 12)
             t, st = B.timed_solves(1, ctx, gauge, b, x, k, "f32", 12)
             res[str(k)] = {**summarize(st), "sources": kind, "s_per_batch": round(t, 4),
-                           "solves_per_s": round(k / t, 3)}
+                           "solves_per_s": round(k / t, 3), "tuning": tuned}
             log(f"24^3x48 k={k}: {t*1e3:.1f} ms, {k/t:.2f} solves/s {summarize(st)}")
             del b, x, bt
     return res
