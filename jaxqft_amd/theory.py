@@ -242,6 +242,42 @@ class WilsonNf2Theory:
     def resident_gauge(self, U) -> ResidentGauge:
         return ResidentGauge(self, U)
 
+    def tune(self, U, nrhs: int = 1, reps: int = 3) -> Dict[str, Dict[str, object]]:
+        """Pick this lattice's stencil launch shapes with jaxqft_amd.autotune.tune
+        for the solver's inner precision and link encoding and for the FP64
+        stencil of the reliable updates, at nrhs right-hand sides per
+        configuration (timed on Gaussian spinors), and install them in the
+        context: later operator calls and solves use them.  Launches with one
+        right-hand side per configuration and with several are tuned
+        separately (nrhs decides which).  Returns the chosen shape per
+        precision."""
+        from .autotune import tune as _tune
+
+        ctx, g = self._gauge(U)
+        ncol = self.Bs * int(nrhs)
+        if ncol < 1:
+            raise ValueError("nrhs must be positive")
+        gen = torch.Generator(device="cuda").manual_seed(int(self.seed))
+        shape = (ncol, *self.lattice_shape, self.Ns, self.Nc)
+        src = torch.complex(torch.randn(shape, dtype=torch.float64, device="cuda", generator=gen),
+                            torch.randn(shape, dtype=torch.float64, device="cuda", generator=gen))
+        inner = nat.PREC[self.solver_precision]
+        plan = [(inner, int(self.solver_recon))]
+        if inner != nat.WL_F64:
+            plan.append((nat.WL_F64, 12 if self.Nc == 3 else 18))
+        chosen = {}
+        for pc, recon in plan:
+            fi, fo = nat.Field(ctx, pc, ncol), nat.Field(ctx, pc, ncol)
+            fi.import_(src)
+            t = _tune(ctx, g, pc, recon, fi, fo, nrhs=int(nrhs), reps=reps)
+            chosen[self._prec_name(pc)] = {k: v for k, v in t.items() if k not in ("ranked", "us")}
+        torch.cuda.synchronize()
+        return chosen
+
+    @staticmethod
+    def _prec_name(pc: int) -> str:
+        return {v: k for k, v in nat.PREC.items()}[pc]
+
     def _apply(self, U, psi, op: int):
         self._check_fermion(psi)
         ctx, g = self._gauge(U)

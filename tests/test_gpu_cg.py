@@ -428,6 +428,49 @@ def test_cg_with_marching_stencil(prec, march, slab, box):
     assert true_resid(L, 3, 0.05, (1, 1, 1, -1), U, b, x) <= 1e-10
 
 
+@pytest.mark.parametrize("prec,nc", [("f16", 3), ("f32", 1)])
+def test_theory_tune_then_solve(prec, nc):
+    """th.tune installs launch shapes for the inner precision and the FP64
+    update stencil; the operator after it agrees with the one before to
+    rounding and the solves, one and two configurations, meet the FP64 true
+    residual; a 4-RHS tuning leaves the one-RHS shapes alone and the 4-RHS
+    solve meets the true residual too."""
+    print(f"[gpu] theory tune {prec} nc={nc}", flush=True)
+    from jaxqft_amd import _native as nat
+    from jaxqft_amd import fields
+    from jaxqft_amd.su3_wilson_nf2 import SU3WilsonNf2
+    from jaxqft_amd.u1_wilson_nf2 import U1WilsonNf2
+
+    L = (8, 8, 4, 8)
+    bc = (1, 1, 1, -1)
+    rng = fields.rng_for(1090 + nc)
+    cls = SU3WilsonNf2 if nc == 3 else U1WilsonNf2
+    U = fields.su3_hot_start(rng, 2, L, 0.5) if nc == 3 else fields.u1_hot_start(rng, 2, L, 0.5)
+    b = fields.random_fermion(rng, (2, *L, 4, nc))
+    for bs in (1, 2):
+        th = cls(lattice_shape=L, mass=0.05, fermion_bc=bc, batch_size=bs, dtype=np.complex128, cg_tol=1e-10,
+                 cg_maxiter=4000, solver_precision=prec, solver_recon=12 if nc == 3 else 18)
+        D0 = th.apply_D(U[:bs], b[:bs])
+        chosen = th.tune(U[:bs], nrhs=1)
+        assert set(chosen) == {prec, "f64"}
+        for name, t in chosen.items():
+            got = th._context().get_tuning(nat.PREC[name], 1)
+            assert all(got[k] == t[k] for k in ("march", "slab", "cols_outer", "lds_box")), (name, got, t)
+        D1 = th.apply_D(U[:bs], b[:bs])
+        assert np.linalg.norm(D1 - D0) <= 1e-14 * np.linalg.norm(D0)
+        x = th.solve_normal(U[:bs], b[:bs])
+        for j in range(bs):
+            assert th.last_solve_stats[j]["status"] == "converged"
+            assert true_resid(L, nc, 0.05, bc, U[j:j + 1], b[j:j + 1], x[j:j + 1]) <= 1e-10
+    one = {p: th._context().get_tuning(nat.PREC[p], 1) for p in chosen}
+    th.tune(U[:bs], nrhs=4)
+    assert {p: th._context().get_tuning(nat.PREC[p], 1) for p in chosen} == one
+    rhs = fields.random_fermion(rng, (bs, 4, *L, 4, nc))
+    X = th.solve_normal_multi(U[:bs], rhs)
+    for j in range(4):
+        assert true_resid(L, nc, 0.05, bc, U[:bs], rhs[:, j], X[:, j]) <= 1e-10
+
+
 @pytest.mark.parametrize("delta", [1e-4, 1e-3])
 def test_small_delta_still_converges(delta):
     """A reliable-update trigger below what FP16 storage can follow is raised
