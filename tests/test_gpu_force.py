@@ -227,6 +227,32 @@ def test_monomial_force_with_guess():
     assert len(thg.solver_guess.xs) == 2 and not thg.eop_solver_guess.xs
 
 
+def test_resident_force_graph_and_complex64():
+    """The resident-field force with HIP-graph iteration blocks (the solve
+    then runs on a side stream when the caller is on the default stream)
+    equals the force without graphs bit for bit; complex64 phi gives the
+    complex128 force rounded once (X and Y stay FP64 inside), to within the
+    complex64 rounding of phi itself."""
+    print("[gpu] resident force graph / complex64", flush=True)
+    import torch
+
+    from jaxqft_amd import fields
+
+    L = (4, 4, 4, 8)
+    bc = (1, 1, 1, -1)
+    rng = fields.rng_for(1870)
+    U = torch.from_numpy(fields.su3_hot_start(rng, 2, L, 0.5)).cuda()
+    phi = torch.from_numpy(fields.random_fermion(rng, (2, *L, 4, 3))).cuda()
+    kw = dict(batch=2, cg_tol=1e-12, cg_maxiter=3000, solver_precision="f16", solver_recon=12)
+    F = theory(L, 3, 0.1, bc, **kw).pseudofermion_force(U, phi)
+    Fg = theory(L, 3, 0.1, bc, use_graph=True, **kw).pseudofermion_force(U, phi)
+    assert torch.equal(F, Fg)
+    F64 = theory(L, 3, 0.1, bc, **kw).pseudofermion_force(U, phi.to(torch.complex64))
+    assert F64.dtype == torch.complex64
+    ref = theory(L, 3, 0.1, bc, **kw).pseudofermion_force(U, phi.to(torch.complex64).to(torch.complex128))
+    assert torch.equal(F64, ref.to(torch.complex64))
+
+
 def test_force_is_action_derivative_16x4():
     """At 16^4, with the action S = phi^dag (D^dag D)^-1 phi evaluated by the GPU
     solver (FP64, tol 1e-13), (S(eps) - S(-eps)) / 2 eps for U -> exp(eps H) U
