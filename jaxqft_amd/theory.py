@@ -625,8 +625,9 @@ class WilsonNf2Theory:
         sp, keep = self._solve_params()
         stats = (nat.WlSolveStats * self.Bs)()
         dt = nat.WL_C128 if pt.dtype == torch.complex128 else nat.WL_C64
+        # outputs are allocated on the caller's stream, which owns them afterwards
         F = torch.empty(tuple(self.field_shape()), dtype=pt.dtype, device=pt.device)
-        X = Y = None
+        xs, ys = (torch.empty_like(pt), torch.empty_like(pt)) if keep_xy else (None, None)
         lib = ctx._lib
 
         def run():
@@ -640,25 +641,22 @@ class WilsonNf2Theory:
             nat.check(lib.wl_pf_force_field(ctx.h, g.h, fx.h, fy.h, nat.c_void_p(F.data_ptr()), self._layout_code(),
                                             dt, h))
             if keep_xy:
-                xs, ys = torch.empty_like(pt), torch.empty_like(pt)
                 fx.export(xs)
                 fy.export(ys)
-                return xs, ys
-            return None, None
 
         if sp.use_graph and nat.current_stream() == 0:
             # graph capture needs a stream other than the default one
             side = torch.cuda.Stream()
             side.wait_stream(torch.cuda.current_stream())
             with torch.cuda.stream(side):
-                X, Y = run()
+                run()
             torch.cuda.current_stream().wait_stream(side)
         else:
-            X, Y = run()
+            run()
         del keep
         self.last_solve_stats = nat.stats_list(stats, self.Bs)
-        if keep_xy:
-            X, Y = from_device(X, kind, orig), from_device(Y, kind, orig)
+        X = from_device(xs, kind, orig) if keep_xy else None
+        Y = from_device(ys, kind, orig) if keep_xy else None
         return from_device(F, kind, orig), X, Y
 
     def _m0_value(self) -> float:
