@@ -84,6 +84,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lime", action="store_true", help="skip the SciDAC payload decode")
+    ap.add_argument("--no-published", action="store_true",
+                    help="skip the 8^3x16 workloads of the reference's published CPU timings")
     ap.add_argument("--profile-only", action="store_true",
                     help="run only the Dslash variants and a calibration copy (for rocprofv3 counter passes)")
     ap.add_argument("--solve-trace", action="store_true",
@@ -535,6 +537,8 @@ def main():
         detail.update(propagator_chrono(B, args, U, L))
     if not args.no_serving:
         detail["serving"] = serving(B, args)
+    if not args.no_published:
+        detail["published_workloads_8x8x8x16"] = published_workloads(B, args)
     if not args.no_lime:
         detail["lime_decode"] = lime_decode(B, args, L, not args.no_cpu)
     if not args.no_cpu:
@@ -689,6 +693,10 @@ def compact_record(args, d, L, k):
     c4 = d.get("multi_rhs_24x24x24x48")
     if c4:
         line["config4_24^3x48"] = {kk: v["solves_per_s"] for kk, v in c4.items()}
+    pw = d.get("published_workloads_8x8x8x16")
+    if pw:
+        line["published_cpu_workloads_8^3x16"] = {k: {"ms": v["ms"], "reference_cpu_ms": v["reference_cpu_ms"]}
+                                                  for k, v in pw["cases"].items()}
     srv = d.get("serving")
     if srv:
         cb = srv["continuous_batching"]
@@ -902,6 +910,72 @@ def config4(B, args):
             log(f"24^3x48 k={k}: {t*1e3:.1f} ms, {k/t:.2f} solves/s {summarize(st)}")
             del b, x, bt
     return res
+
+
+# The reference's published timings (HANDOFF.md:2161-2179: CPU, 20 threads, JAX
+# jit, complex64; its `--tests perf` / `eoperf` / `eopforce` runs at 8^3x16,
+# mass 0.05, weak field hot_start(0.05), su3_wilson_nf2.py:1719-1721,
+# :2602-2650), in ms per call.
+PUBLISHED_CPU_MS = {"D": 0.85, "DdagD": 1.64, "cg_243_iters_tol0": 1450.0, "eo_cg_86_iters_tol0": 400.0,
+                    "eo_analytic_force_tol1e-7": 68.2}
+
+
+def published_workloads(B, args):
+    """The workloads behind the reference's published CPU timings, run through
+    the drop-in (SU3WilsonNf2, complex64 tensors resident on the device, as the
+    reference's default dtype): D and D^dag D per call, CG on D^dag D for a
+    fixed 243 iterations and even-odd Schur CG for a fixed 86 (tol 0, FP32
+    inner with FP64 reliable updates, the arithmetic of a complex64 solve kept
+    honest by the FP64 updates), and the even-odd monomial's analytic force
+    with CG tol 1e-7.  Each figure is the median wall time per call over
+    repeated calls, host overhead included, beside the published CPU figure."""
+    torch = B.torch
+    from jaxqft_amd import fields
+    from jaxqft_amd.su3_wilson_nf2 import SU3WilsonNf2
+
+    L = (8, 8, 8, 16)
+    rng = fields.rng_for(fields.SEED_BASE + 9)
+    U = torch.from_numpy(fields.su3_hot_start(rng, 1, L, 0.05).astype(np.complex64)).cuda()
+    psi = torch.from_numpy(fields.random_fermion(rng, (1, *L, 4, 3)).astype(np.complex64)).cuda()
+    base = dict(lattice_shape=L, mass=0.05, fermion_bc=(1, 1, 1, -1), dtype=np.complex64, solver_precision="f32",
+                solver_recon=12, use_graph=bool(args.graph), poll_every=args.poll)
+
+    def med(fn, n):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(n):
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)) * 1e3
+
+    cases = {}
+    with torch.cuda.stream(B.stream):
+        th = SU3WilsonNf2(cg_tol=0.0, cg_maxiter=243, **base)
+        G = th.resident_gauge(U)
+        cases["D"] = {"ms": med(lambda: th.apply_D(G, psi), 50)}
+        cases["DdagD"] = {"ms": med(lambda: th.apply_normal(G, psi), 50)}
+        cases["cg_243_iters_tol0"] = {"ms": med(lambda: th.solve_normal(G, psi), 5),
+                                      "iters": th.last_solve_stats[0]["iters"]}
+        the = SU3WilsonNf2(cg_tol=0.0, cg_maxiter=86, **base)
+        Ge = the.resident_gauge(U)
+        rhs_e = the.project_even(psi)
+        cases["eo_cg_86_iters_tol0"] = {"ms": med(lambda: the.solve_eop_even_normal(Ge, rhs_e), 5),
+                                        "iters": the.last_solve_stats[0]["iters"]}
+        thf = SU3WilsonNf2(cg_tol=1e-7, cg_maxiter=1000, **base)
+        Gf = thf.resident_gauge(U)
+        phi_e = thf.apply_eo_schur_even_dagger(Gf, rhs_e, normalized=True)
+        cases["eo_analytic_force_tol1e-7"] = {"ms": med(lambda: thf.eop_pseudofermion_force(Gf, phi_e), 5),
+                                              "iters": thf.last_solve_stats[0]["iters"]}
+    for k, v in cases.items():
+        v["ms"] = round(v["ms"], 4)
+        v["reference_cpu_ms"] = PUBLISHED_CPU_MS[k]
+        v["ratio"] = round(PUBLISHED_CPU_MS[k] / v["ms"], 1)
+        log(f"8^3x16 {k}: {v['ms']:.4f} ms per call, published CPU {PUBLISHED_CPU_MS[k]} ms ({v['ratio']}x)")
+    return {"lattice": list(L), "links": "hot_start(0.05), complex64", "mass": 0.05, "cases": cases,
+            "source": "the reference's published CPU timings (20 threads, JAX jit, complex64) for these workloads"}
 
 
 def serving(B, args):
