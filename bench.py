@@ -624,7 +624,7 @@ def cg_step_roofline(args, V, k, prof, stats):
 
 
 def compact_record(args, d, L, k):
-    """The driver-facing record: the headline, the roofline of the launch
+    """The one-line stdout record: the headline, the roofline of the launch
     that dominates the timed step (D^dag with the fused CG step), the
     standalone Dslash launch beside it, the streaming ceiling, the CPU
     baseline and one summary figure per other configuration."""
