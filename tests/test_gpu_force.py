@@ -120,6 +120,32 @@ def test_eo_monomial_force(nc):
     assert not thg.eop_solver_guess.xs
 
 
+def test_eo_monomial_force_batched_torch_resident():
+    """The even-odd monomial force on a batch of two configurations given as
+    torch tensors on the device, with the links resident (resident_gauge) and
+    the launch shapes tuned (th.tune): each configuration's force equals the
+    force computed for it alone from numpy inputs."""
+    print("[gpu] eo monomial force batched torch", flush=True)
+    import torch
+
+    from jaxqft_amd import fields
+
+    L = (4, 4, 4, 8)
+    bc = (1, 1, 1, -1)
+    rng = fields.rng_for(1880)
+    U = fields.su3_hot_start(rng, 2, L, 0.5)
+    kw = dict(cg_tol=1e-12, cg_maxiter=3000, solver_precision="f32", solver_recon=12)
+    th2 = theory(L, 3, 0.08, bc, batch=2, **kw)
+    phi = th2.project_even(fields.random_fermion(rng, (2, *L, 4, 3)))
+    G = th2.resident_gauge(torch.from_numpy(U).cuda())
+    th2.tune(G, nrhs=1)
+    F2 = th2.eop_pseudofermion_force(G, torch.from_numpy(phi).cuda())
+    assert isinstance(F2, torch.Tensor) and F2.is_cuda
+    for j in range(2):
+        F1 = theory(L, 3, 0.08, bc, **kw).eop_pseudofermion_force(U[j:j + 1], phi[j:j + 1])
+        assert rel(F2[j:j + 1].cpu().numpy(), F1) < 1e-9, j
+
+
 def test_monomial_force_composition():
     print("[gpu] force from phi", flush=True)
     from jaxqft_amd import fields
