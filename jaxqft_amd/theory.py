@@ -198,6 +198,7 @@ class WilsonNf2Theory:
         self._rng = _fields.rng_for(self.seed)
         self._ctx: Optional[nat.Context] = None
         self._force_fields = None  # (ctx, b, x, y): resident FP64 fields of the force step
+        self._parity_masks: Dict[Any, Any] = {}  # even-site masks: NumPy, and per torch device
 
     # ------------------------------------------------------------------ shapes
     def fermion_shape(self) -> Tuple[int, ...]:
@@ -383,15 +384,26 @@ class WilsonNf2Theory:
         self._check_fermion(psi_even, "psi_even")
         return self._eo(U, psi_even, nat.WL_EO_SCHUR_E_DAG if daggered else nat.WL_EO_SCHUR_E, normalized)
 
+    def _parity_mask(self, parity: int, device=None):
+        """Boolean mask of the sites of one parity, shaped (1, *L, 1, 1); built
+        once per lattice (on the host, then once per torch device)."""
+        key = (parity, None if device is None else str(device))
+        m = self._parity_masks.get(key)
+        if m is None:
+            if device is None:
+                even = np.indices(self.lattice_shape).sum(axis=0) % 2 == 0
+                m = (even if parity == 0 else ~even).reshape((1, *self.lattice_shape, 1, 1))
+            else:
+                m = torch.as_tensor(self._parity_mask(parity), device=device)
+            self._parity_masks[key] = m
+        return m
+
     def _parity_project(self, psi, parity: int):
         """psi on the sites of one parity (sum of coordinates mod 2), zero on the
         others (su3_wilson_nf2.py:884-892)."""
-        even = np.indices(self.lattice_shape).sum(axis=0) % 2 == 0
-        keep = even if parity == 0 else ~even
-        mask = keep.reshape((1, *self.lattice_shape, 1, 1))
         if isinstance(psi, torch.Tensor):
-            return psi * torch.as_tensor(mask, device=psi.device).to(psi.dtype)
-        return np.asarray(psi) * mask.astype(np.asarray(psi).real.dtype)
+            return psi * self._parity_mask(parity, psi.device).to(psi.dtype)
+        return np.asarray(psi) * self._parity_mask(parity).astype(np.asarray(psi).real.dtype)
 
     def project_even(self, psi):
         return self._parity_project(psi, 0)
