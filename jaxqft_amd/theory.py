@@ -31,7 +31,11 @@ from . import fields as _fields
 
 
 def _vdot_batch(a, b):
-    """Re <a_b, b_b> for each batch element b (leading axis), as a NumPy array."""
+    """Re <a_b, b_b> for each batch element b (leading axis), as a NumPy array.
+    Two tensors on one GPU are reduced there: only the B results cross to the
+    host."""
+    if isinstance(a, torch.Tensor) and isinstance(b, torch.Tensor) and a.is_cuda and b.device == a.device:
+        return torch.sum(torch.real(torch.conj(a) * b), dim=tuple(range(1, a.ndim))).cpu().numpy()
     aa = np.asarray(a.cpu() if isinstance(a, torch.Tensor) else a)
     bb = np.asarray(b.cpu() if isinstance(b, torch.Tensor) else b)
     return np.real(np.sum(np.conj(aa) * bb, axis=tuple(range(1, aa.ndim))))

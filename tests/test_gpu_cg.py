@@ -99,6 +99,15 @@ def test_pseudofermion_identity_on_gpu():
     assert abs(S[0] - np.vdot(eta, eta).real) / np.vdot(eta, eta).real < 1e-10
     st = th.normal_residual_stats(U, phi, th.solve_normal(U, phi))
     assert st["max_rel_residual"] <= 1e-12
+    # device tensors: the action is reduced on the GPU and agrees with the host sum
+    import torch
+
+    St = th.pseudofermion_action(torch.from_numpy(U).cuda(), torch.from_numpy(phi).cuda())
+    assert isinstance(St, np.ndarray) and St.shape == (1,)
+    assert abs(St[0] - S[0]) <= 1e-12 * abs(S[0])
+    Se = th.eop_pseudofermion_action(U, phi)
+    Set = th.eop_pseudofermion_action(torch.from_numpy(U).cuda(), torch.from_numpy(phi).cuda())
+    assert abs(Set[0] - Se[0]) <= 1e-12 * abs(Se[0])
 
 
 def test_status_maxiter_and_nan():
