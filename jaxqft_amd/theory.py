@@ -415,6 +415,11 @@ class WilsonNf2Theory:
     def project_odd(self, psi):
         return self._parity_project(psi, 1)
 
+    # the reference's even-odd monomial calls the projections under these
+    # names (su3_wilson_nf2.py:490-505, :884-892)
+    _project_even = project_even
+    _project_odd = project_odd
+
     def _solve_eo(self, U, rhs, x0, nrhs: int = 1, masses=None, **overrides):
         ctx, g = self._gauge(U)
         pt, kind, orig = to_device(rhs)

@@ -160,6 +160,24 @@ def test_theory_validation_and_shapes():
     assert th.dirac.flops_per_site_dslash(nc=3) == 3552
 
 
+def test_parity_projections_under_the_reference_names():
+    """_project_even / _project_odd (the names the reference's even-odd monomial
+    calls) are the projections: complementary, idempotent, and zero on the
+    sites of the other parity (sum of coordinates mod 2)."""
+    from jaxqft_amd.su3_wilson_nf2 import SU3WilsonNf2
+
+    L = (2, 3, 4, 2)
+    th = SU3WilsonNf2(lattice_shape=L)
+    psi = fields.random_fermion(fields.rng_for(7), (1, *L, 4, 3))
+    e, o = th._project_even(psi), th._project_odd(psi)
+    np.testing.assert_array_equal(e + o, psi)
+    np.testing.assert_array_equal(th._project_even(e), e)
+    for x in np.ndindex(*L):
+        zero = o if sum(x) % 2 == 0 else e
+        assert not np.any(zero[(0, *x)])
+    np.testing.assert_array_equal(th.project_even(psi), e)
+
+
 def test_su3_field_generator():
     rng = fields.rng_for(5)
     U = fields.su3_hot_start(rng, 1, (4, 4, 4, 4), 0.5).reshape(-1, 3, 3)
