@@ -247,6 +247,40 @@ class WilsonNf2Theory:
     def resident_gauge(self, U) -> ResidentGauge:
         return ResidentGauge(self, U)
 
+    def _links_with_fermion_bc(self, U):
+        """U with the fermion boundary phase on the links U_mu(x) of the slice
+        x_mu = L_mu - 1 (su3_wilson_nf2.py:751-759), for callers that build on
+        the phased links (the conserved current, core/measurements.py:1620);
+        the kernels apply the same phase on the hop that crosses the boundary.
+        One configuration or a batch, NumPy or torch, in the theory's layout;
+        U itself when every phase is 1."""
+        if all(abs(complex(p) - 1.0) <= 1e-12 for p in self.fermion_bc):
+            return U
+        fac = self._parity_masks.get("bc_factor")
+        if fac is None:
+            nmat = len(self.field_shape()) - self.Nd - 2  # trailing colour axes: 2 for SU(3), 0 for U(1)
+            mu_first = self.layout == "BMXYIJ"
+            shape = ((self.Nd, *self.lattice_shape) if mu_first else (*self.lattice_shape, self.Nd)) + (1,) * nmat
+            fac = np.ones(shape, dtype=np.complex128)
+            for mu, p in enumerate(self.fermion_bc):
+                if abs(complex(p) - 1.0) <= 1e-12:
+                    continue
+                sl = [slice(None)] * len(shape)
+                sl[0 if mu_first else self.Nd] = mu
+                sl[(1 + mu) if mu_first else mu] = int(self.lattice_shape[mu]) - 1
+                fac[tuple(sl)] *= complex(p)
+            self._parity_masks["bc_factor"] = fac
+        if U.ndim == fac.ndim + 1:
+            f = fac[None]
+        elif U.ndim == fac.ndim:
+            f = fac
+        else:
+            raise ValueError(f"Unexpected link tensor rank for fermion BC application: ndim={U.ndim}")
+        if isinstance(U, torch.Tensor):
+            return U * torch.as_tensor(f, device=U.device).to(U.dtype)
+        U = np.asarray(U)
+        return U * f.astype(U.dtype)
+
     def tune(self, U, nrhs: int = 1, reps: int = 3) -> Dict[str, Dict[str, object]]:
         """Pick this lattice's stencil launch shapes with jaxqft_amd.autotune.tune
         for the solver's inner precision and link encoding and for the FP64

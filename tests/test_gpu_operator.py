@@ -590,3 +590,20 @@ def test_foreign_and_mismatched_handles_rejected():
     torch.cuda.synchronize()
     assert torch.equal(out, ref)
     del keep
+
+
+def test_links_with_fermion_bc_on_device():
+    """_links_with_fermion_bc of a device tensor equals the NumPy result, and
+    stays on the device."""
+    print("[gpu] links with fermion bc (torch)", flush=True)
+    import torch
+
+    from jaxqft_amd import fields
+    from jaxqft_amd.su3_wilson_nf2 import SU3WilsonNf2
+
+    L = (4, 4, 4, 8)
+    th = SU3WilsonNf2(lattice_shape=L, fermion_bc=(np.exp(0.3j), 1, 1, -1), batch_size=2)
+    U = fields.su3_hot_start(fields.rng_for(1890), 2, L, 0.5)
+    got = th._links_with_fermion_bc(torch.from_numpy(U).cuda())
+    assert got.is_cuda
+    np.testing.assert_allclose(got.cpu().numpy(), th._links_with_fermion_bc(U), rtol=1e-15, atol=0)

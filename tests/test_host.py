@@ -178,6 +178,37 @@ def test_parity_projections_under_the_reference_names():
     np.testing.assert_array_equal(th.project_even(psi), e)
 
 
+@pytest.mark.parametrize("nc,layout", [(3, "BMXYIJ"), (3, "BXYMIJ"), (1, "BMXYIJ"), (1, "BXYMIJ")])
+def test_links_with_fermion_bc(nc, layout):
+    """_links_with_fermion_bc multiplies U_mu(x) on the slice x_mu = L_mu - 1 by
+    the direction's phase and leaves every other link alone (checked link by
+    link), batched and for one configuration; periodic phases return U itself;
+    a link array of another rank is refused with the reference's message."""
+    from jaxqft_amd.su3_wilson_nf2 import SU3WilsonNf2
+    from jaxqft_amd.u1_wilson_nf2 import U1WilsonNf2
+
+    L = (2, 3, 2, 4)
+    bc = (np.exp(0.4j), 1, -1, np.exp(-1.1j))
+    cls = SU3WilsonNf2 if nc == 3 else U1WilsonNf2
+    th = cls(lattice_shape=L, fermion_bc=bc, batch_size=2, layout=layout)
+    rng = fields.rng_for(11)
+    U = fields.su3_hot_start(rng, 2, L, 0.5, layout=layout) if nc == 3 else fields.u1_hot_start(rng, 2, L, 0.5,
+                                                                                                layout=layout)
+    got = th._links_with_fermion_bc(U)
+    for b in range(2):
+        for mu in range(4):
+            for x in np.ndindex(*L):
+                idx = (b, mu, *x) if layout == "BMXYIJ" else (b, *x, mu)
+                want = U[idx] * (th.fermion_bc[mu] if x[mu] == L[mu] - 1 else 1.0)
+                # one rounding of a complex product (vectorised and scalar products may differ by an ulp)
+                np.testing.assert_allclose(got[idx], want, rtol=1e-15, atol=0)
+    np.testing.assert_array_equal(th._links_with_fermion_bc(U[0]), got[0])
+    per = cls(lattice_shape=L, fermion_bc=(1, 1, 1, 1), batch_size=2, layout=layout)
+    assert per._links_with_fermion_bc(U) is U
+    with pytest.raises(ValueError, match="Unexpected link tensor rank"):
+        th._links_with_fermion_bc(U[0, 0])
+
+
 def test_su3_field_generator():
     rng = fields.rng_for(5)
     U = fields.su3_hot_start(rng, 1, (4, 4, 4, 4), 0.5).reshape(-1, 3, 3)
