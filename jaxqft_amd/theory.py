@@ -202,7 +202,9 @@ class WilsonNf2Theory:
         self._rng = _fields.rng_for(self.seed)
         self._ctx: Optional[nat.Context] = None
         self._force_fields = None  # (ctx, b, x, y): resident FP64 fields of the force step
-        self._parity_masks: Dict[Any, Any] = {}  # even-site masks: NumPy, and per torch device
+        # per-lattice constants built on first use: parity masks (NumPy, and per
+        # torch device) and the boundary-phase factor of the links
+        self._parity_masks: Dict[Any, Any] = {}
 
     # ------------------------------------------------------------------ shapes
     def fermion_shape(self) -> Tuple[int, ...]:
