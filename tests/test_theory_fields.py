@@ -105,3 +105,26 @@ def test_solver_guess_mre_fits_target_and_falls_back():
     # without the target the history extrapolates as 'poly'
     poly = g.predict(xs[0])
     assert np.allclose(poly, 3 * xs[2] - 3 * xs[1] + xs[0])
+
+
+@pytest.mark.parametrize("cls_name", ["SU3WilsonNf2", "U1WilsonNf2"])
+def test_model_surface_used_by_the_reference_callers(cls_name):
+    """Every model attribute that the reference's pseudofermion monomials
+    (su3_wilson_nf2.py:218-590, u1_wilson_nf2.py:96-293) and measurements
+    (core/measurements.py:277-371, :1600-1625) read on the fermion path exists
+    on the drop-in, private names included."""
+    import importlib
+
+    mod = importlib.import_module("jaxqft_amd." + ("su3_wilson_nf2" if cls_name == "SU3WilsonNf2" else "u1_wilson_nf2"))
+    th = getattr(mod, cls_name)(lattice_shape=(4, 4, 4, 4))
+    used = ["solver_form", "random_fermion", "dtype", "_project_even", "_project_odd", "solve_normal",
+            "solve_normal_with_guess", "solve_eop_even_normal", "solve_eop_even_normal_with_guess",
+            "apply_eo_schur_even", "apply_eo_schur_even_dagger", "apply_D", "apply_Ddag", "solve_dagger",
+            "solve_dagger_eo", "solve_direct", "solve_split_with_intermediate", "solve_eop_even_dagger",
+            "solve_eop_even_split_with_intermediate", "pseudofermion_action", "eop_pseudofermion_action",
+            "pseudofermion_force_from_solution", "eop_pseudofermion_force_from_solution", "fermion_shape",
+            "gamma", "wilson_r", "dirac", "layout", "lattice_shape", "_links_with_fermion_bc",
+            "use_solver_guess", "solver_guess_mode", "solver_guess_history"]
+    missing = [n for n in used if not hasattr(th, n)]
+    assert not missing, missing
+    assert hasattr(th.dirac, "spin_eye")
