@@ -301,7 +301,8 @@ int wl_pf_force(wl_ctx* ctx, wl_gauge* g, const void* X, const void* Y, void* F,
  * configuration, e.g. X from wl_cg_normal_field and Y = D X from wl_dslash),
  * so that a monomial force step (su3_wilson_nf2.py:378-392) keeps its
  * solution on the device in the internal layout; F as for wl_pf_force.
- * Results are identical to wl_pf_force on the exported fields. */
+ * X and Y are read in FP64 whatever dtype F has: the result equals
+ * wl_pf_force on the fields exported as complex128, rounded once to F's dtype. */
 int wl_pf_force_field(wl_ctx* ctx, wl_gauge* g, const wl_field* X, const wl_field* Y, void* F, int layout, int dtype,
                       void* stream);
 /* Solve S_e^dag S_e x_e = P_e b with the normalized even Schur complement
